@@ -1,0 +1,253 @@
+"""Python surface of the pdhg engine slot (see package docstring)."""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _core
+from ._core import EngineConfig, Options, SolveInfo
+
+_ENGINES = ("auto", "simplex", "pdhg", "milp", "qp")
+
+
+class Status(enum.IntEnum):
+    """Same enumerators as igaos::Status (include/igaos/status.h:5-14)."""
+    Optimal = 0
+    NearOptimal = 1
+    Feasible = 2
+    Infeasible = 3
+    Unbounded = 4
+    IterationLimit = 5
+    TimeLimit = 6
+    Error = 7
+
+
+STATUS_NAMES = ["optimal", "near-optimal", "feasible", "infeasible",
+                "unbounded", "iteration-limit", "time-limit", "error"]
+
+
+@dataclass
+class Solution:
+    """Fields of igaos::Solution (include/igaos/solution.h:10-22).
+
+    ``status`` is the status string (python/igaos_bindings.cpp:107-109),
+    ``status_enum`` the enum; ``y`` follows the pdhg convention
+    d = c + A'y.  ``info`` carries engine diagnostics.
+    """
+    status: str = "error"
+    status_enum: Status = Status.Error
+    objective: float = 0.0
+    x: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    y: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    row_activity: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    pinf: float = 0.0
+    dinf: float = 0.0
+    rel_gap: float = 0.0
+    iterations: int = 0
+    solve_time_ms: float = 0.0
+    message: str = ""
+    info: dict = field(default_factory=dict)
+
+    def __repr__(self):
+        return (f"<igaos.Solution status='{self.status}' "
+                f"objective={self.objective:.6f}>")
+
+
+class Model:
+    """An LP held by the C library (the reference's io::Model)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError(_core.last_error() or "model construction failed")
+        self._h = C.c_void_p(handle)
+        m, n, nnz = C.c_int(), C.c_int(), C.c_int()
+        oc = C.c_double()
+        fr, rg, ni = C.c_int(), C.c_int(), C.c_int()
+        _core.lib().igaos_model_dims(self._h, C.byref(m), C.byref(n),
+                                     C.byref(nnz), C.byref(oc), C.byref(fr),
+                                     C.byref(rg), C.byref(ni))
+        self.m, self.n, self._nnz = m.value, n.value, nnz.value
+        self.obj_const = oc.value
+        self.n_fr_parsed, self.n_ranges_parsed = fr.value, rg.value
+        self.n_int = ni.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _core._lib is not None:
+            _core._lib.igaos_model_free(h)
+            self._h = None
+
+    def nnz(self):
+        return self._nnz
+
+    def arrays(self) -> dict:
+        """Copy of the model's arrays (rmin, rmax, cl, cu, c, CSR, CSC)."""
+        m, n, z = self.m, self.n, self._nnz
+        out = dict(rmin=np.zeros(m), rmax=np.zeros(m), cl=np.zeros(n),
+                   cu=np.zeros(n), c=np.zeros(n),
+                   ap=np.zeros(m + 1, np.int32), ai=np.zeros(z, np.int32),
+                   ax=np.zeros(z), cp=np.zeros(n + 1, np.int32),
+                   ci=np.zeros(z, np.int32), acx=np.zeros(z))
+        ptr = {k: (v.ctypes.data_as(C.POINTER(C.c_int)) if v.dtype == np.int32
+                   else v.ctypes.data_as(C.POINTER(C.c_double)))
+               for k, v in out.items()}
+        _core.lib().igaos_model_get(self._h, ptr["rmin"], ptr["rmax"],
+                                    ptr["cl"], ptr["cu"], ptr["c"], ptr["ap"],
+                                    ptr["ai"], ptr["ax"], ptr["cp"],
+                                    ptr["ci"], ptr["acx"])
+        return out
+
+    def to_lp(self, name=""):
+        from .lpgen import LP
+        a = self.arrays()
+        return LP(m=self.m, n=self.n, obj_const=self.obj_const, name=name, **a)
+
+    @property
+    def stats(self) -> dict:
+        """Row/column class counts (python/igaos_bindings.cpp:51-88)."""
+        a = self.arrays()
+        e = l = g = ranged = 0
+        for lo, hi in zip(a["rmin"], a["rmax"]):
+            if lo == hi:
+                e += 1
+                continue
+            fl, fu = math.isfinite(lo), math.isfinite(hi)
+            g += fl
+            l += fu
+            ranged += fl and fu
+        nfree = nfixed = nboxed = nones = 0
+        for lo, hi in zip(a["cl"], a["cu"]):
+            fl, fu = math.isfinite(lo), math.isfinite(hi)
+            if not fl and not fu:
+                nfree += 1
+            elif fl and fu and lo == hi:
+                nfixed += 1
+            elif fl and fu:
+                nboxed += 1
+            else:
+                nones += 1
+        return dict(rows_e=e, rows_l=max(l - ranged, 0), rows_g=g,
+                    rows_ranged=ranged, cols_free=nfree, cols_fixed=nfixed,
+                    cols_boxed=nboxed, cols_one_sided=nones, n_int=self.n_int,
+                    n_fr_parsed=self.n_fr_parsed,
+                    n_ranges_parsed=self.n_ranges_parsed,
+                    obj_const=self.obj_const)
+
+    def __repr__(self):
+        return f"<igaos.Model m={self.m} n={self.n} nnz={self._nnz}>"
+
+
+def _has_pdhg() -> bool:
+    try:
+        return _core.device_count() > 0
+    except ImportError:
+        return False
+
+
+def read_mps(path: str) -> Model:
+    """Parse a free-format MPS file (src/io/model.hpp:69-218)."""
+    h = _core.lib().igaos_model_read_mps(os.fsencode(path))
+    if not h:
+        raise RuntimeError(_core.last_error())
+    return Model(h)
+
+
+def model_from_lp(lp) -> Model:
+    """Wrap an ``igaos_amd.lpgen.LP`` (CSR + CSC arrays) as a Model."""
+    d = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    i = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+    arrs = [d(lp.c), d(lp.cl), d(lp.cu), d(lp.rmin), d(lp.rmax), i(lp.ap),
+            i(lp.ai), d(lp.ax), i(lp.cp), i(lp.ci), d(lp.acx)]
+    ptrs = [a.ctypes.data_as(C.POINTER(C.c_int)) if a.dtype == np.int32
+            else a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
+    h = _core.lib().igaos_model_from_csr_csc(lp.m, lp.n, *ptrs,
+                                             float(lp.obj_const))
+    return Model(h)
+
+
+def _run(model: Model, opt: Options, cfg: EngineConfig | None) -> Solution:
+    L = _core.lib()
+    x = np.zeros(model.n)
+    y = np.zeros(model.m)
+    info = SolveInfo()
+    msg = C.create_string_buffer(512)
+    rc = L.igaos_pdhg_solve(model._h, C.byref(opt),
+                            C.byref(cfg) if cfg is not None else None,
+                            x.ctypes.data_as(C.POINTER(C.c_double)),
+                            y.ctypes.data_as(C.POINTER(C.c_double)),
+                            C.byref(info), msg, 512)
+    if rc != 0:
+        raise ValueError(_core.last_error())
+    st = Status(info.status)
+    extra = {k: getattr(info, k) for k, _ in SolveInfo._fields_}
+    return Solution(status=STATUS_NAMES[st], status_enum=st,
+                    objective=info.objective, x=x, y=y,
+                    row_activity=np.zeros(0), pinf=info.pinf, dinf=info.dinf,
+                    rel_gap=info.rel_gap, iterations=info.iterations,
+                    solve_time_ms=info.solve_time_ms,
+                    message=msg.value.decode(errors="replace"), info=extra)
+
+
+def _options(time_limit, tolerance, max_iterations, presolve, seed,
+             verbosity=0) -> Options:
+    o = Options()
+    _core.lib().igaos_options_default(C.byref(o))
+    o.time_limit_s = float(time_limit)
+    o.tolerance = float(tolerance)
+    o.max_iterations = int(max_iterations)
+    o.presolve = int(bool(presolve))
+    o.seed = int(seed)
+    o.verbosity = int(verbosity)
+    return o
+
+
+def _config(precision=64, graph_slots=None, max_repair_m=None) -> EngineConfig:
+    c = EngineConfig()
+    _core.lib().igaos_engine_config_default(C.byref(c))
+    c.precision = int(precision)
+    if graph_slots is not None:
+        c.graph_slots = int(graph_slots)
+    if max_repair_m is not None:
+        c.max_repair_m = int(max_repair_m)
+    return c
+
+
+def solve(path, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
+          presolve=True, seed=0, engine="auto") -> Solution:
+    """``igaos.solve`` (python/igaos_bindings.cpp:26-46, 144-147).
+
+    ``engine="pdhg"`` runs the MI355X engine.  The other valid names
+    (auto, simplex, milp, qp) belong to engines outside the pdhg slot and
+    return an ``error`` Solution; unknown names raise ValueError.
+    """
+    if engine not in _ENGINES:
+        raise ValueError(f"engine must be one of auto|simplex|pdhg|milp|qp, "
+                         f"got '{engine}'")
+    model = read_mps(path)
+    if engine != "pdhg":
+        return Solution(message=f"engine '{engine}' is not part of this build "
+                                f"(pdhg engine slot only)")
+    return _run(model, _options(time_limit, tolerance, max_iterations,
+                                presolve, seed), None)
+
+
+def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
+             precision=64, verbosity=0, graph_slots=None,
+             max_repair_m=None) -> Solution:
+    """Solve an in-memory LP (``lpgen.LP`` or ``Model``) with the engine.
+
+    ``precision=64`` is the plain FP64 engine; ``precision=32`` runs FP32
+    iterations on an FP64-refined correction problem.
+    """
+    model = lp if isinstance(lp, Model) else model_from_lp(lp)
+    return _run(model, _options(time_limit, tolerance, max_iterations, True,
+                                0, verbosity),
+                _config(precision, graph_slots, max_repair_m))
+
+
+has_pdhg = _has_pdhg()

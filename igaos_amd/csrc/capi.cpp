@@ -1,0 +1,302 @@
+// extern "C" layer of libigaos_pdhg.so: see include/igaos_pdhg.h.
+#include "igaos_pdhg.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "igaos/status.h"
+#include "model.hpp"
+#include "pdhg.hpp"
+#include "pdhg_kernels.hpp"
+
+struct igaos_model {
+    igaos::io::Model m;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& what) {
+    g_err = what;
+    return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int igaos_pdhg_abi_version(void) { return IGAOS_PDHG_ABI_VERSION; }
+
+int igaos_pdhg_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+const char* igaos_last_error(void) { return g_err.c_str(); }
+
+const char* igaos_status_name(int status) {
+    if (status < 0 || status > IGAOS_ERROR) return "unknown";
+    return igaos::status_name(static_cast<igaos::Status>(status));
+}
+
+igaos_model* igaos_model_read_mps(const char* path) {
+    try {
+        auto* h = new igaos_model;
+        try {
+            igaos::io::parse_mps(path ? path : "", h->m);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return h;
+    } catch (const std::exception& e) {
+        fail(e.what());
+        return nullptr;
+    }
+}
+
+igaos_model* igaos_model_from_triplets(int m, int n, int nnz,
+                                       const int* rows, const int* cols,
+                                       const double* vals, const double* c,
+                                       const double* cl, const double* cu,
+                                       const double* rmin, const double* rmax,
+                                       double obj_const) {
+    if (m < 0 || n < 0 || nnz < 0) {
+        fail("negative size");
+        return nullptr;
+    }
+    std::vector<igaos::io::Triplet> t(nnz);
+    for (int p = 0; p < nnz; ++p) {
+        if (rows[p] < 0 || rows[p] >= m || cols[p] < 0 || cols[p] >= n) {
+            fail("triplet index out of range");
+            return nullptr;
+        }
+        t[p] = {rows[p], cols[p], vals[p]};
+    }
+    auto* h = new igaos_model;
+    auto& md = h->m;
+    md.m = m;
+    md.n = n;
+    md.c.assign(c, c + n);
+    md.cl.assign(cl, cl + n);
+    md.cu.assign(cu, cu + n);
+    md.rmin.assign(rmin, rmin + m);
+    md.rmax.assign(rmax, rmax + m);
+    md.integ.assign(n, 0);
+    md.obj_const = obj_const;
+    igaos::io::counting_sort(t, m, n, md.ap, md.ai, md.ax, md.cp, md.ci, md.acx);
+    return h;
+}
+
+igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
+                                      const double* cl, const double* cu,
+                                      const double* rmin, const double* rmax,
+                                      const int* ap, const int* ai,
+                                      const double* ax, const int* cp,
+                                      const int* ci, const double* acx,
+                                      double obj_const) {
+    if (m < 0 || n < 0 || !ap || !cp) {
+        fail("invalid sizes or pointers");
+        return nullptr;
+    }
+    const int nnz = ap[m];
+    if (cp[n] != nnz || nnz < 0) {
+        fail("CSR and CSC nonzero counts differ");
+        return nullptr;
+    }
+    auto* h = new igaos_model;
+    auto& md = h->m;
+    md.m = m;
+    md.n = n;
+    md.c.assign(c, c + n);
+    md.cl.assign(cl, cl + n);
+    md.cu.assign(cu, cu + n);
+    md.rmin.assign(rmin, rmin + m);
+    md.rmax.assign(rmax, rmax + m);
+    md.ap.assign(ap, ap + m + 1);
+    md.ai.assign(ai, ai + nnz);
+    md.ax.assign(ax, ax + nnz);
+    md.cp.assign(cp, cp + n + 1);
+    md.ci.assign(ci, ci + nnz);
+    md.acx.assign(acx, acx + nnz);
+    md.integ.assign(n, 0);
+    md.obj_const = obj_const;
+    return h;
+}
+
+void igaos_model_free(igaos_model* model) { delete model; }
+
+void igaos_model_dims(const igaos_model* h, int* m, int* n, int* nnz,
+                      double* obj_const, int* n_fr, int* n_ranges,
+                      int* n_int) {
+    if (!h) return;
+    const auto& md = h->m;
+    if (m) *m = md.m;
+    if (n) *n = md.n;
+    if (nnz) *nnz = md.nnz();
+    if (obj_const) *obj_const = md.obj_const;
+    if (n_fr) *n_fr = md.n_fr_parsed;
+    if (n_ranges) *n_ranges = md.n_ranges_parsed;
+    if (n_int) {
+        int k = 0;
+        for (unsigned char v : md.integ) k += v != 0;
+        *n_int = k;
+    }
+}
+
+void igaos_model_get(const igaos_model* h, double* rmin, double* rmax,
+                     double* cl, double* cu, double* c, int* ap, int* ai,
+                     double* ax, int* cp, int* ci, double* acx) {
+    if (!h) return;
+    const auto& md = h->m;
+    auto cpy = [](auto* dst, const auto& src) {
+        if (dst && !src.empty()) std::memcpy(dst, src.data(), sizeof(src[0]) * src.size());
+    };
+    cpy(rmin, md.rmin);
+    cpy(rmax, md.rmax);
+    cpy(cl, md.cl);
+    cpy(cu, md.cu);
+    cpy(c, md.c);
+    cpy(ap, md.ap);
+    cpy(ai, md.ai);
+    cpy(ax, md.ax);
+    cpy(cp, md.cp);
+    cpy(ci, md.ci);
+    cpy(acx, md.acx);
+}
+
+void igaos_options_default(igaos_options* opt) {
+    if (!opt) return;
+    igaos::Options o;
+    opt->time_limit_s = o.time_limit_s;
+    opt->tolerance = o.tolerance;
+    opt->max_iterations = o.max_iterations;
+    opt->seed = o.seed;
+    opt->presolve = o.presolve ? 1 : 0;
+    opt->restarts = o.restarts ? 1 : 0;
+    opt->verbosity = o.verbosity;
+}
+
+void igaos_engine_config_default(igaos_engine_config* cfg) {
+    if (!cfg) return;
+    igaos::pdhg::EngineConfig e;
+    cfg->precision = e.precision;
+    cfg->graph_slots = e.graph_slots;
+    cfg->max_repair_m = e.max_repair_m;
+    cfg->device = e.device;
+}
+
+int igaos_pdhg_solve(const igaos_model* model, const igaos_options* opt,
+                     const igaos_engine_config* cfg, double* x, double* y,
+                     igaos_solve_info* info, char* msg, int msglen) {
+    if (!model || !opt) return fail("null model or options");
+    igaos::Options o;
+    o.time_limit_s = opt->time_limit_s;
+    o.tolerance = opt->tolerance;
+    o.max_iterations = opt->max_iterations;
+    o.seed = opt->seed;
+    o.presolve = opt->presolve != 0;
+    o.restarts = opt->restarts != 0;
+    o.verbosity = opt->verbosity;
+    igaos::pdhg::EngineConfig ec;
+    if (cfg) {
+        if (cfg->precision != 32 && cfg->precision != 64)
+            return fail("precision must be 32 or 64");
+        ec.precision = cfg->precision;
+        ec.graph_slots = std::max(0, cfg->graph_slots);
+        ec.max_repair_m = cfg->max_repair_m;
+        ec.device = cfg->device;
+    }
+    igaos::pdhg::SolveInfo si;
+    igaos::Solution s;
+    try {
+        s = igaos::pdhg::solve_ex(model->m, o, ec, &si);
+    } catch (const std::exception& e) {
+        s = igaos::Solution{};
+        s.message = e.what();
+    }
+    if (x && s.x.size() == static_cast<size_t>(model->m.n))
+        std::memcpy(x, s.x.data(), sizeof(double) * s.x.size());
+    if (y && s.y.size() == static_cast<size_t>(model->m.m))
+        std::memcpy(y, s.y.data(), sizeof(double) * s.y.size());
+    if (msg && msglen > 0) {
+        std::strncpy(msg, s.message.c_str(), msglen - 1);
+        msg[msglen - 1] = '\0';
+    }
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->status = static_cast<int>(s.status);
+        info->objective = s.objective;
+        info->pinf = s.pinf;
+        info->dinf = s.dinf;
+        info->rel_gap = s.rel_gap;
+        info->iterations = s.iterations;
+        info->solve_time_ms = s.solve_time_ms;
+        info->setup_ms = si.setup_ms;
+        info->attempts = si.attempts;
+        info->restarts = si.restarts;
+        info->checks = si.checks;
+        info->refinements = si.refinements;
+        info->eta = si.eta;
+        info->omega = si.omega;
+        info->lam2 = si.lam2;
+        info->eta0 = si.eta0;
+        info->omega0 = si.omega0;
+        info->last_is_avg = si.last_is_avg;
+        info->used_incumbent = si.used_incumbent;
+        info->repaired = si.repaired;
+        info->kernel_ms = si.kernel_ms;
+    }
+    if (s.status == igaos::Status::Error) g_err = s.message;
+    return 0;
+}
+
+int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap) {
+    if (nrows < 0 || !ptr) return fail("invalid row pointer");
+    const std::vector<int> rb = igaos::dev::build_row_blocks(ptr, nrows);
+    const int k = static_cast<int>(rb.size());
+    if (out && cap >= k) std::memcpy(out, rb.data(), sizeof(int) * k);
+    return k;
+}
+
+int igaos_spmv_csr_f64(int nblk, const int* rowblocks, const int* ptr,
+                       const int* idx, const double* val, const double* x,
+                       double* y, void* stream) {
+    if (nblk < 0) return fail("negative block count");
+    igaos::dev::spmv({rowblocks, nblk}, ptr, idx, val, x, y,
+                     static_cast<hipStream_t>(stream));
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail(hipGetErrorString(e));
+}
+
+int igaos_spmv_csr_f32(int nblk, const int* rowblocks, const int* ptr,
+                       const int* idx, const float* val, const float* x,
+                       float* y, void* stream) {
+    if (nblk < 0) return fail("negative block count");
+    igaos::dev::spmv({rowblocks, nblk}, ptr, idx, val, x, y,
+                     static_cast<hipStream_t>(stream));
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail(hipGetErrorString(e));
+}
+
+int igaos_ruiz_f64(int m, int n, const int* ap, const int* ai,
+                   const double* ax, const int* cp, const int* ci,
+                   const double* acx, double* dr, double* dc, void* stream) {
+    if (m < 0 || n < 0) return fail("negative size");
+    igaos::dev::ruiz(m, n, ap, ai, ax, cp, ci, acx, dr, dc,
+                     static_cast<hipStream_t>(stream));
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail(hipGetErrorString(e));
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+#pragma once
+// Device-side building blocks shared by the single-LP and batched PDHG
+// kernels: wavefront/workgroup reductions and the row-block SpMV core.
+//
+// Row-block SpMV (CSR-Adaptive, Greathouse & Daga, SC'14): rows are cut on
+// the host into contiguous blocks holding at most NNZ_MAX nonzeros and at
+// most ROWS_MAX rows; a row longer than NNZ_MAX gets a block of its own.
+// A multi-row block streams its values and column indices with fully
+// coalesced loads into LDS as products val*x[col], then 2^k lanes per row
+// (k chosen from the block's row count) reduce each row with xor shuffles
+// inside a 64-lane wavefront.  A single long row is reduced by the whole
+// workgroup.  Results land in LDS so that the fused epilogue walks the
+// block's rows with consecutive threads (coalesced vector reads/writes).
+// The summation order depends only on the block layout, so results are
+// run-to-run deterministic.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace igaos::dev {
+
+constexpr int WAVE = 64;
+constexpr int BS = 256;         // threads per row-block workgroup
+constexpr int NNZ_MAX = 2048;   // nonzeros staged per multi-row block
+constexpr int ROWS_MAX = 256;   // rows per block (one epilogue row per thread)
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Workgroup sum with a fixed combination order; every thread gets the
+// result.  `red` must hold BS/WAVE entries.  Contains __syncthreads().
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x / WAVE;
+    __syncthreads();
+    if ((threadIdx.x & (WAVE - 1)) == 0) red[w] = v;
+    __syncthreads();
+    T s = red[0];
+#pragma unroll
+    for (int i = 1; i < BS / WAVE; ++i) s += red[i];
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* red) {
+    v = wave_max(v);
+    const int w = threadIdx.x / WAVE;
+    __syncthreads();
+    if ((threadIdx.x & (WAVE - 1)) == 0) red[w] = v;
+    __syncthreads();
+    T s = red[0];
+#pragma unroll
+    for (int i = 1; i < BS / WAVE; ++i) s = fmax(s, red[i]);
+    return s;
+}
+
+// Agent-scope loads/stores for partials read by the last workgroup of the
+// same launch (the 8 XCDs have separate L2s).
+__device__ __forceinline__ void store_agent(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_agent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Last-workgroup-done election: every workgroup calls this once after
+// publishing its partials; returns true (uniformly within the workgroup)
+// only in the last one to arrive, which also resets the counter.
+__device__ __forceinline__ bool elect_last(unsigned int* ticket, int* s_flag) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int prev = atomicAdd(ticket, 1u);
+        *s_flag = (prev == gridDim.x * gridDim.y - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    const bool last = *s_flag != 0;
+    if (last) {
+        __threadfence();
+        if (threadIdx.x == 0) atomicExch(ticket, 0u);
+    }
+    return last;
+}
+
+// Fixed-order sum of `cnt` partials (strided per thread, then tree).
+__device__ __forceinline__ double fixed_sum(const double* p, int cnt,
+                                            double* red) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < cnt; i += BS) s += load_agent(p + i);
+    return block_sum(s, red);
+}
+__device__ __forceinline__ double fixed_max(const double* p, int cnt,
+                                            double* red) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < cnt; i += BS) s = fmax(s, load_agent(p + i));
+    return block_max(s, red);
+}
+
+// Gather functors: x[j] for plain vectors, xref[j] + (double)xd[j] for an
+// FP64 reference plus an FP32 correction.
+template <typename V>
+struct GatherPlain {
+    const V* x;
+    __device__ __forceinline__ V operator()(int j) const { return x[j]; }
+};
+template <typename V>
+struct GatherRef {
+    const double* ref;
+    const V* d;
+    __device__ __forceinline__ double operator()(int j) const {
+        return ref ? ref[j] + static_cast<double>(d[j])
+                   : static_cast<double>(d[j]);
+    }
+};
+
+// Row sums of one row block [r0, r1): s_rows[r - r0] = sum_p val[p]*x(col).
+// `s_prod` holds NNZ_MAX entries of Acc, `s_rows` ROWS_MAX entries, `red`
+// BS/WAVE entries.  Contains __syncthreads(); all threads must call it.
+template <typename Acc, typename Val, typename G>
+__device__ __forceinline__ void rowblock_sums(const int* __restrict__ ptr,
+                                              const int* __restrict__ idx,
+                                              const Val* __restrict__ val,
+                                              const G& gx, int r0, int r1,
+                                              Acc* s_rows, Acc* s_prod,
+                                              Acc* red) {
+    const int tid = threadIdx.x;
+    const int nrows = r1 - r0;
+    const int p0 = ptr[r0];
+    const int p1 = ptr[r1];
+    const int nz = p1 - p0;
+    if (nrows == 1 && nz > NNZ_MAX) {
+        Acc s = Acc(0);
+        for (int p = p0 + tid; p < p1; p += BS)
+            s += static_cast<Acc>(val[p]) * static_cast<Acc>(gx(idx[p]));
+        s = block_sum(s, red);
+        if (tid == 0) s_rows[0] = s;
+        __syncthreads();
+        return;
+    }
+    for (int q = tid; q < nz; q += BS)
+        s_prod[q] = static_cast<Acc>(val[p0 + q]) *
+                    static_cast<Acc>(gx(idx[p0 + q]));
+    __syncthreads();
+    int tpr = 1;
+    while (tpr < WAVE && tpr * 2 * nrows <= BS) tpr *= 2;
+    const int row = tid / tpr;
+    const int lane = tid & (tpr - 1);
+    Acc s = Acc(0);
+    if (row < nrows) {
+        const int a = ptr[r0 + row] - p0;
+        const int b = ptr[r0 + row + 1] - p0;
+        for (int q = a + lane; q < b; q += tpr) s += s_prod[q];
+    }
+    for (int o = tpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (row < nrows && lane == 0) s_rows[row] = s;
+    __syncthreads();
+}
+
+// Row maxima of |val| * f(row, col): used by Ruiz equilibration.
+}  // namespace igaos::dev

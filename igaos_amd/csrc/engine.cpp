@@ -1,0 +1,732 @@
+// igaos::pdhg::solve on one MI355X: the reference's restarted adaptive PDHG
+// (src/pdhg/pdhg.cu:457-1156) with every iterate, the line search and the
+// KKT reductions resident in HBM.
+//
+// Structure of a solve:
+//  * setup (untimed, as in the reference, pdhg.cu:692): upload the model,
+//    Ruiz equilibration and scaling on the device, power iteration for
+//    ||A~||_2^2, eta0 = 1.2/||A~||, omega0 = clamp(||c~||/||q~||, 0.3, 3);
+//  * iterations: hipGraph replays of `graph_slots` attempt slots; each slot
+//    is one line-search attempt (k_primal + k_dual).  The dual kernel's last
+//    workgroup decides acceptance on the device and freezes the state when
+//    an accepted iteration lands on a multiple of 64 (or MAXIT), so the host
+//    only synchronises once per check period;
+//  * check (host control, device reductions): flush the pending averaging,
+//    evaluate both iterates' KKT metrics in FP64 on the device, then apply
+//    the reference's termination, dual repair, incumbent, rollback,
+//    restart and primal-weight rules (pdhg.cu:801-1084).
+//
+// Precision 32 runs the same iteration in FP32 on a correction problem:
+// x = x_ref + dx, y = y_ref + dy with (x_ref, y_ref) in FP64.  The FP32
+// kernels see the shifted bounds clamp(dx .. scl - x_ref, scu - x_ref), the
+// FP64-computed reduced cost g_ref = c~ + A~'y_ref, the row bounds less
+// A~x_ref, and y_ref as the dual clamp offset; A~ is held in FP32 as well.
+// KKT checks rebuild x, y in FP64 and use the FP64 matrix.  Every restart
+// folds the anchor's correction into (x_ref, y_ref) in FP64, recomputes the
+// residual data with FP64 SpMVs and restarts the FP32 correction at zero:
+// an outer iterative refinement whose inner solver is PDHG in FP32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "pdhg.hpp"
+#include "pdhg_kernels.hpp"
+
+namespace igaos::pdhg {
+
+namespace {
+
+struct HipFailure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCK(expr)                                                          \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess)                                                \
+            throw HipFailure(std::string("HIP error in PDHG engine: ") +     \
+                             hipGetErrorString(e_) + " at " #expr);          \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t k) {
+        n = k;
+        HIPCK(hipMalloc(&p, sizeof(T) * (k > 0 ? k : 1)));
+    }
+    void upload(const T* h, size_t k, hipStream_t s) {
+        alloc(k);
+        if (k) HIPCK(hipMemcpyAsync(p, h, sizeof(T) * k, hipMemcpyHostToDevice, s));
+    }
+    void zero(hipStream_t s) {
+        HIPCK(hipMemsetAsync(p, 0, sizeof(T) * (n > 0 ? n : 1), s));
+    }
+};
+
+double now_s() {
+    return std::chrono::duration<double>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+template <typename V>
+class Engine {
+  public:
+    Engine(const io::Model& lp, const Options& opt, const EngineConfig& cfg)
+        : lp_(lp), opt_(opt), cfg_(cfg), m_(lp.m), n_(lp.n) {}
+    ~Engine() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        if (ev0_) (void)hipEventDestroy(ev0_);
+        if (ev1_) (void)hipEventDestroy(ev1_);
+        if (s_) (void)hipStreamDestroy(s_);
+    }
+    Solution run(SolveInfo* info);
+
+  private:
+    static constexpr bool F32 = std::is_same<V, float>::value;
+    const io::Model& lp_;
+    const Options& opt_;
+    EngineConfig cfg_;
+    const int m_, n_;
+    hipStream_t s_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+
+    // original data (FP64) and scaled data
+    DBuf<int> ap_, ai_, cp_, ci_, rbA_, rbT_;
+    DBuf<double> ax_, acx_, c_, cl_, cu_, rmin_, rmax_;
+    DBuf<double> dr_, dc_, sax_, sacx_, sc_, scl_, scu_, srmin_, srmax_;
+    // iteration vectors of precision V
+    DBuf<V> vax_, vacx_;  // FP32 copies of the scaled matrix
+    DBuf<V> x0_, x1_, y0_, y1_, xbar_, ata_, xavg_, yavg_, axcur_, adx_;
+    DBuf<V> lo_, hi_, gref_, rlo_, rhi_, yoff_;  // FP32 correction data
+    // FP64 references of the FP32 engine and scratch
+    DBuf<double> xref_, yref_, aref_, atref_;
+    DBuf<double> xb_, yb_, pesx_, pesy_, tmpm_, tmpn_;
+    DBuf<double> partT_, partA_, partC_, partR_, partR2_, rowres_, red_, scal_;
+    DBuf<unsigned int> tickets_;
+    DBuf<LsState> st_;
+    DBuf<Metrics> met_;
+    dev::IterBufs<V> ib_;
+    dev::KktData kd_;
+    dev::RowBlocks rA_, rT_;
+    double lam2_ = 1.0, eta0_ = 1.0, omega0_ = 1.0;
+
+    void setup();
+    double power_iteration();
+    double dsum(const double* a, const double* b, long n);
+    void check_metrics(int cur, Metrics& mc, Metrics& ma);
+    void recompute_refs();
+    void sync() { HIPCK(hipStreamSynchronize(s_)); }
+    LsState read_state() {
+        LsState h;
+        HIPCK(hipMemcpyAsync(&h, st_.p, sizeof(LsState), hipMemcpyDeviceToHost, s_));
+        sync();
+        return h;
+    }
+    void write_state(const LsState& h) {
+        HIPCK(hipMemcpyAsync(st_.p, &h, sizeof(LsState), hipMemcpyHostToDevice, s_));
+    }
+    V* xcur(int c) { return c ? x1_.p : x0_.p; }
+    V* ycur(int c) { return c ? y1_.p : y0_.p; }
+    void copy_v(V* dst, const V* src, long k) {
+        if (k) HIPCK(hipMemcpyAsync(dst, src, sizeof(V) * k, hipMemcpyDeviceToDevice, s_));
+    }
+    void copy_d(double* dst, const double* src, long k) {
+        if (k) HIPCK(hipMemcpyAsync(dst, src, sizeof(double) * k, hipMemcpyDeviceToDevice, s_));
+    }
+    void zero_v(V* p, long k) { dev::fill(p, k, V(0), s_); }
+    // full FP64 scaled iterate (x or y) of a V vector, into dst
+    void full(double* dst, const double* ref, const V* d, long k);
+};
+
+template <typename V>
+void Engine<V>::full(double* dst, const double* ref, const V* d, long k) {
+    if constexpr (F32) {
+        dev::combine(ref, d, dst, k, s_);
+    } else {
+        (void)ref;
+        copy_d(dst, d, k);
+    }
+}
+
+template <typename V>
+double Engine<V>::dsum(const double* a, const double* b, long n) {
+    dev::sumsq_diff(a, b, n, red_.p, tickets_.p + 2, scal_.p, s_);
+    double h = 0.0;
+    HIPCK(hipMemcpyAsync(&h, scal_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
+    sync();
+    return h;
+}
+
+template <typename V>
+double Engine<V>::power_iteration() {
+    // pdhg.cu:605-631: v0 = 1/sqrt(n); 30 x { w = A'A v; v = w/||w|| };
+    // lam2 = ||A v||^2 (non-finite or <= 0 -> 1).
+    DBuf<double> v, w;
+    v.alloc(n_);
+    w.alloc(n_);
+    dev::fill(v.p, n_, 1.0 / std::sqrt(static_cast<double>(n_)), s_);
+    for (int it = 0; it < 30; ++it) {
+        dev::spmv(rA_, ap_.p, ai_.p, sax_.p, v.p, tmpm_.p, s_);
+        dev::spmv(rT_, cp_.p, ci_.p, sacx_.p, tmpm_.p, w.p, s_);
+        const double nw = std::sqrt(dsum(w.p, nullptr, n_));
+        if (nw < 1e-300) break;
+        dev::scale_into(v.p, w.p, n_, 1.0 / nw, s_);
+    }
+    dev::spmv(rA_, ap_.p, ai_.p, sax_.p, v.p, tmpm_.p, s_);
+    const double na = std::sqrt(dsum(tmpm_.p, nullptr, m_));
+    double lam2 = na * na;
+    if (!(lam2 > 0) || !std::isfinite(lam2)) lam2 = 1.0;
+    return lam2;
+}
+
+template <typename V>
+void Engine<V>::setup() {
+    HIPCK(hipSetDevice(cfg_.device));
+    HIPCK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    HIPCK(hipEventCreate(&ev0_));
+    HIPCK(hipEventCreate(&ev1_));
+    const long nnz = lp_.nnz();
+    ap_.upload(lp_.ap.data(), m_ + 1, s_);
+    ai_.upload(lp_.ai.data(), nnz, s_);
+    ax_.upload(lp_.ax.data(), nnz, s_);
+    cp_.upload(lp_.cp.data(), n_ + 1, s_);
+    ci_.upload(lp_.ci.data(), nnz, s_);
+    acx_.upload(lp_.acx.data(), nnz, s_);
+    c_.upload(lp_.c.data(), n_, s_);
+    cl_.upload(lp_.cl.data(), n_, s_);
+    cu_.upload(lp_.cu.data(), n_, s_);
+    rmin_.upload(lp_.rmin.data(), m_, s_);
+    rmax_.upload(lp_.rmax.data(), m_, s_);
+    const std::vector<int> hbA = dev::build_row_blocks(lp_.ap.data(), m_);
+    const std::vector<int> hbT = dev::build_row_blocks(lp_.cp.data(), n_);
+    rbA_.upload(hbA.data(), hbA.size(), s_);
+    rbT_.upload(hbT.data(), hbT.size(), s_);
+    rA_ = {rbA_.p, static_cast<int>(hbA.size()) - 1};
+    rT_ = {rbT_.p, static_cast<int>(hbT.size()) - 1};
+
+    dr_.alloc(m_);
+    dc_.alloc(n_);
+    dev::ruiz(m_, n_, ap_.p, ai_.p, ax_.p, cp_.p, ci_.p, acx_.p, dr_.p, dc_.p, s_);
+    sax_.alloc(nnz);
+    sacx_.alloc(nnz);
+    sc_.alloc(n_);
+    scl_.alloc(n_);
+    scu_.alloc(n_);
+    srmin_.alloc(m_);
+    srmax_.alloc(m_);
+    dev::scale_data(m_, n_, ap_.p, ai_.p, ax_.p, cp_.p, ci_.p, acx_.p, dr_.p,
+                    dc_.p, c_.p, cl_.p, cu_.p, rmin_.p, rmax_.p, sax_.p,
+                    sacx_.p, sc_.p, scl_.p, scu_.p, srmin_.p, srmax_.p, s_);
+
+    tmpm_.alloc(m_);
+    tmpn_.alloc(n_);
+    red_.alloc(1024);
+    scal_.alloc(4);
+    tickets_.alloc(8);
+    tickets_.zero(s_);
+
+    lam2_ = power_iteration();
+    eta0_ = 1.2 / std::sqrt(lam2_);
+    {  // pdhg.cu:647-663
+        const double cn = dsum(sc_.p, nullptr, n_);
+        dev::sumsq_q(srmin_.p, srmax_.p, m_, red_.p, tickets_.p + 2, scal_.p, s_);
+        double qn = 0.0;
+        HIPCK(hipMemcpyAsync(&qn, scal_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
+        sync();
+        omega0_ = 1.0;
+        if (cn > 1e-24 && qn > 1e-24)
+            omega0_ = std::clamp(std::sqrt(cn / qn), 0.3, 3.0);
+    }
+    double cmax = 0.0;
+    dev::absmax(c_.p, n_, red_.p, tickets_.p + 2, scal_.p, s_);
+    HIPCK(hipMemcpyAsync(&cmax, scal_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
+    sync();
+
+    // iteration buffers
+    x0_.alloc(n_); x1_.alloc(n_); xbar_.alloc(n_); ata_.alloc(n_); xavg_.alloc(n_);
+    y0_.alloc(m_); y1_.alloc(m_); yavg_.alloc(m_); axcur_.alloc(m_); adx_.alloc(m_);
+    for (DBuf<V>* b : {&x0_, &x1_, &xbar_, &ata_, &xavg_, &y0_, &y1_, &yavg_, &axcur_, &adx_})
+        zero_v(b->p, b->n);
+    xb_.alloc(n_); yb_.alloc(m_); pesx_.alloc(n_); pesy_.alloc(m_);
+    partT_.alloc(std::max(rT_.nblk, 1));
+    partA_.alloc(2 * std::max(rA_.nblk, 1));
+    partC_.alloc(8 * std::max(rT_.nblk, 1));
+    partR_.alloc(8 * std::max(rA_.nblk, 1));
+    partR2_.alloc(2 * 2 * 1024);
+    rowres_.alloc(8);
+    st_.alloc(1);
+    met_.alloc(2);
+
+    ib_.m = m_;
+    ib_.n = n_;
+    ib_.ap = ap_.p; ib_.ai = ai_.p; ib_.cp = cp_.p; ib_.ci = ci_.p;
+    ib_.rbA = rA_; ib_.rbT = rT_;
+    ib_.x[0] = x0_.p; ib_.x[1] = x1_.p; ib_.y[0] = y0_.p; ib_.y[1] = y1_.p;
+    ib_.xbar = xbar_.p; ib_.ata = ata_.p; ib_.xavg = xavg_.p; ib_.yavg = yavg_.p;
+    ib_.axcur = axcur_.p; ib_.adx = adx_.p;
+    ib_.partT = partT_.p; ib_.partA = partA_.p;
+    ib_.st = st_.p; ib_.ticket = tickets_.p + 3;
+    if constexpr (F32) {
+        vax_.alloc(nnz);
+        vacx_.alloc(nnz);
+        dev::to_float(sax_.p, vax_.p, nnz, s_);
+        dev::to_float(sacx_.p, vacx_.p, nnz, s_);
+        ib_.ax = vax_.p;
+        ib_.acx = vacx_.p;
+        xref_.alloc(n_); yref_.alloc(m_); aref_.alloc(m_); atref_.alloc(n_);
+        lo_.alloc(n_); hi_.alloc(n_); gref_.alloc(n_);
+        rlo_.alloc(m_); rhi_.alloc(m_); yoff_.alloc(m_);
+        dev::fill(xref_.p, n_, 0.0, s_);
+        dev::fill(yref_.p, m_, 0.0, s_);
+        recompute_refs();
+        ib_.lo = lo_.p; ib_.hi = hi_.p; ib_.gref = gref_.p;
+        ib_.rlo = rlo_.p; ib_.rhi = rhi_.p; ib_.yoff = yoff_.p;
+    } else {
+        ib_.ax = sax_.p;
+        ib_.acx = sacx_.p;
+        ib_.lo = scl_.p; ib_.hi = scu_.p; ib_.gref = sc_.p;
+        ib_.rlo = srmin_.p; ib_.rhi = srmax_.p; ib_.yoff = nullptr;
+    }
+
+    kd_.m = m_;
+    kd_.n = n_;
+    kd_.ap = ap_.p; kd_.ai = ai_.p; kd_.cp = cp_.p; kd_.ci = ci_.p;
+    kd_.sax = sax_.p; kd_.sacx = sacx_.p;
+    kd_.rbA = rA_; kd_.rbT = rT_;
+    kd_.c = c_.p; kd_.cl = cl_.p; kd_.cu = cu_.p;
+    kd_.rmin = rmin_.p; kd_.rmax = rmax_.p;
+    kd_.dr = dr_.p; kd_.dc = dc_.p;
+    kd_.cmax = cmax;
+    kd_.partC = partC_.p; kd_.partR = partR_.p; kd_.partR2 = partR2_.p;
+    kd_.rowres = rowres_.p; kd_.tickets = tickets_.p;
+    sync();
+}
+
+// FP32 engine: residual data of the correction problem at (x_ref, y_ref).
+template <typename V>
+void Engine<V>::recompute_refs() {
+    if constexpr (F32) {
+        dev::spmv(rA_, ap_.p, ai_.p, sax_.p, xref_.p, aref_.p, s_);
+        dev::spmv(rT_, cp_.p, ci_.p, sacx_.p, yref_.p, atref_.p, s_);
+        dev::refine_data(m_, n_, scl_.p, scu_.p, sc_.p, xref_.p, atref_.p,
+                         srmin_.p, srmax_.p, aref_.p, yref_.p, lo_.p, hi_.p,
+                         gref_.p, rlo_.p, rhi_.p, yoff_.p, s_);
+    }
+}
+
+template <typename V>
+void Engine<V>::check_metrics(int cur, Metrics& mc, Metrics& ma) {
+    dev::KktIter<V> it;
+    if constexpr (F32) {
+        it.xref[0] = it.xref[1] = xref_.p;
+        it.yref[0] = it.yref[1] = yref_.p;
+    }
+    it.xd[0] = xcur(cur);
+    it.yd[0] = ycur(cur);
+    it.xd[1] = xavg_.p;
+    it.yd[1] = yavg_.p;
+    dev::launch_kkt(kd_, it, met_.p, s_);
+    Metrics h[2];
+    HIPCK(hipMemcpyAsync(h, met_.p, sizeof(h), hipMemcpyDeviceToHost, s_));
+    sync();
+    mc = h[0];
+    ma = h[1];
+}
+
+template <typename V>
+Solution Engine<V>::run(SolveInfo* info) {
+    Solution sol;
+    const double t_setup0 = now_s();
+    setup();
+    const double INF = std::numeric_limits<double>::infinity();
+    const double t_setup1 = now_s();
+
+    const long MAXIT = (opt_.time_limit_s > 0 && opt_.time_limit_s < 1e6)
+                           ? 20000000L
+                           : opt_.max_iterations;
+    const double TOL = opt_.tolerance;
+
+    LsState st{};
+    st.eta = eta0_;
+    st.omega = omega0_;
+    st.k = 1;
+    st.hn = 0;
+    st.maxit = MAXIT;
+    st.fresh = 1;
+    st.cur = 0;
+    st.frozen = MAXIT < 1 ? 1 : 0;
+    write_state(st);
+
+    const int G = cfg_.graph_slots;
+    if (G > 0) {
+        HIPCK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        for (int g = 0; g < G; ++g) dev::launch_attempt(ib_, s_);
+        HIPCK(hipStreamEndCapture(s_, &graph_));
+        HIPCK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    }
+    auto launch_slots = [&](long slots) {
+        if (G > 0) {
+            const long reps = (slots + G - 1) / G;
+            for (long r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(exec_, s_));
+        } else {
+            for (long r = 0; r < slots; ++r) dev::launch_attempt(ib_, s_);
+        }
+    };
+
+    RestartState rs{-1.0, INF, 0};
+    bool have_pe = false;
+    Metrics last{};
+    bool last_is_avg = false;
+    Metrics inc{};
+    bool have_inc = false;
+    double inc_op = 0.0;
+    long last_rollback = 0;
+    long next_repair = 5000, repair_spacing = 2000;
+    bool repaired = false, time_limit = false;
+    long iters_run = 0, restarts = 0, checks = 0, refinements = 0;
+    std::vector<double> rep_x, rep_y;  // repaired pair (original units)
+    float kernel_ms_total = 0.0f;
+
+    const double t0 = now_s();
+    while (true) {
+        if (!st.frozen) {
+            // attempts until the device freezes at the next check boundary
+            HIPCK(hipEventRecord(ev0_, s_));
+            while (true) {
+                const long k = st.k;
+                long next = ((k - 1) / CHECK_EVERY + 1) * CHECK_EVERY;
+                if (next > MAXIT) next = MAXIT;
+                launch_slots(std::max(1L, next - k + 1));
+                st = read_state();
+                if (st.frozen) break;
+            }
+            HIPCK(hipEventRecord(ev1_, s_));
+            HIPCK(hipEventSynchronize(ev1_));
+            float ms = 0.0f;
+            HIPCK(hipEventElapsedTime(&ms, ev0_, ev1_));
+            kernel_ms_total += ms;
+        }
+        const long kdone = st.k - 1;
+        iters_run = kdone;
+        if (st.mix_pending) {
+            dev::launch_flush(ib_, st.cur, st.mix_w, s_);
+            st.mix_pending = 0;
+        }
+        if (kdone <= 0 || kdone % CHECK_EVERY != 0) break;  // MAXIT reached
+        const int cur = st.cur;
+        ++checks;
+
+        // ---- check (pdhg.cu:801-1084)
+        Metrics mc, ma;
+        check_metrics(cur, mc, ma);
+        const double ec = mc.finite_res() ? mc.err() : INF;
+        const double ea = ma.finite_res() ? ma.err() : INF;
+        last = (ec <= ea) ? mc : ma;
+        last_is_avg = (ea < ec);
+        if (opt_.verbosity > 0 && kdone % (CHECK_EVERY * 40) == 0)
+            std::fprintf(stderr, "[pdhg] k=%ld pinf=%.3e dinf=%.3e gap=%.3e\n",
+                         kdone, last.pinf, last.dinf, last.gap);
+        if (passes(mc, TOL)) { last = mc; last_is_avg = false; break; }
+        if (passes(ma, TOL)) { last = ma; last_is_avg = true; break; }
+
+        if (kdone >= 5000 && kdone >= next_repair) {  // pdhg.cu:854-931
+            next_repair = kdone + repair_spacing;
+            repair_spacing = std::min(repair_spacing * 3, 30000L);
+            for (int cc = 1; cc <= 2 && m_ <= cfg_.max_repair_m; ++cc) {
+                const Metrics& t = cc == 1 ? ma : inc;
+                if (cc == 2 && !have_inc) break;
+                if (!(t.pinf <= TOL && t.prow <= TOL && t.dinf <= 10.0 * TOL &&
+                      !(std::isfinite(t.gap) && t.gap <= TOL)))
+                    continue;
+                std::vector<double> sx(n_), sy(m_), rx(n_), ry_it(m_), ry(m_);
+                if (cc == 1) {
+                    full(tmpn_.p, F32 ? xref_.p : nullptr, xavg_.p, n_);
+                    full(tmpm_.p, F32 ? yref_.p : nullptr, yavg_.p, m_);
+                } else {
+                    copy_d(tmpn_.p, xb_.p, n_);
+                    full(tmpm_.p, F32 ? yref_.p : nullptr, ycur(cur), m_);
+                }
+                HIPCK(hipMemcpyAsync(sx.data(), tmpn_.p, sizeof(double) * n_, hipMemcpyDeviceToHost, s_));
+                HIPCK(hipMemcpyAsync(sy.data(), tmpm_.p, sizeof(double) * m_, hipMemcpyDeviceToHost, s_));
+                std::vector<double> hdc(n_), hdr(m_);
+                HIPCK(hipMemcpyAsync(hdc.data(), dc_.p, sizeof(double) * n_, hipMemcpyDeviceToHost, s_));
+                HIPCK(hipMemcpyAsync(hdr.data(), dr_.p, sizeof(double) * m_, hipMemcpyDeviceToHost, s_));
+                sync();
+                for (int j = 0; j < n_; ++j) rx[j] = sx[j] * hdc[j];
+                for (int i = 0; i < m_; ++i) ry_it[i] = 1.0 * sy[i] / hdr[i];
+                std::vector<double> rAx(m_, 0.0), rd(n_);
+                for (int j = 0; j < n_; ++j)
+                    for (int p = lp_.cp[j]; p < lp_.cp[j + 1]; ++p)
+                        rAx[lp_.ci[p]] += lp_.acx[p] * rx[j];
+                static const double ctol[2] = {1e-6, 1e-1};
+                bool ok_rep = false;
+                Metrics mr{};
+                for (int tier = 0; tier < 2 && !ok_rep; ++tier) {
+                    if (!host_dual_repair(lp_, rx.data(), rAx.data(), ry_it.data(), ctol[tier], ry.data()))
+                        continue;
+                    for (int j = 0; j < n_; ++j) {
+                        double dj = lp_.c[j];
+                        for (int p = lp_.cp[j]; p < lp_.cp[j + 1]; ++p) dj += lp_.acx[p] * ry[lp_.ci[p]];
+                        rd[j] = dj;
+                    }
+                    const HostMetrics t2 = host_evaluate(lp_, rx.data(), ry.data(), rAx.data(), rd.data());
+                    if (t2.pinf <= TOL && t2.prow <= TOL && std::isfinite(t2.gap) && t2.gap <= TOL) {
+                        ok_rep = true;
+                        mr = Metrics{t2.pinf, t2.dinf, t2.gap, t2.op, t2.od, t2.prow};
+                    }
+                }
+                if (!ok_rep) continue;
+                rep_x = rx;
+                rep_y.resize(m_);
+                for (int i = 0; i < m_; ++i) rep_y[i] = 1.0 * (ry[i] * hdr[i] / 1.0) / hdr[i];
+                if (opt_.verbosity > 1)
+                    std::fprintf(stderr, "[pdhg] dual repair c=%d @k=%ld gap=%.3e\n", cc, kdone, mr.gap);
+                last = mr;
+                last_is_avg = (cc == 1);
+                repaired = true;
+                break;
+            }
+            if (repaired) break;
+        }
+
+        // incumbent (pdhg.cu:933-958)
+        int inc_src = -1;
+        for (int cc = 0; cc < 2; ++cc) {
+            const Metrics& t = cc ? ma : mc;
+            if (t.pinf <= 1e-4 && std::isfinite(t.op) && (!have_inc || t.op < inc_op)) {
+                have_inc = true;
+                inc_op = t.op;
+                inc = t;
+                inc_src = cc;
+            }
+        }
+        if (inc_src >= 0) {
+            const double* xr = F32 ? xref_.p : nullptr;
+            const double* yr = F32 ? yref_.p : nullptr;
+            full(xb_.p, xr, inc_src ? xavg_.p : xcur(cur), n_);
+            full(yb_.p, yr, inc_src ? yavg_.p : ycur(cur), m_);
+        }
+        // rollback (pdhg.cu:965-988)
+        if (mc.pinf >= 0.5 && have_inc && kdone - last_rollback > 2000) {
+            if constexpr (F32) {
+                copy_d(xref_.p, xb_.p, n_);
+                copy_d(yref_.p, yb_.p, m_);
+                for (V* p : {x0_.p, x1_.p, xavg_.p}) zero_v(p, n_);
+                for (V* p : {y0_.p, y1_.p, yavg_.p, axcur_.p}) zero_v(p, m_);
+                recompute_refs();
+                ++refinements;
+            } else {
+                copy_v(xcur(cur), xb_.p, n_);
+                copy_v(ycur(cur), yb_.p, m_);
+                copy_v(xavg_.p, xb_.p, n_);
+                copy_v(yavg_.p, yb_.p, m_);
+                dev::spmv(rA_, ap_.p, ai_.p, sax_.p, xcur(cur), axcur_.p, s_);
+            }
+            st.eta = std::min(st.eta, 0.5 * eta0_);
+            st.hn = 0;
+            rs.epoch_start = kdone;
+            rs.epoch_kkt = -1.0;
+            rs.prev_cand = INF;
+            last_rollback = kdone;
+            if (opt_.verbosity > 1) std::fprintf(stderr, "[pdhg] rollback @k=%ld to inc\n", kdone);
+        }
+        // restart + primal weight (pdhg.cu:989-1075)
+        const double cand = std::min(ec, ea);
+        const long t_epoch = kdone - rs.epoch_start;
+        if (restart_decide(rs, cand, kdone)) {
+            ++restarts;
+            if constexpr (F32) {
+                // fold the anchor's correction into the FP64 reference
+                if (ea < ec) {
+                    dev::fold(xref_.p, xavg_.p, n_, s_);
+                    dev::fold(yref_.p, yavg_.p, m_, s_);
+                } else {
+                    dev::fold(xref_.p, xcur(cur), n_, s_);
+                    dev::fold(yref_.p, ycur(cur), m_, s_);
+                }
+                for (V* p : {x0_.p, x1_.p, xavg_.p}) zero_v(p, n_);
+                for (V* p : {y0_.p, y1_.p, yavg_.p, axcur_.p}) zero_v(p, m_);
+                recompute_refs();
+                ++refinements;
+            } else {
+                if (ea < ec) {
+                    copy_v(xcur(cur), xavg_.p, n_);
+                    copy_v(ycur(cur), yavg_.p, m_);
+                }
+                copy_v(xavg_.p, xcur(cur), n_);
+                copy_v(yavg_.p, ycur(cur), m_);
+                dev::spmv(rA_, ap_.p, ai_.p, sax_.p, xcur(cur), axcur_.p, s_);
+            }
+            st.hn = 0;
+            rs.epoch_start = kdone;
+            rs.epoch_kkt = cand;
+            rs.prev_cand = INF;
+            const double* sxp;
+            const double* syp;
+            if constexpr (F32) {
+                sxp = xref_.p;
+                syp = yref_.p;
+            } else {
+                sxp = xcur(cur);
+                syp = ycur(cur);
+            }
+            if (have_pe) {
+                const double dxn = dsum(sxp, pesx_.p, n_);
+                const double dyn = dsum(syp, pesy_.p, m_);
+                st.omega = primal_weight(st.omega, dxn, dyn);
+            }
+            copy_d(pesx_.p, sxp, n_);
+            copy_d(pesy_.p, syp, m_);
+            have_pe = true;
+            if (opt_.verbosity > 1)
+                std::fprintf(stderr, "[pdhg] restart @k=%ld t=%ld r=%.3e eta=%.3e om=%.3e\n",
+                             kdone, t_epoch, cand, st.eta, st.omega);
+        }
+        if (now_s() - t0 > opt_.time_limit_s) {
+            sol.status = Status::TimeLimit;
+            sol.message = "wall-clock budget exhausted";
+            time_limit = true;
+            break;
+        }
+        if (kdone >= MAXIT) break;
+        st.frozen = 0;
+        st.fresh = 1;
+        st.retries = 0;
+        write_state(st);
+    }
+    sync();
+    const double secs = now_s() - t0;
+
+    if (!time_limit) {
+        if (certified(last, TOL)) {
+            sol.status = Status::NearOptimal;
+            sol.message = "converged: residuals + duality gap @tol";
+        } else {
+            sol.status = Status::IterationLimit;
+            sol.message = last.pinf <= TOL && last.dinf <= TOL
+                              ? "iteration cap: residuals @tol but gap not certified (dual bound -inf or > tol)"
+                              : "iteration cap reached before tolerance";
+        }
+    }
+    int src = last_is_avg ? 1 : 0;  // 0 current, 1 average, 2 incumbent
+    bool used_inc = false;
+    if (!certified(last, TOL) && have_inc && inc.pinf <= TOL && std::isfinite(inc.op) &&
+        (inc.op < last.op || last.pinf > TOL)) {
+        last = inc;
+        last_is_avg = false;
+        src = 2;
+        used_inc = true;
+    }
+    sol.objective = last.op + lp_.obj_const;
+    sol.pinf = last.pinf;
+    sol.dinf = last.dinf;
+    sol.rel_gap = last.gap;
+    sol.iterations = iters_run;
+    sol.solve_time_ms = secs * 1000.0;
+    sol.x.assign(n_, 0.0);
+    sol.y.assign(m_, 0.0);
+    if (repaired && !used_inc) {
+        sol.x = rep_x;
+        sol.y = rep_y;
+    } else {
+        DBuf<double> ox, oy;
+        ox.alloc(n_);
+        oy.alloc(m_);
+        const double* xr = F32 ? xref_.p : nullptr;
+        const double* yr = F32 ? yref_.p : nullptr;
+        if (src == 2) {
+            dev::unscale_out(m_, n_, dc_.p, dr_.p, nullptr, xb_.p, nullptr, nullptr, yb_.p, nullptr, ox.p, oy.p, s_);
+        } else {
+            const V* fx = src == 1 ? xavg_.p : xcur(st.cur);
+            const V* fy = src == 1 ? yavg_.p : ycur(st.cur);
+            if constexpr (F32)
+                dev::unscale_out(m_, n_, dc_.p, dr_.p, xr, nullptr, fx, yr, nullptr, fy, ox.p, oy.p, s_);
+            else
+                dev::unscale_out(m_, n_, dc_.p, dr_.p, nullptr, fx, nullptr, nullptr, fy, nullptr, ox.p, oy.p, s_);
+        }
+        if (n_) HIPCK(hipMemcpyAsync(sol.x.data(), ox.p, sizeof(double) * n_, hipMemcpyDeviceToHost, s_));
+        if (m_) HIPCK(hipMemcpyAsync(sol.y.data(), oy.p, sizeof(double) * m_, hipMemcpyDeviceToHost, s_));
+        sync();
+    }
+    if (info) {
+        info->setup_ms = (t_setup1 - t_setup0) * 1000.0;
+        info->solve_ms = secs * 1000.0;
+        info->attempts = static_cast<long>(st.attempts);
+        info->restarts = restarts;
+        info->checks = checks;
+        info->refinements = refinements;
+        info->eta = st.eta;
+        info->omega = st.omega;
+        info->lam2 = lam2_;
+        info->eta0 = eta0_;
+        info->omega0 = omega0_;
+        info->last_is_avg = last_is_avg ? 1 : 0;
+        info->used_incumbent = used_inc ? 1 : 0;
+        info->repaired = repaired ? 1 : 0;
+        info->kernel_ms = kernel_ms_total;
+    }
+    return sol;
+}
+
+}  // namespace
+
+Solution solve_ex(const io::Model& model, const Options& options,
+                  const EngineConfig& cfg, SolveInfo* info) {
+    Solution sol;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg.device) {
+        (void)hipGetLastError();
+        sol.status = Status::Error;
+        sol.message = "PDHG engine unavailable: no HIP GPU device visible "
+                      "(this build runs the engine on MI355X via HIP, not CUDA)";
+        return sol;
+    }
+    if (model.m < 0 || model.n < 0 || static_cast<int>(model.ap.size()) != model.m + 1 ||
+        static_cast<int>(model.cp.size()) != model.n + 1) {
+        sol.status = Status::Error;
+        sol.message = "malformed model: CSR/CSC pointer sizes do not match m, n";
+        return sol;
+    }
+    try {
+        if (cfg.precision == 32) {
+            Engine<float> e(model, options, cfg);
+            return e.run(info);
+        }
+        Engine<double> e(model, options, cfg);
+        return e.run(info);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "[pdhg] %s\n", e.what());
+        sol = Solution{};
+        sol.status = Status::Error;
+        sol.message = e.what();
+        return sol;
+    }
+}
+
+Solution solve(const io::Model& model, const Options& options) {
+    return solve_ex(model, options, EngineConfig{}, nullptr);
+}
+
+}  // namespace igaos::pdhg

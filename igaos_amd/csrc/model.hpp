@@ -1,0 +1,52 @@
+#pragma once
+// LP model container and MPS reader of the pdhg engine slot.
+//
+// io::Model has the same fields and meaning as the reference's
+// src/io/model.hpp:16-37: min c'x + obj_const s.t. rmin <= Ax <= rmax,
+// cl <= x <= cu, with A stored both as CSR (ap/ai/ax) and CSC (cp/ci/acx),
+// FP64 values, int32 indices and +-inf for missing bounds.  The engine
+// borrows it read-only.
+
+#include <string>
+#include <vector>
+
+namespace igaos::io {
+
+struct Model {
+    int m = 0, n = 0;
+    std::vector<double> rmin, rmax;
+    std::vector<double> cl, cu;
+    std::vector<double> c;
+    std::vector<int> ap, ai;
+    std::vector<double> ax;
+    std::vector<int> cp, ci;
+    std::vector<double> acx;
+    std::vector<unsigned char> integ;
+    double obj_const = 0.0;
+    int n_fr_parsed = 0;
+    int n_ranges_parsed = 0;
+    std::vector<int> q_i, q_j;
+    std::vector<double> q_v;
+
+    bool has_quadratic() const { return !q_v.empty(); }
+    int nnz() const { return static_cast<int>(ax.size()); }
+};
+
+struct Triplet {
+    int row, col;
+    double val;
+};
+
+// Triplets -> CSR + CSC by two stable bucket sorts: entries keep their
+// input order inside each row and each column (src/io/model.hpp:39-67).
+void counting_sort(const std::vector<Triplet>& trips, int rows, int cols,
+                   std::vector<int>& ap, std::vector<int>& ai,
+                   std::vector<double>& ax, std::vector<int>& cp,
+                   std::vector<int>& ci, std::vector<double>& acx);
+
+// Free-format MPS reader with the reference's section semantics
+// (src/io/model.hpp:69-218).  Throws std::runtime_error on I/O errors and
+// std::out_of_range for a BOUNDS entry naming an unknown column.
+void parse_mps(const std::string& path, Model& model);
+
+}  // namespace igaos::io

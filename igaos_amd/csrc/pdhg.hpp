@@ -1,0 +1,48 @@
+#pragma once
+// The pdhg engine slot: igaos::pdhg::solve, same signature and semantics
+// as the reference's src/pdhg/pdhg.hpp:9 (dispatched by
+// solve_with_engine, src/api/engine.hpp:32-38), backed by HIP kernels on
+// one MI355X.  solve_ex adds engine-private knobs without touching Options.
+
+#include "igaos/options.h"
+#include "igaos/solution.h"
+#include "model.hpp"
+
+namespace igaos::pdhg {
+
+struct EngineConfig {
+    int precision = 64;      // 64: plain FP64 engine; 32: FP32 inner + FP64 refinement
+    int graph_slots = 16;    // attempts captured per hipGraph replay (0 = no graph)
+    int max_repair_m = 4096; // host dense dual repair only up to this many rows
+    int device = 0;
+};
+
+struct SolveInfo {
+    double setup_ms = 0.0;     // upload, scaling, power iteration
+    double solve_ms = 0.0;     // iteration loop (the reference's solve_time_ms)
+    long attempts = 0;         // primal/dual attempts incl. line-search retries
+    long restarts = 0;
+    long checks = 0;
+    long refinements = 0;      // FP64 re-anchors of the FP32 correction problem
+    double eta = 0.0, omega = 0.0, lam2 = 0.0, eta0 = 0.0, omega0 = 0.0;
+    int last_is_avg = 0;
+    int used_incumbent = 0;
+    int repaired = 0;
+    double kernel_ms = 0.0;    // device time of the attempt kernels (events)
+};
+
+Solution solve(const io::Model& model, const Options& options);
+Solution solve_ex(const io::Model& model, const Options& options,
+                  const EngineConfig& cfg, SolveInfo* info);
+
+// Host-side KKT evaluation and dense dual repair in original units
+// (src/pdhg/pdhg.cu:173-254, 288-448); used at check time only.
+struct HostMetrics {
+    double pinf, dinf, gap, op, od, prow;
+};
+HostMetrics host_evaluate(const io::Model& lp, const double* x,
+                          const double* y, const double* Ax, const double* d);
+bool host_dual_repair(const io::Model& lp, const double* x, const double* Ax,
+                      const double* y_it, double col_tol, double* y);
+
+}  // namespace igaos::pdhg

@@ -1,0 +1,764 @@
+// Single-LP PDHG kernels for gfx950: row-block SpMV, the fused primal and
+// dual steps with the on-device line-search decision, KKT evaluation in
+// original units, Ruiz equilibration and fixed-order reductions.
+//
+// Per accepted iteration the engine launches two kernels:
+//   k_primal : A'y (transposed CSR = CSC of A) fused with the primal step
+//              x' = clamp(x - tau (g + A'y), lo, hi), xbar = 2x' - x and the
+//              |dx|^2 partial (src/pdhg/pdhg.cu:32-45), plus the pending
+//              averaging of x (pdhg.cu:127-141);
+//   k_dual   : A xbar fused with the dual step in Moreau form
+//              y' = t - sigma clamp((t + yoff)/sigma, rlo, rhi),
+//              t = y + sigma A xbar (pdhg.cu:47-61), Adx = (A xbar - Ax)/2,
+//              the |dy|^2 and dy'Adx partials, the pending averaging of y
+//              and the A x commit; its last workgroup reduces all partials
+//              in a fixed order and applies the line-search rule
+//              (pdhg.cu:751-794) to the device-resident LsState.
+// A rejected attempt re-runs both kernels from the same (x, y); k_primal
+// then reuses the stored A'y.
+
+#include "dev_common.hpp"
+#include "pdhg_kernels.hpp"
+
+namespace igaos::dev {
+
+using pdhg::LsState;
+using pdhg::Metrics;
+
+std::vector<int> build_row_blocks(const int* ptr, int nrows) {
+    std::vector<int> rb;
+    rb.reserve(static_cast<size_t>(nrows) / 8 + 2);
+    rb.push_back(0);
+    int r = 0;
+    while (r < nrows) {
+        const int len0 = ptr[r + 1] - ptr[r];
+        if (len0 > NNZ_MAX) {
+            ++r;
+            rb.push_back(r);
+            continue;
+        }
+        long nz = 0;
+        int cnt = 0;
+        while (r < nrows && cnt < ROWS_MAX) {
+            const int len = ptr[r + 1] - ptr[r];
+            if (len > NNZ_MAX || nz + len > NNZ_MAX) break;
+            nz += len;
+            ++cnt;
+            ++r;
+        }
+        rb.push_back(r);
+    }
+    return rb;
+}
+
+__device__ __forceinline__ void block_rows(const RowBlocks& rb, int blk,
+                                           int& r0, int& r1) {
+    if (blk < rb.nblk) {
+        r0 = rb.rb[blk];
+        r1 = rb.rb[blk + 1];
+    } else {
+        r0 = r1 = 0;
+    }
+}
+
+static inline int grid_of(int nblk) { return nblk > 0 ? nblk : 1; }
+
+// ---------------------------------------------------------------- SpMV --
+
+template <typename Acc, typename Val, typename G>
+__global__ __launch_bounds__(BS) void k_spmv(RowBlocks rb,
+                                             const int* __restrict__ ptr,
+                                             const int* __restrict__ idx,
+                                             const Val* __restrict__ val,
+                                             G gx, Acc* __restrict__ y) {
+    __shared__ Acc s_prod[NNZ_MAX];
+    __shared__ Acc s_rows[ROWS_MAX];
+    __shared__ Acc red[BS / WAVE];
+    int r0, r1;
+    block_rows(rb, blockIdx.x, r0, r1);
+    rowblock_sums<Acc>(ptr, idx, val, gx, r0, r1, s_rows, s_prod, red);
+    for (int t = threadIdx.x; t < r1 - r0; t += BS) y[r0 + t] = s_rows[t];
+}
+
+void spmv(const RowBlocks& rb, const int* ptr, const int* idx,
+          const double* val, const double* x, double* y, hipStream_t s) {
+    if (rb.nblk == 0) return;
+    k_spmv<double, double><<<rb.nblk, BS, 0, s>>>(rb, ptr, idx, val,
+                                                   GatherPlain<double>{x}, y);
+}
+void spmv(const RowBlocks& rb, const int* ptr, const int* idx,
+          const float* val, const float* x, float* y, hipStream_t s) {
+    if (rb.nblk == 0) return;
+    k_spmv<float, float><<<rb.nblk, BS, 0, s>>>(rb, ptr, idx, val,
+                                                 GatherPlain<float>{x}, y);
+}
+void spmv_ref(const RowBlocks& rb, const int* ptr, const int* idx,
+              const double* val, const double* xref, const float* xd,
+              double* y, hipStream_t s) {
+    if (rb.nblk == 0) return;
+    k_spmv<double, double><<<rb.nblk, BS, 0, s>>>(
+        rb, ptr, idx, val, GatherRef<float>{xref, xd}, y);
+}
+
+// ----------------------------------------------------- primal / dual --
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_primal(IterBufs<V> b) {
+    __shared__ V s_prod[NNZ_MAX];
+    __shared__ V s_rows[ROWS_MAX];
+    __shared__ V red_v[BS / WAVE];
+    __shared__ double red[BS / WAVE];
+    const LsState st = *b.st;
+    if (st.frozen) return;
+    const int cur = st.cur;
+    const V tau = static_cast<V>(st.eta / st.omega);
+    const V w = static_cast<V>(st.mix_w);
+    const bool fresh = st.fresh != 0;
+    const bool mix = fresh && st.mix_pending != 0;
+    const V* __restrict__ x = b.x[cur];
+    V* __restrict__ xn = b.x[cur ^ 1];
+    const V* __restrict__ y = b.y[cur];
+    int r0, r1;
+    block_rows(b.rbT, blockIdx.x, r0, r1);
+    if (fresh)
+        rowblock_sums<V>(b.cp, b.ci, b.acx, GatherPlain<V>{y}, r0, r1, s_rows,
+                         s_prod, red_v);
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < r1 - r0; t += BS) {
+        const int j = r0 + t;
+        V a;
+        if (fresh) {
+            a = s_rows[t];
+            b.ata[j] = a;
+        } else {
+            a = b.ata[j];
+        }
+        const V xo = x[j];
+        if (mix) {
+            const V xa = b.xavg[j];
+            b.xavg[j] = xa + w * (xo - xa);
+        }
+        const V g = b.gref[j] + a;
+        const V v = fmin(fmax(xo - tau * g, b.lo[j]), b.hi[j]);
+        xn[j] = v;
+        b.xbar[j] = V(2) * v - xo;
+        const V d = v - xo;
+        acc += static_cast<double>(d) * static_cast<double>(d);
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) store_agent(b.partT + blockIdx.x, acc);
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_dual(IterBufs<V> b) {
+    __shared__ V s_prod[NNZ_MAX];
+    __shared__ V s_rows[ROWS_MAX];
+    __shared__ V red_v[BS / WAVE];
+    __shared__ double red[BS / WAVE];
+    __shared__ int s_flag;
+    const LsState st = *b.st;
+    if (st.frozen) return;
+    const int cur = st.cur;
+    const V sig = static_cast<V>(st.eta * st.omega);
+    const V w = static_cast<V>(st.mix_w);
+    const bool mix = st.fresh != 0 && st.mix_pending != 0;
+    const V* __restrict__ y = b.y[cur];
+    V* __restrict__ yn = b.y[cur ^ 1];
+    int r0, r1;
+    block_rows(b.rbA, blockIdx.x, r0, r1);
+    rowblock_sums<V>(b.ap, b.ai, b.ax, GatherPlain<V>{b.xbar}, r0, r1, s_rows,
+                     s_prod, red_v);
+    double a1 = 0.0, a2 = 0.0;
+    for (int t = threadIdx.x; t < r1 - r0; t += BS) {
+        const int i = r0 + t;
+        const V tm = s_rows[t];
+        const V yo = y[i];
+        V axc = b.axcur[i];
+        if (mix) {
+            const V ya = b.yavg[i];
+            b.yavg[i] = ya + w * (yo - ya);
+            axc = axc + b.adx[i];
+            b.axcur[i] = axc;
+        }
+        const V tt = yo + sig * tm;
+        V u = b.yoff ? tt + b.yoff[i] : tt;
+        u = u / sig;
+        const V yv = tt - sig * fmin(fmax(u, b.rlo[i]), b.rhi[i]);
+        yn[i] = yv;
+        const V dy = yv - yo;
+        const V ad = V(0.5) * (tm - axc);
+        b.adx[i] = ad;
+        a1 += static_cast<double>(dy) * static_cast<double>(dy);
+        a2 += static_cast<double>(dy) * static_cast<double>(ad);
+    }
+    a1 = block_sum(a1, red);
+    a2 = block_sum(a2, red);
+    const int nA = gridDim.x;
+    if (threadIdx.x == 0) {
+        store_agent(b.partA + blockIdx.x, a1);
+        store_agent(b.partA + nA + blockIdx.x, a2);
+    }
+    if (!elect_last(b.ticket, &s_flag)) return;
+    const double d0 = fixed_sum(b.partT, b.rbT.nblk, red);
+    const double d1 = fixed_sum(b.partA, nA, red);
+    const double d2 = fixed_sum(b.partA + nA, nA, red);
+    if (threadIdx.x == 0) {
+        LsState s = *b.st;
+        pdhg::ls_decide(s, d0, d1, d2);
+        *b.st = s;
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_flush(int n, int m, const V* x,
+                                              V* xavg, const V* y, V* yavg,
+                                              const V* adx, V* axcur, V w) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) xavg[i] += w * (x[i] - xavg[i]);
+    if (i < m) {
+        yavg[i] += w * (y[i] - yavg[i]);
+        axcur[i] += adx[i];
+    }
+}
+
+template <typename V>
+static void attempt_impl(const IterBufs<V>& b, hipStream_t s) {
+    if (b.rbT.nblk > 0) k_primal<V><<<b.rbT.nblk, BS, 0, s>>>(b);
+    k_dual<V><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
+}
+void launch_attempt(const IterBufs<double>& b, hipStream_t s) { attempt_impl(b, s); }
+void launch_attempt(const IterBufs<float>& b, hipStream_t s) { attempt_impl(b, s); }
+
+template <typename V>
+static void flush_impl(const IterBufs<V>& b, int cur, double w, hipStream_t s) {
+    const long mx = b.n > b.m ? b.n : b.m;
+    if (mx == 0) return;
+    k_flush<V><<<(mx + BS - 1) / BS, BS, 0, s>>>(b.n, b.m, b.x[cur], b.xavg,
+                                                  b.y[cur], b.yavg, b.adx,
+                                                  b.axcur, static_cast<V>(w));
+}
+void launch_flush(const IterBufs<double>& b, int cur, double w, hipStream_t s) { flush_impl(b, cur, w, s); }
+void launch_flush(const IterBufs<float>& b, int cur, double w, hipStream_t s) { flush_impl(b, cur, w, s); }
+
+// ------------------------------------------------------------------ KKT --
+// Column pass: d = c + (A~'y~)/dc, dinf numerator, c'x and the column part
+// of the Lagrangian bound (pdhg.cu:206-233).  Row pass: Ax = dr (A~x~),
+// pinf / prow numerators and max|y| (pdhg.cu:180-201, 229).  Second row
+// pass: the row part of the bound, which needs max|y| (pdhg.cu:234-244).
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_kkt_cols(KktData d, KktIter<V> it) {
+    __shared__ double s_prod[NNZ_MAX];
+    __shared__ double s_rows[ROWS_MAX];
+    __shared__ double red[BS / WAVE];
+    const int q = blockIdx.y;
+    int r0, r1;
+    block_rows(d.rbT, blockIdx.x, r0, r1);
+    rowblock_sums<double>(d.cp, d.ci, d.sacx, GatherRef<V>{it.yref[q], it.yd[q]},
+                          r0, r1, s_rows, s_prod, red);
+    const GatherRef<V> gx{it.xref[q], it.xd[q]};
+    const double dz = 1e-8 * (1.0 + d.cmax);
+    double dv = 0.0, op = 0.0, od = 0.0, unb = 0.0;
+    for (int t = threadIdx.x; t < r1 - r0; t += BS) {
+        const int j = r0 + t;
+        const double ata = s_rows[t];
+        if (it.ata_out[q]) it.ata_out[q][j] = ata;
+        const double hx = gx(j) * d.dc[j];
+        const double hd = d.c[j] + ata / d.dc[j];
+        const double cl = d.cl[j], cu = d.cu[j];
+        const double bnd = fabs(__builtin_isfinite(cu) ? cu : cl);
+        const double tol = 1e-9 * (1.0 + bnd);
+        double v;
+        if (hx <= cl + tol && hd > 0) v = 0.0;
+        else if (hx >= cu - tol && hd < 0) v = 0.0;
+        else v = fabs(hd);
+        dv = fmax(dv, v);
+        op += d.c[j] * hx;
+        if (hd >= 0) {
+            if (__builtin_isfinite(cl)) od += hd * cl;
+            else if (hd > dz) unb = 1.0;
+        } else {
+            if (__builtin_isfinite(cu)) od += hd * cu;
+            else if (-hd > dz) unb = 1.0;
+        }
+    }
+    dv = block_max(dv, red);
+    op = block_sum(op, red);
+    od = block_sum(od, red);
+    unb = block_max(unb, red);
+    if (threadIdx.x == 0) {
+        double* p = d.partC + (static_cast<long>(q) * gridDim.x + blockIdx.x) * 4;
+        store_agent(p + 0, dv);
+        store_agent(p + 1, op);
+        store_agent(p + 2, od);
+        store_agent(p + 3, unb);
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_kkt_rows(KktData d, KktIter<V> it) {
+    __shared__ double s_prod[NNZ_MAX];
+    __shared__ double s_rows[ROWS_MAX];
+    __shared__ double red[BS / WAVE];
+    __shared__ int s_flag;
+    const int q = blockIdx.y;
+    int r0, r1;
+    block_rows(d.rbA, blockIdx.x, r0, r1);
+    rowblock_sums<double>(d.ap, d.ai, d.sax, GatherRef<V>{it.xref[q], it.xd[q]},
+                          r0, r1, s_rows, s_prod, red);
+    const GatherRef<V> gy{it.yref[q], it.yd[q]};
+    double pv = 0.0, pa = 0.0, prow = 0.0, ymax = 0.0;
+    for (int t = threadIdx.x; t < r1 - r0; t += BS) {
+        const int i = r0 + t;
+        const double tm = s_rows[t];
+        if (it.tm_out[q]) it.tm_out[q][i] = tm;
+        const double ax = tm * d.dr[i];
+        const double lo = d.rmin[i], hi = d.rmax[i];
+        const double e = fmax(lo - ax, ax - hi);
+        pv = fmax(pv, e);
+        pa = fmax(pa, fabs(fmin(fmax(ax, lo), hi)));
+        const double viol = fmax(0.0, e);
+        double sc = fabs(ax);
+        if (__builtin_isfinite(lo)) sc = fmax(sc, fabs(lo));
+        if (__builtin_isfinite(hi)) sc = fmax(sc, fabs(hi));
+        prow = fmax(prow, viol / (1.0 + sc));
+        const double hy = gy(i) * (1.0 / d.dr[i]);
+        ymax = fmax(ymax, fabs(hy));
+    }
+    pv = block_max(pv, red);
+    pa = block_max(pa, red);
+    prow = block_max(prow, red);
+    ymax = block_max(ymax, red);
+    const int nb = gridDim.x;
+    if (threadIdx.x == 0) {
+        double* p = d.partR + (static_cast<long>(q) * nb + blockIdx.x) * 4;
+        store_agent(p + 0, pv);
+        store_agent(p + 1, pa);
+        store_agent(p + 2, prow);
+        store_agent(p + 3, ymax);
+    }
+    if (!elect_last(d.tickets + 0, &s_flag)) return;
+    for (int qq = 0; qq < 2; ++qq) {
+        double r[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; ++k) {
+            double s = 0.0;
+            for (int bb = threadIdx.x; bb < nb; bb += BS)
+                s = fmax(s, load_agent(d.partR + (static_cast<long>(qq) * nb + bb) * 4 + k));
+            r[k] = block_max(s, red);
+        }
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 4; ++k) store_agent(d.rowres + qq * 4 + k, r[k]);
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_kkt_final(KktData d, KktIter<V> it,
+                                                  int nbC, Metrics* out) {
+    __shared__ double red[BS / WAVE];
+    __shared__ int s_flag;
+    const int q = blockIdx.y;
+    const double ymax_q = d.rowres[q * 4 + 3];
+    const double yz = 1e-8 * (1.0 + ymax_q);
+    const GatherRef<V> gy{it.yref[q], it.yd[q]};
+    double od = 0.0, unb = 0.0;
+    for (long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x; i < d.m;
+         i += static_cast<long>(gridDim.x) * BS) {
+        const double hy = gy(static_cast<int>(i)) * (1.0 / d.dr[i]);
+        if (hy > yz) {
+            if (__builtin_isfinite(d.rmax[i])) od -= hy * d.rmax[i];
+            else unb = 1.0;
+        } else if (hy < -yz) {
+            if (__builtin_isfinite(d.rmin[i])) od -= hy * d.rmin[i];
+            else unb = 1.0;
+        }
+    }
+    od = block_sum(od, red);
+    unb = block_max(unb, red);
+    const int nb = gridDim.x;
+    if (threadIdx.x == 0) {
+        double* p = d.partR2 + (static_cast<long>(q) * nb + blockIdx.x) * 2;
+        store_agent(p + 0, od);
+        store_agent(p + 1, unb);
+    }
+    if (!elect_last(d.tickets + 1, &s_flag)) return;
+    for (int qq = 0; qq < 2; ++qq) {
+        double dv = 0.0, op = 0.0, odc = 0.0, unbc = 0.0, odr = 0.0, unbr = 0.0;
+        for (int bb = threadIdx.x; bb < nbC; bb += BS) {
+            const double* p = d.partC + (static_cast<long>(qq) * nbC + bb) * 4;
+            dv = fmax(dv, load_agent(p + 0));
+            op += load_agent(p + 1);
+            odc += load_agent(p + 2);
+            unbc = fmax(unbc, load_agent(p + 3));
+        }
+        for (int bb = threadIdx.x; bb < nb; bb += BS) {
+            const double* p = d.partR2 + (static_cast<long>(qq) * nb + bb) * 2;
+            odr += load_agent(p + 0);
+            unbr = fmax(unbr, load_agent(p + 1));
+        }
+        dv = block_max(dv, red);
+        op = block_sum(op, red);
+        odc = block_sum(odc, red);
+        unbc = block_max(unbc, red);
+        odr = block_sum(odr, red);
+        unbr = block_max(unbr, red);
+        if (threadIdx.x == 0) {
+            const double pv = load_agent(d.rowres + qq * 4 + 0);
+            const double pa = load_agent(d.rowres + qq * 4 + 1);
+            const double prow = load_agent(d.rowres + qq * 4 + 2);
+            Metrics mt;
+            mt.pinf = fmax(pv, 0.0) / (1.0 + pa);
+            mt.prow = prow;
+            mt.dinf = dv / (1.0 + d.cmax);
+            mt.op = op;
+            const bool bounded = unbc == 0.0 && unbr == 0.0;
+            const double odv = odc + odr;
+            mt.od = bounded ? odv : -__builtin_huge_val();
+            mt.gap = bounded ? fmax(0.0, op - odv) / (1.0 + fabs(op))
+                             : __builtin_huge_val();
+            out[qq] = mt;
+        }
+    }
+}
+
+static int kkt_rows2_blocks(int m) {
+    long b = (static_cast<long>(m) + BS * 4 - 1) / (BS * 4);
+    if (b < 1) b = 1;
+    if (b > 1024) b = 1024;
+    return static_cast<int>(b);
+}
+
+template <typename V>
+static void kkt_impl(const KktData& d, const KktIter<V>& it, Metrics* out,
+                     hipStream_t s) {
+    const int nbC = grid_of(d.rbT.nblk);
+    const int nbR = grid_of(d.rbA.nblk);
+    k_kkt_cols<V><<<dim3(nbC, 2), BS, 0, s>>>(d, it);
+    k_kkt_rows<V><<<dim3(nbR, 2), BS, 0, s>>>(d, it);
+    k_kkt_final<V><<<dim3(kkt_rows2_blocks(d.m), 2), BS, 0, s>>>(d, it, nbC, out);
+}
+void launch_kkt(const KktData& d, const KktIter<double>& it, Metrics* out,
+                hipStream_t s) { kkt_impl(d, it, out, s); }
+void launch_kkt(const KktData& d, const KktIter<float>& it, Metrics* out,
+                hipStream_t s) { kkt_impl(d, it, out, s); }
+
+// ------------------------------------------------------- Ruiz / scale --
+// One wavefront per row; a row's maximum is exact in any order, so the
+// factors match a sequential host loop bit for bit.
+
+__global__ __launch_bounds__(BS) void k_ruiz_rows(int m, const int* ap,
+                                                  const int* ai,
+                                                  const double* ax,
+                                                  const double* dc,
+                                                  double* dr) {
+    const long wid = (static_cast<long>(blockIdx.x) * BS + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (wid >= m) return;
+    const int i = static_cast<int>(wid);
+    const double dri = dr[i];
+    double mx = 0.0;
+    for (int p = ap[i] + lane; p < ap[i + 1]; p += WAVE)
+        mx = fmax(mx, fabs(ax[p]) * dc[ai[p]] / dri);
+    mx = wave_max(mx);
+    if (lane == 0 && mx > 0 && __builtin_isfinite(mx)) dr[i] = dri * sqrt(mx);
+}
+
+__global__ __launch_bounds__(BS) void k_ruiz_cols(int n, const int* cp,
+                                                  const int* ci,
+                                                  const double* acx,
+                                                  const double* dr,
+                                                  double* dc) {
+    const long wid = (static_cast<long>(blockIdx.x) * BS + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (wid >= n) return;
+    const int j = static_cast<int>(wid);
+    const double dcj = dc[j];
+    double mx = 0.0;
+    for (int p = cp[j] + lane; p < cp[j + 1]; p += WAVE)
+        mx = fmax(mx, fabs(acx[p]) / dr[ci[p]] * dcj);
+    mx = wave_max(mx);
+    if (lane == 0 && mx > 0 && __builtin_isfinite(mx)) dc[j] = dcj / sqrt(mx);
+}
+
+__global__ void k_fix_ones(double* v, long len) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < len && !(v[i] > 0 && __builtin_isfinite(v[i]))) v[i] = 1.0;
+}
+
+static int wave_grid(long rows) {
+    return static_cast<int>((rows * WAVE + BS - 1) / BS);
+}
+static int elem_grid(long len) { return static_cast<int>((len + BS - 1) / BS); }
+
+void fill(double* v, long n, double val, hipStream_t s);
+
+void ruiz(int m, int n, const int* ap, const int* ai, const double* ax,
+          const int* cp, const int* ci, const double* acx, double* dr,
+          double* dc, hipStream_t s) {
+    fill(dr, m, 1.0, s);
+    fill(dc, n, 1.0, s);
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        if (m > 0) k_ruiz_rows<<<wave_grid(m), BS, 0, s>>>(m, ap, ai, ax, dc, dr);
+        if (n > 0) k_ruiz_cols<<<wave_grid(n), BS, 0, s>>>(n, cp, ci, acx, dr, dc);
+    }
+    if (m > 0) k_fix_ones<<<elem_grid(m), BS, 0, s>>>(dr, m);
+    if (n > 0) k_fix_ones<<<elem_grid(n), BS, 0, s>>>(dc, n);
+}
+
+// CSR: out = val * dc[col] / dr[row];  CSC: out = val * dc[col] / dr[row]
+// with the CSC "row" being the column (pdhg.cu:503-512 operand order).
+__global__ __launch_bounds__(BS) void k_scale_csr(int m, const int* ap,
+                                                  const int* ai,
+                                                  const double* ax,
+                                                  const double* dc,
+                                                  const double* dr,
+                                                  double* sax) {
+    const long wid = (static_cast<long>(blockIdx.x) * BS + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (wid >= m) return;
+    const int i = static_cast<int>(wid);
+    const double d = dr[i];
+    for (int p = ap[i] + lane; p < ap[i + 1]; p += WAVE)
+        sax[p] = ax[p] * dc[ai[p]] / d;
+}
+__global__ __launch_bounds__(BS) void k_scale_csc(int n, const int* cp,
+                                                  const int* ci,
+                                                  const double* acx,
+                                                  const double* dc,
+                                                  const double* dr,
+                                                  double* sacx) {
+    const long wid = (static_cast<long>(blockIdx.x) * BS + threadIdx.x) / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (wid >= n) return;
+    const int j = static_cast<int>(wid);
+    const double d = dc[j];
+    for (int p = cp[j] + lane; p < cp[j + 1]; p += WAVE)
+        sacx[p] = acx[p] * d / dr[ci[p]];
+}
+__global__ void k_scale_vecs(int m, int n, const double* dr, const double* dc,
+                             const double* c, const double* cl,
+                             const double* cu, const double* rmin,
+                             const double* rmax, double* sc, double* scl,
+                             double* scu, double* srmin, double* srmax) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) {
+        sc[i] = c[i] * dc[i] / 1.0;
+        scl[i] = cl[i] / dc[i];
+        scu[i] = cu[i] / dc[i];
+    }
+    if (i < m) {
+        srmin[i] = rmin[i] / dr[i];
+        srmax[i] = rmax[i] / dr[i];
+    }
+}
+
+void scale_data(int m, int n, const int* ap, const int* ai, const double* ax,
+                const int* cp, const int* ci, const double* acx,
+                const double* dr, const double* dc, const double* c,
+                const double* cl, const double* cu, const double* rmin,
+                const double* rmax, double* sax, double* sacx, double* sc,
+                double* scl, double* scu, double* srmin, double* srmax,
+                hipStream_t s) {
+    if (m > 0) k_scale_csr<<<wave_grid(m), BS, 0, s>>>(m, ap, ai, ax, dc, dr, sax);
+    if (n > 0) k_scale_csc<<<wave_grid(n), BS, 0, s>>>(n, cp, ci, acx, dc, dr, sacx);
+    const long mx = m > n ? m : n;
+    if (mx > 0)
+        k_scale_vecs<<<elem_grid(mx), BS, 0, s>>>(m, n, dr, dc, c, cl, cu, rmin,
+                                                  rmax, sc, scl, scu, srmin, srmax);
+}
+
+// ------------------------------------------------------- reductions --
+
+int reduce_blocks(long n) {
+    long b = (n + BS * 8 - 1) / (BS * 8);
+    if (b < 1) b = 1;
+    if (b > 1024) b = 1024;
+    return static_cast<int>(b);
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_sumsq_diff(const V* a, const V* b,
+                                                   long n, double* part,
+                                                   unsigned int* ticket,
+                                                   double* out) {
+    __shared__ double red[BS / WAVE];
+    __shared__ int s_flag;
+    double s = 0.0;
+    for (long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * BS) {
+        const double d = b ? static_cast<double>(a[i]) - static_cast<double>(b[i])
+                           : static_cast<double>(a[i]);
+        s += d * d;
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) store_agent(part + blockIdx.x, s);
+    if (!elect_last(ticket, &s_flag)) return;
+    const double t = fixed_sum(part, gridDim.x, red);
+    if (threadIdx.x == 0) *out = t;
+}
+
+void sumsq_diff(const double* a, const double* b, long n, double* part,
+                unsigned int* ticket, double* out, hipStream_t s) {
+    k_sumsq_diff<double><<<reduce_blocks(n), BS, 0, s>>>(a, b, n, part, ticket, out);
+}
+void sumsq_diff(const float* a, const float* b, long n, double* part,
+                unsigned int* ticket, double* out, hipStream_t s) {
+    k_sumsq_diff<float><<<reduce_blocks(n), BS, 0, s>>>(a, b, n, part, ticket, out);
+}
+
+__global__ __launch_bounds__(BS) void k_sumsq_q(const double* srmin,
+                                                const double* srmax, long m,
+                                                double* part,
+                                                unsigned int* ticket,
+                                                double* out) {
+    __shared__ double red[BS / WAVE];
+    __shared__ int s_flag;
+    double s = 0.0;
+    for (long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x; i < m;
+         i += static_cast<long>(gridDim.x) * BS) {
+        const double q = __builtin_isfinite(srmax[i])
+                             ? srmax[i]
+                             : (__builtin_isfinite(srmin[i]) ? srmin[i] : 0.0);
+        s += q * q;
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) store_agent(part + blockIdx.x, s);
+    if (!elect_last(ticket, &s_flag)) return;
+    const double t = fixed_sum(part, gridDim.x, red);
+    if (threadIdx.x == 0) *out = t;
+}
+void sumsq_q(const double* srmin, const double* srmax, long m, double* part,
+             unsigned int* ticket, double* out, hipStream_t s) {
+    k_sumsq_q<<<reduce_blocks(m), BS, 0, s>>>(srmin, srmax, m, part, ticket, out);
+}
+
+__global__ __launch_bounds__(BS) void k_absmax(const double* v, long n,
+                                               double* part,
+                                               unsigned int* ticket,
+                                               double* out) {
+    __shared__ double red[BS / WAVE];
+    __shared__ int s_flag;
+    double s = 0.0;
+    for (long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * BS)
+        s = fmax(s, fabs(v[i]));
+    s = block_max(s, red);
+    if (threadIdx.x == 0) store_agent(part + blockIdx.x, s);
+    if (!elect_last(ticket, &s_flag)) return;
+    const double t = fixed_max(part, gridDim.x, red);
+    if (threadIdx.x == 0) *out = t;
+}
+void absmax(const double* v, long n, double* part, unsigned int* ticket,
+            double* out, hipStream_t s) {
+    k_absmax<<<reduce_blocks(n), BS, 0, s>>>(v, n, part, ticket, out);
+}
+
+// ------------------------------------------------------ elementwise --
+
+template <typename T>
+__global__ void k_fill(T* v, long n, T val) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) v[i] = val;
+}
+void fill(double* v, long n, double val, hipStream_t s) {
+    if (n > 0) k_fill<double><<<elem_grid(n), BS, 0, s>>>(v, n, val);
+}
+void fill(float* v, long n, float val, hipStream_t s) {
+    if (n > 0) k_fill<float><<<elem_grid(n), BS, 0, s>>>(v, n, val);
+}
+
+__global__ void k_scale_into(double* out, const double* in, long n, double f) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) out[i] = in[i] * f;
+}
+void scale_into(double* out, const double* in, long n, double f, hipStream_t s) {
+    if (n > 0) k_scale_into<<<elem_grid(n), BS, 0, s>>>(out, in, n, f);
+}
+
+__global__ void k_to_float(const double* in, float* out, long n) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) out[i] = static_cast<float>(in[i]);
+}
+void to_float(const double* in, float* out, long n, hipStream_t s) {
+    if (n > 0) k_to_float<<<elem_grid(n), BS, 0, s>>>(in, out, n);
+}
+
+__global__ void k_combine(const double* ref, const float* d, double* out, long n) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) out[i] = ref[i] + static_cast<double>(d[i]);
+}
+void combine(const double* ref, const float* d, double* out, long n, hipStream_t s) {
+    if (n > 0) k_combine<<<elem_grid(n), BS, 0, s>>>(ref, d, out, n);
+}
+
+__global__ void k_fold(double* ref, float* d, long n) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) {
+        ref[i] += static_cast<double>(d[i]);
+        d[i] = 0.0f;
+    }
+}
+void fold(double* ref, float* d, long n, hipStream_t s) {
+    if (n > 0) k_fold<<<elem_grid(n), BS, 0, s>>>(ref, d, n);
+}
+
+__global__ void k_refine_data(int m, int n, const double* scl,
+                              const double* scu, const double* sc,
+                              const double* xref, const double* ata_ref,
+                              const double* srmin, const double* srmax,
+                              const double* aref, const double* yref,
+                              float* lo, float* hi, float* gref, float* rlo,
+                              float* rhi, float* yoff) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) {
+        lo[i] = static_cast<float>(scl[i] - xref[i]);
+        hi[i] = static_cast<float>(scu[i] - xref[i]);
+        gref[i] = static_cast<float>(sc[i] + ata_ref[i]);
+    }
+    if (i < m) {
+        rlo[i] = static_cast<float>(srmin[i] - aref[i]);
+        rhi[i] = static_cast<float>(srmax[i] - aref[i]);
+        yoff[i] = static_cast<float>(yref[i]);
+    }
+}
+void refine_data(int m, int n, const double* scl, const double* scu,
+                 const double* sc, const double* xref, const double* ata_ref,
+                 const double* srmin, const double* srmax,
+                 const double* aref, const double* yref, float* lo, float* hi,
+                 float* gref, float* rlo, float* rhi, float* yoff,
+                 hipStream_t s) {
+    const long mx = m > n ? m : n;
+    if (mx > 0)
+        k_refine_data<<<elem_grid(mx), BS, 0, s>>>(m, n, scl, scu, sc, xref,
+                                                   ata_ref, srmin, srmax, aref,
+                                                   yref, lo, hi, gref, rlo,
+                                                   rhi, yoff);
+}
+
+__global__ void k_unscale_out(int m, int n, const double* dc, const double* dr,
+                              const double* xref, const double* xd64,
+                              const float* xd32, const double* yref,
+                              const double* yd64, const float* yd32,
+                              double* x, double* y) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) {
+        double sx = xd64 ? xd64[i] : static_cast<double>(xd32[i]);
+        if (xref) sx = xref[i] + sx;
+        x[i] = sx * dc[i];
+    }
+    if (i < m) {
+        double sy = yd64 ? yd64[i] : static_cast<double>(yd32[i]);
+        if (yref) sy = yref[i] + sy;
+        y[i] = 1.0 * sy / dr[i];
+    }
+}
+void unscale_out(int m, int n, const double* dc, const double* dr,
+                 const double* xref, const double* xd64, const float* xd32,
+                 const double* yref, const double* yd64, const float* yd32,
+                 double* x, double* y, hipStream_t s) {
+    const long mx = m > n ? m : n;
+    if (mx > 0)
+        k_unscale_out<<<elem_grid(mx), BS, 0, s>>>(m, n, dc, dr, xref, xd64,
+                                                   xd32, yref, yd64, yd32, x, y);
+}
+
+}  // namespace igaos::dev

@@ -1,0 +1,139 @@
+#pragma once
+// Host-callable launchers of the single-LP PDHG kernels (gfx950).
+// All pointers are device pointers unless stated; every launcher enqueues
+// on the given stream and returns without synchronising.
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "pdhg_ctrl.hpp"
+
+namespace igaos::dev {
+
+// Row-block layout of one CSR matrix (block starts, nblk + 1 entries).
+struct RowBlocks {
+    const int* rb = nullptr;
+    int nblk = 0;
+};
+
+// Host: cut rows [0, nrows) of a CSR row pointer into row blocks.
+std::vector<int> build_row_blocks(const int* ptr, int nrows);
+
+// y = A x  (plain SpMV).
+void spmv(const RowBlocks& rb, const int* ptr, const int* idx,
+          const double* val, const double* x, double* y, hipStream_t s);
+void spmv(const RowBlocks& rb, const int* ptr, const int* idx,
+          const float* val, const float* x, float* y, hipStream_t s);
+// y = A (xref + xd) in FP64 (xref may be null).
+void spmv_ref(const RowBlocks& rb, const int* ptr, const int* idx,
+              const double* val, const double* xref, const float* xd,
+              double* y, hipStream_t s);
+
+// Buffers of one single-LP PDHG iteration.  x[c], y[c] hold the current
+// iterate for c = LsState::cur; the other slot receives the attempt.
+template <typename V>
+struct IterBufs {
+    int m = 0, n = 0;
+    const int *ap = nullptr, *ai = nullptr, *cp = nullptr, *ci = nullptr;
+    const V *ax = nullptr, *acx = nullptr;
+    RowBlocks rbA, rbT;
+    V* x[2] = {nullptr, nullptr};
+    V* y[2] = {nullptr, nullptr};
+    V *xbar = nullptr, *ata = nullptr, *xavg = nullptr, *yavg = nullptr;
+    V *axcur = nullptr, *adx = nullptr;
+    const V *gref = nullptr, *lo = nullptr, *hi = nullptr;   // primal data
+    const V *rlo = nullptr, *rhi = nullptr, *yoff = nullptr; // dual data
+    double* partT = nullptr;  // nblkT partials of |dx|^2
+    double* partA = nullptr;  // 2*nblkA partials of |dy|^2, dy'Adx
+    pdhg::LsState* st = nullptr;
+    unsigned int* ticket = nullptr;
+};
+
+// One line-search attempt: fused A'y + primal step, fused A xbar + dual
+// step + line-search decision (both no-ops once the state is frozen).
+void launch_attempt(const IterBufs<double>& b, hipStream_t s);
+void launch_attempt(const IterBufs<float>& b, hipStream_t s);
+// Apply the averaging / A x commit of the last accepted step.
+void launch_flush(const IterBufs<double>& b, int cur, double w, hipStream_t s);
+void launch_flush(const IterBufs<float>& b, int cur, double w, hipStream_t s);
+
+// KKT evaluation in original units for two iterates at once
+// (src/pdhg/pdhg.cu:173-254 via metrics_of, 694-733).
+struct KktData {
+    int m = 0, n = 0;
+    const int *ap = nullptr, *ai = nullptr, *cp = nullptr, *ci = nullptr;
+    const double *sax = nullptr, *sacx = nullptr;  // scaled FP64 A, A'
+    RowBlocks rbA, rbT;
+    const double *c = nullptr, *cl = nullptr, *cu = nullptr;
+    const double *rmin = nullptr, *rmax = nullptr;  // original units
+    const double *dr = nullptr, *dc = nullptr;
+    double cmax = 0.0;
+    double *partC = nullptr, *partR = nullptr, *partR2 = nullptr;
+    double* rowres = nullptr;        // 2 x 4 row-pass results
+    unsigned int* tickets = nullptr; // 2 counters
+};
+template <typename V>
+struct KktIter {
+    const double* xref[2] = {nullptr, nullptr};
+    const V* xd[2] = {nullptr, nullptr};
+    const double* yref[2] = {nullptr, nullptr};
+    const V* yd[2] = {nullptr, nullptr};
+    double* ata_out[2] = {nullptr, nullptr};  // optional A~'y~ (scaled)
+    double* tm_out[2] = {nullptr, nullptr};   // optional A~x~ (scaled)
+};
+void launch_kkt(const KktData& d, const KktIter<double>& it,
+                pdhg::Metrics* out, hipStream_t s);
+void launch_kkt(const KktData& d, const KktIter<float>& it,
+                pdhg::Metrics* out, hipStream_t s);
+
+// Ruiz equilibration (10 sweeps, src/pdhg/pdhg.cu:469-492) on device.
+void ruiz(int m, int n, const int* ap, const int* ai, const double* ax,
+          const int* cp, const int* ci, const double* acx, double* dr,
+          double* dc, hipStream_t s);
+// Scaled data (pdhg.cu:499-513).
+void scale_data(int m, int n, const int* ap, const int* ai, const double* ax,
+                const int* cp, const int* ci, const double* acx,
+                const double* dr, const double* dc, const double* c,
+                const double* cl, const double* cu, const double* rmin,
+                const double* rmax, double* sax, double* sacx, double* sc,
+                double* scl, double* scu, double* srmin, double* srmax,
+                hipStream_t s);
+
+// out[0] = sum (a - b)^2 (b may be null), fixed order; `part` holds at
+// least reduce_blocks(n) doubles and `ticket` one zeroed counter.
+int reduce_blocks(long n);
+void sumsq_diff(const double* a, const double* b, long n, double* part,
+                unsigned int* ticket, double* out, hipStream_t s);
+void sumsq_diff(const float* a, const float* b, long n, double* part,
+                unsigned int* ticket, double* out, hipStream_t s);
+// out[0] = sum_j q_j^2 with q = srmax if finite, else srmin if finite, else 0
+void sumsq_q(const double* srmin, const double* srmax, long m, double* part,
+             unsigned int* ticket, double* out, hipStream_t s);
+// out[0] = max |v|
+void absmax(const double* v, long n, double* part, unsigned int* ticket,
+            double* out, hipStream_t s);
+
+// Elementwise helpers.
+void fill(double* v, long n, double val, hipStream_t s);
+void fill(float* v, long n, float val, hipStream_t s);
+void scale_into(double* out, const double* in, long n, double f, hipStream_t s);
+void to_float(const double* in, float* out, long n, hipStream_t s);
+void combine(const double* ref, const float* d, double* out, long n,
+             hipStream_t s);
+// out = ref + d for each, in place on ref; d zeroed.
+void fold(double* ref, float* d, long n, hipStream_t s);
+// Shifted FP32 data of the correction problem (see engine.cpp).
+void refine_data(int m, int n, const double* scl, const double* scu,
+                 const double* sc, const double* xref, const double* ata_ref,
+                 const double* srmin, const double* srmax,
+                 const double* aref, const double* yref, float* lo, float* hi,
+                 float* gref, float* rlo, float* rhi, float* yoff,
+                 hipStream_t s);
+// Original-unit output: x = dc * sx (sx = ref + d), y = sy / dr.
+void unscale_out(int m, int n, const double* dc, const double* dr,
+                 const double* xref, const double* xd64, const float* xd32,
+                 const double* yref, const double* yd64, const float* yd32,
+                 double* x, double* y, hipStream_t s);
+
+}  // namespace igaos::dev

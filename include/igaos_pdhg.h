@@ -1,0 +1,139 @@
+/* C-ABI of the MI355X PDHG engine (libigaos_pdhg.so).
+ *
+ * This is the drop-in boundary for the reference's `pdhg` engine slot.
+ * Every entry point takes plain pointers and sizes; none takes or returns
+ * a C++ or torch type.  Functions return 0 on success and a negative code
+ * on failure (igaos_last_error() then describes it) unless stated.
+ * Pointers documented as "device" must be HIP device allocations on the
+ * current device; "host" pointers are ordinary host memory.
+ */
+#ifndef IGAOS_PDHG_H
+#define IGAOS_PDHG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IGAOS_PDHG_ABI_VERSION 1
+
+/* Status codes: the order of igaos::Status (include/igaos/status.h:5-14). */
+enum {
+    IGAOS_OPTIMAL = 0,
+    IGAOS_NEAR_OPTIMAL = 1,
+    IGAOS_FEASIBLE = 2,
+    IGAOS_INFEASIBLE = 3,
+    IGAOS_UNBOUNDED = 4,
+    IGAOS_ITERATION_LIMIT = 5,
+    IGAOS_TIME_LIMIT = 6,
+    IGAOS_ERROR = 7
+};
+
+int igaos_pdhg_abi_version(void);
+/* Number of visible HIP devices (0 when no GPU is present). */
+int igaos_pdhg_device_count(void);
+/* Text of the last error raised on this thread ("" if none). */
+const char* igaos_last_error(void);
+/* igaos::status_name (include/igaos/status.h:16-28). */
+const char* igaos_status_name(int status);
+
+/* ---- model: the reference's io::Model (src/io/model.hpp:16-37) ------- */
+typedef struct igaos_model igaos_model;
+
+/* Replaces igaos::io::parse_mps (src/io/model.hpp:69-218).  Returns NULL
+ * on failure. */
+igaos_model* igaos_model_read_mps(const char* path);
+/* Builds a model from triplets in the given order; CSR and CSC keep that
+ * order inside each row / column like counting_sort (model.hpp:39-67). */
+igaos_model* igaos_model_from_triplets(int m, int n, int nnz,
+                                       const int* rows, const int* cols,
+                                       const double* vals, const double* c,
+                                       const double* cl, const double* cu,
+                                       const double* rmin, const double* rmax,
+                                       double obj_const);
+/* Builds a model from ready CSR + CSC arrays (copied). */
+igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
+                                      const double* cl, const double* cu,
+                                      const double* rmin, const double* rmax,
+                                      const int* ap, const int* ai,
+                                      const double* ax, const int* cp,
+                                      const int* ci, const double* acx,
+                                      double obj_const);
+void igaos_model_free(igaos_model* model);
+/* Sizes and MPS counters (any output may be NULL). */
+void igaos_model_dims(const igaos_model* model, int* m, int* n, int* nnz,
+                      double* obj_const, int* n_fr_parsed,
+                      int* n_ranges_parsed, int* n_int);
+/* Copies the model's arrays out (any output may be NULL). */
+void igaos_model_get(const igaos_model* model, double* rmin, double* rmax,
+                     double* cl, double* cu, double* c, int* ap, int* ai,
+                     double* ax, int* cp, int* ci, double* acx);
+
+/* ---- solve: igaos::pdhg::solve (src/pdhg/pdhg.hpp:9) ------------------- */
+/* Mirrors igaos::Options (include/igaos/options.h:5-13). */
+typedef struct {
+    double time_limit_s;
+    double tolerance;
+    long max_iterations;
+    int seed;
+    int presolve;
+    int restarts;
+    int verbosity;
+} igaos_options;
+void igaos_options_default(igaos_options* opt);
+
+/* Engine-private knobs (no reference counterpart). */
+typedef struct {
+    int precision;    /* 64: plain FP64; 32: FP32 inner + FP64 refinement */
+    int graph_slots;  /* attempts per hipGraph replay; 0 = direct launches */
+    int max_repair_m; /* dense dual repair only for m <= this */
+    int device;
+} igaos_engine_config;
+void igaos_engine_config_default(igaos_engine_config* cfg);
+
+/* Solution fields of igaos::Solution (include/igaos/solution.h:10-22)
+ * plus engine diagnostics. */
+typedef struct {
+    int status;
+    double objective, pinf, dinf, rel_gap;
+    long iterations;
+    double solve_time_ms;
+    double setup_ms;
+    long attempts, restarts, checks, refinements;
+    double eta, omega, lam2, eta0, omega0;
+    int last_is_avg, used_incumbent, repaired;
+    double kernel_ms;
+} igaos_solve_info;
+
+/* Replaces igaos::pdhg::solve (src/pdhg/pdhg.hpp:9, pdhg.cu:457-1156).
+ * x (host, n) and y (host, m) receive the solution in original units with
+ * the dual sign convention d = c + A'y; msg (host, msglen bytes) receives
+ * Solution::message.  cfg may be NULL (plain FP64 defaults).  Like the
+ * reference it never fails loudly for solver outcomes: the status field
+ * carries error / iteration-limit / time-limit; the return value is
+ * negative only for invalid arguments. */
+int igaos_pdhg_solve(const igaos_model* model, const igaos_options* opt,
+                     const igaos_engine_config* cfg, double* x, double* y,
+                     igaos_solve_info* info, char* msg, int msglen);
+
+/* ---- kernel layer (device pointers, HIP stream or NULL) ----------------- */
+/* Host: row-block layout of a CSR row pointer (host) for the SpMV
+ * kernels.  Writes at most cap entries; returns the number of entries
+ * (blocks + 1), or the required count if cap is too small. */
+int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap);
+/* y = A x with A in CSR (device arrays); replaces the cusparseSpMV calls
+ * of src/pdhg/pdhg.cu:150-156 for A and A' (CSR of A' = CSC of A). */
+int igaos_spmv_csr_f64(int nblk, const int* rowblocks, const int* ptr,
+                       const int* idx, const double* val, const double* x,
+                       double* y, void* stream);
+int igaos_spmv_csr_f32(int nblk, const int* rowblocks, const int* ptr,
+                       const int* idx, const float* val, const float* x,
+                       float* y, void* stream);
+/* Ruiz equilibration, 10 sweeps (pdhg.cu:469-492), device arrays. */
+int igaos_ruiz_f64(int m, int n, const int* ap, const int* ai,
+                   const double* ax, const int* cp, const int* ci,
+                   const double* acx, double* dr, double* dc, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
