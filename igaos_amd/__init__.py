@@ -19,6 +19,7 @@ from .api import (  # noqa: F401
     solve,
     solve_lp,
     model_from_lp,
+    Engine,
 )
 
 __version__ = "0.1.0"

@@ -68,6 +68,13 @@ SIGNATURES = {
                                    C.POINTER(EngineConfig), _dp, _dp,
                                    C.POINTER(SolveInfo), C.c_char_p,
                                    C.c_int]),
+    "igaos_pdhg_create": (_vp, [_vp, C.POINTER(EngineConfig)]),
+    "igaos_pdhg_run": (C.c_int, [_vp, C.POINTER(Options), _dp, _dp,
+                                 C.POINTER(SolveInfo), C.c_char_p, C.c_int]),
+    "igaos_pdhg_setup_ms": (C.c_double, [_vp]),
+    "igaos_pdhg_destroy": (None, [_vp]),
+    "igaos_pdhg_profile_attempts": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip,
+                                              _ip]),
     "igaos_rowblocks_build": (C.c_int, [C.c_int, _ip, _ip, C.c_int]),
     "igaos_spmv_csr_f64": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),

@@ -143,9 +143,13 @@ class Model:
 
 
 def _has_pdhg() -> bool:
+    # False when the library is not built yet (e.g. while building it) or
+    # no HIP device is visible; every solve still goes through the library
+    # and fails loudly if it is missing.
     try:
         return _core.device_count() > 0
-    except ImportError:
+    except (ImportError, OSError, AttributeError):
+        _core._lib = None
         return False
 
 
@@ -170,19 +174,72 @@ def model_from_lp(lp) -> Model:
     return Model(h)
 
 
-def _run(model: Model, opt: Options, cfg: EngineConfig | None) -> Solution:
+def _run(model: Model, opt: Options, cfg: EngineConfig | None,
+         engine=None) -> Solution:
     L = _core.lib()
     x = np.zeros(model.n)
     y = np.zeros(model.m)
     info = SolveInfo()
     msg = C.create_string_buffer(512)
-    rc = L.igaos_pdhg_solve(model._h, C.byref(opt),
-                            C.byref(cfg) if cfg is not None else None,
-                            x.ctypes.data_as(C.POINTER(C.c_double)),
-                            y.ctypes.data_as(C.POINTER(C.c_double)),
-                            C.byref(info), msg, 512)
+    xp = x.ctypes.data_as(C.POINTER(C.c_double))
+    yp = y.ctypes.data_as(C.POINTER(C.c_double))
+    if engine is None:
+        rc = L.igaos_pdhg_solve(model._h, C.byref(opt),
+                                C.byref(cfg) if cfg is not None else None,
+                                xp, yp, C.byref(info), msg, 512)
+    else:
+        rc = L.igaos_pdhg_run(engine, C.byref(opt), xp, yp, C.byref(info),
+                              msg, 512)
     if rc != 0:
         raise ValueError(_core.last_error())
+    return _solution(info, x, y, msg)
+
+
+class Engine:
+    """A prepared engine (upload + scaling + power iteration done once).
+
+    ``run(...)`` solves afresh from x = y = 0 on the HBM-resident data and
+    returns the same Solution ``solve_lp`` would.
+    """
+
+    def __init__(self, lp, precision=64, graph_slots=None, max_repair_m=None):
+        self.model = lp if isinstance(lp, Model) else model_from_lp(lp)
+        self.precision = precision
+        cfg = _config(precision, graph_slots, max_repair_m)
+        h = _core.lib().igaos_pdhg_create(self.model._h, C.byref(cfg))
+        if not h:
+            raise RuntimeError(_core.last_error())
+        self._h = C.c_void_p(h)
+        self.setup_ms = _core.lib().igaos_pdhg_setup_ms(self._h)
+
+    def run(self, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
+            verbosity=0) -> Solution:
+        return _run(self.model, _options(time_limit, tolerance,
+                                         max_iterations, True, 0, verbosity),
+                    None, engine=self._h)
+
+    def profile_attempts(self, attempts=20):
+        """Mean device ms per launch of (k_primal, k_dual) and grid sizes."""
+        a, b = C.c_double(), C.c_double()
+        ga, gb = C.c_int(), C.c_int()
+        rc = _core.lib().igaos_pdhg_profile_attempts(
+            self._h, int(attempts), C.byref(a), C.byref(b), C.byref(ga),
+            C.byref(gb))
+        if rc != 0:
+            raise RuntimeError(_core.last_error())
+        return dict(ms_primal=a.value, ms_dual=b.value,
+                    blocks_primal=ga.value, blocks_dual=gb.value)
+
+    def close(self):
+        if getattr(self, "_h", None) and _core._lib is not None:
+            _core._lib.igaos_pdhg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _solution(info, x, y, msg) -> Solution:
     st = Status(info.status)
     extra = {k: getattr(info, k) for k, _ in SolveInfo._fields_}
     return Solution(status=STATUS_NAMES[st], status_enum=st,

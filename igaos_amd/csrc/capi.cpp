@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -196,11 +197,10 @@ void igaos_engine_config_default(igaos_engine_config* cfg) {
     cfg->device = e.device;
 }
 
-int igaos_pdhg_solve(const igaos_model* model, const igaos_options* opt,
-                     const igaos_engine_config* cfg, double* x, double* y,
-                     igaos_solve_info* info, char* msg, int msglen) {
-    if (!model || !opt) return fail("null model or options");
-    igaos::Options o;
+namespace {
+
+bool to_options(const igaos_options* opt, igaos::Options& o) {
+    if (!opt) return false;
     o.time_limit_s = opt->time_limit_s;
     o.tolerance = opt->tolerance;
     o.max_iterations = opt->max_iterations;
@@ -208,26 +208,25 @@ int igaos_pdhg_solve(const igaos_model* model, const igaos_options* opt,
     o.presolve = opt->presolve != 0;
     o.restarts = opt->restarts != 0;
     o.verbosity = opt->verbosity;
-    igaos::pdhg::EngineConfig ec;
-    if (cfg) {
-        if (cfg->precision != 32 && cfg->precision != 64)
-            return fail("precision must be 32 or 64");
-        ec.precision = cfg->precision;
-        ec.graph_slots = std::max(0, cfg->graph_slots);
-        ec.max_repair_m = cfg->max_repair_m;
-        ec.device = cfg->device;
-    }
-    igaos::pdhg::SolveInfo si;
-    igaos::Solution s;
-    try {
-        s = igaos::pdhg::solve_ex(model->m, o, ec, &si);
-    } catch (const std::exception& e) {
-        s = igaos::Solution{};
-        s.message = e.what();
-    }
-    if (x && s.x.size() == static_cast<size_t>(model->m.n))
+    return true;
+}
+
+bool to_config(const igaos_engine_config* cfg, igaos::pdhg::EngineConfig& ec) {
+    if (!cfg) return true;
+    if (cfg->precision != 32 && cfg->precision != 64) return false;
+    ec.precision = cfg->precision;
+    ec.graph_slots = std::max(0, cfg->graph_slots);
+    ec.max_repair_m = cfg->max_repair_m;
+    ec.device = cfg->device;
+    return true;
+}
+
+void export_solution(const igaos::Solution& s, const igaos::pdhg::SolveInfo& si,
+                     int m, int n, double* x, double* y, igaos_solve_info* info,
+                     char* msg, int msglen) {
+    if (x && s.x.size() == static_cast<size_t>(n))
         std::memcpy(x, s.x.data(), sizeof(double) * s.x.size());
-    if (y && s.y.size() == static_cast<size_t>(model->m.m))
+    if (y && s.y.size() == static_cast<size_t>(m))
         std::memcpy(y, s.y.data(), sizeof(double) * s.y.size());
     if (msg && msglen > 0) {
         std::strncpy(msg, s.message.c_str(), msglen - 1);
@@ -258,6 +257,92 @@ int igaos_pdhg_solve(const igaos_model* model, const igaos_options* opt,
         info->kernel_ms = si.kernel_ms;
     }
     if (s.status == igaos::Status::Error) g_err = s.message;
+}
+
+}  // namespace
+
+struct igaos_engine {
+    std::unique_ptr<igaos::pdhg::EngineBase> e;
+    int m = 0, n = 0;
+};
+
+int igaos_pdhg_solve(const igaos_model* model, const igaos_options* opt,
+                     const igaos_engine_config* cfg, double* x, double* y,
+                     igaos_solve_info* info, char* msg, int msglen) {
+    igaos::Options o;
+    igaos::pdhg::EngineConfig ec;
+    if (!model || !to_options(opt, o)) return fail("null model or options");
+    if (!to_config(cfg, ec)) return fail("precision must be 32 or 64");
+    igaos::pdhg::SolveInfo si;
+    const igaos::Solution s = igaos::pdhg::solve_ex(model->m, o, ec, &si);
+    export_solution(s, si, model->m.m, model->m.n, x, y, info, msg, msglen);
+    return 0;
+}
+
+igaos_engine* igaos_pdhg_create(const igaos_model* model,
+                                const igaos_engine_config* cfg) {
+    igaos::pdhg::EngineConfig ec;
+    if (!model) {
+        fail("null model");
+        return nullptr;
+    }
+    if (!to_config(cfg, ec)) {
+        fail("precision must be 32 or 64");
+        return nullptr;
+    }
+    try {
+        auto* h = new igaos_engine;
+        h->m = model->m.m;
+        h->n = model->m.n;
+        try {
+            h->e = igaos::pdhg::make_engine(model->m, ec);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return h;
+    } catch (const std::exception& ex) {
+        fail(ex.what());
+        return nullptr;
+    }
+}
+
+int igaos_pdhg_run(igaos_engine* engine, const igaos_options* opt, double* x,
+                   double* y, igaos_solve_info* info, char* msg, int msglen) {
+    igaos::Options o;
+    if (!engine || !to_options(opt, o)) return fail("null engine or options");
+    igaos::pdhg::SolveInfo si;
+    igaos::Solution s;
+    try {
+        s = engine->e->run(o, &si);
+    } catch (const std::exception& ex) {
+        s = igaos::Solution{};
+        s.status = igaos::Status::Error;
+        s.message = ex.what();
+    }
+    export_solution(s, si, engine->m, engine->n, x, y, info, msg, msglen);
+    return 0;
+}
+
+double igaos_pdhg_setup_ms(const igaos_engine* engine) {
+    return engine ? engine->e->setup_ms() : 0.0;
+}
+
+void igaos_pdhg_destroy(igaos_engine* engine) { delete engine; }
+
+int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
+                                double* ms_primal, double* ms_dual,
+                                int* blocks_primal, int* blocks_dual) {
+    if (!engine) return fail("null engine");
+    try {
+        const auto t = engine->e->profile(attempts < 1 ? 1 : attempts);
+        if (ms_primal) *ms_primal = t.ms_primal;
+        if (ms_dual) *ms_dual = t.ms_dual;
+        if (blocks_primal) *blocks_primal = t.nblk_primal;
+        if (blocks_dual) *blocks_dual = t.nblk_dual;
+    } catch (const std::exception& ex) {
+        return fail(ex.what());
+    }
     return 0;
 }
 

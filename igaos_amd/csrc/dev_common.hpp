@@ -40,8 +40,8 @@ __device__ __forceinline__ T wave_max(T v) {
 }
 
 // Workgroup sum with a fixed combination order; every thread gets the
-// result.  `red` must hold BS/WAVE entries.  Contains __syncthreads().
-template <typename T>
+// result.  `red` must hold NT/WAVE entries.  Contains __syncthreads().
+template <typename T, int NT = BS>
 __device__ __forceinline__ T block_sum(T v, T* red) {
     v = wave_sum(v);
     const int w = threadIdx.x / WAVE;
@@ -50,11 +50,11 @@ __device__ __forceinline__ T block_sum(T v, T* red) {
     __syncthreads();
     T s = red[0];
 #pragma unroll
-    for (int i = 1; i < BS / WAVE; ++i) s += red[i];
+    for (int i = 1; i < NT / WAVE; ++i) s += red[i];
     return s;
 }
 
-template <typename T>
+template <typename T, int NT = BS>
 __device__ __forceinline__ T block_max(T v, T* red) {
     v = wave_max(v);
     const int w = threadIdx.x / WAVE;
@@ -63,50 +63,33 @@ __device__ __forceinline__ T block_max(T v, T* red) {
     __syncthreads();
     T s = red[0];
 #pragma unroll
-    for (int i = 1; i < BS / WAVE; ++i) s = fmax(s, red[i]);
+    for (int i = 1; i < NT / WAVE; ++i) s = fmax(s, red[i]);
     return s;
 }
 
-// Agent-scope loads/stores for partials read by the last workgroup of the
-// same launch (the 8 XCDs have separate L2s).
-__device__ __forceinline__ void store_agent(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_agent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Reductions across workgroups: every workgroup of a kernel writes one
+// partial per quantity, and a follow-up kernel of one FIN-thread workgroup
+// combines them in a fixed order (thread t takes partials t, t+FIN, ...;
+// then the workgroup tree).  No atomics and no cross-workgroup fences, so
+// results are run-to-run deterministic.
+constexpr int FIN = 1024;
 
-// Last-workgroup-done election: every workgroup calls this once after
-// publishing its partials; returns true (uniformly within the workgroup)
-// only in the last one to arrive, which also resets the counter.
-__device__ __forceinline__ bool elect_last(unsigned int* ticket, int* s_flag) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int prev = atomicAdd(ticket, 1u);
-        *s_flag = (prev == gridDim.x * gridDim.y - 1) ? 1 : 0;
+__device__ __forceinline__ double fin_sum(const double* p, long cnt,
+                                          long stride, double* red) {
+    double s0 = 0.0, s1 = 0.0;
+    long i = threadIdx.x;
+    for (; i + FIN < cnt; i += 2 * FIN) {
+        s0 += p[i * stride];
+        s1 += p[(i + FIN) * stride];
     }
-    __syncthreads();
-    const bool last = *s_flag != 0;
-    if (last) {
-        __threadfence();
-        if (threadIdx.x == 0) atomicExch(ticket, 0u);
-    }
-    return last;
+    if (i < cnt) s0 += p[i * stride];
+    return block_sum<double, FIN>(s0 + s1, red);
 }
-
-// Fixed-order sum of `cnt` partials (strided per thread, then tree).
-__device__ __forceinline__ double fixed_sum(const double* p, int cnt,
-                                            double* red) {
+__device__ __forceinline__ double fin_max(const double* p, long cnt,
+                                          long stride, double* red) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < cnt; i += BS) s += load_agent(p + i);
-    return block_sum(s, red);
-}
-__device__ __forceinline__ double fixed_max(const double* p, int cnt,
-                                            double* red) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < cnt; i += BS) s = fmax(s, load_agent(p + i));
-    return block_max(s, red);
+    for (long i = threadIdx.x; i < cnt; i += FIN) s = fmax(s, p[i * stride]);
+    return block_max<double, FIN>(s, red);
 }
 
 // Gather functors: x[j] for plain vectors, xref[j] + (double)xd[j] for an

@@ -88,24 +88,30 @@ double now_s() {
 }
 
 template <typename V>
-class Engine {
+class Engine final : public EngineBase {
   public:
-    Engine(const io::Model& lp, const Options& opt, const EngineConfig& cfg)
-        : lp_(lp), opt_(opt), cfg_(cfg), m_(lp.m), n_(lp.n) {}
-    ~Engine() {
+    Engine(const io::Model& lp, const EngineConfig& cfg)
+        : lp_(lp), cfg_(cfg), m_(lp.m), n_(lp.n) {
+        const double t0 = now_s();
+        setup();
+        setup_ms_ = (now_s() - t0) * 1000.0;
+    }
+    ~Engine() override {
         if (exec_) (void)hipGraphExecDestroy(exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
         if (ev0_) (void)hipEventDestroy(ev0_);
         if (ev1_) (void)hipEventDestroy(ev1_);
         if (s_) (void)hipStreamDestroy(s_);
     }
-    Solution run(SolveInfo* info);
+    Solution run(const Options& opt, SolveInfo* info) override;
+    AttemptTiming profile(int attempts) override;
+    double setup_ms() const override { return setup_ms_; }
 
   private:
     static constexpr bool F32 = std::is_same<V, float>::value;
     const io::Model& lp_;
-    const Options& opt_;
     EngineConfig cfg_;
+    double setup_ms_ = 0.0;
     const int m_, n_;
     hipStream_t s_ = nullptr;
     hipGraph_t graph_ = nullptr;
@@ -133,6 +139,7 @@ class Engine {
     double lam2_ = 1.0, eta0_ = 1.0, omega0_ = 1.0;
 
     void setup();
+    void reset();
     double power_iteration();
     double dsum(const double* a, const double* b, long n);
     void check_metrics(int cur, Metrics& mc, Metrics& ma);
@@ -267,8 +274,6 @@ void Engine<V>::setup() {
     // iteration buffers
     x0_.alloc(n_); x1_.alloc(n_); xbar_.alloc(n_); ata_.alloc(n_); xavg_.alloc(n_);
     y0_.alloc(m_); y1_.alloc(m_); yavg_.alloc(m_); axcur_.alloc(m_); adx_.alloc(m_);
-    for (DBuf<V>* b : {&x0_, &x1_, &xbar_, &ata_, &xavg_, &y0_, &y1_, &yavg_, &axcur_, &adx_})
-        zero_v(b->p, b->n);
     xb_.alloc(n_); yb_.alloc(m_); pesx_.alloc(n_); pesy_.alloc(m_);
     partT_.alloc(std::max(rT_.nblk, 1));
     partA_.alloc(2 * std::max(rA_.nblk, 1));
@@ -298,9 +303,6 @@ void Engine<V>::setup() {
         xref_.alloc(n_); yref_.alloc(m_); aref_.alloc(m_); atref_.alloc(n_);
         lo_.alloc(n_); hi_.alloc(n_); gref_.alloc(n_);
         rlo_.alloc(m_); rhi_.alloc(m_); yoff_.alloc(m_);
-        dev::fill(xref_.p, n_, 0.0, s_);
-        dev::fill(yref_.p, m_, 0.0, s_);
-        recompute_refs();
         ib_.lo = lo_.p; ib_.hi = hi_.p; ib_.gref = gref_.p;
         ib_.rlo = rlo_.p; ib_.rhi = rhi_.p; ib_.yoff = yoff_.p;
     } else {
@@ -321,7 +323,27 @@ void Engine<V>::setup() {
     kd_.cmax = cmax;
     kd_.partC = partC_.p; kd_.partR = partR_.p; kd_.partR2 = partR2_.p;
     kd_.rowres = rowres_.p; kd_.tickets = tickets_.p;
+
+    const int G = cfg_.graph_slots;
+    if (G > 0) {  // G attempt slots, replayed until the state freezes
+        HIPCK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        for (int g = 0; g < G; ++g) dev::launch_attempt(ib_, s_);
+        HIPCK(hipStreamEndCapture(s_, &graph_));
+        HIPCK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    }
     sync();
+}
+
+// Fresh start of a solve on the resident data: x = y = 0 (pdhg.cu:576-580).
+template <typename V>
+void Engine<V>::reset() {
+    for (DBuf<V>* b : {&x0_, &x1_, &xbar_, &ata_, &xavg_, &y0_, &y1_, &yavg_, &axcur_, &adx_})
+        zero_v(b->p, b->n);
+    if constexpr (F32) {
+        dev::fill(xref_.p, n_, 0.0, s_);
+        dev::fill(yref_.p, m_, 0.0, s_);
+        recompute_refs();
+    }
 }
 
 // FP32 engine: residual data of the correction problem at (x_ref, y_ref).
@@ -356,12 +378,10 @@ void Engine<V>::check_metrics(int cur, Metrics& mc, Metrics& ma) {
 }
 
 template <typename V>
-Solution Engine<V>::run(SolveInfo* info) {
+Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
     Solution sol;
-    const double t_setup0 = now_s();
-    setup();
     const double INF = std::numeric_limits<double>::infinity();
-    const double t_setup1 = now_s();
+    reset();
 
     const long MAXIT = (opt_.time_limit_s > 0 && opt_.time_limit_s < 1e6)
                            ? 20000000L
@@ -378,14 +398,9 @@ Solution Engine<V>::run(SolveInfo* info) {
     st.cur = 0;
     st.frozen = MAXIT < 1 ? 1 : 0;
     write_state(st);
+    sync();
 
     const int G = cfg_.graph_slots;
-    if (G > 0) {
-        HIPCK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
-        for (int g = 0; g < G; ++g) dev::launch_attempt(ib_, s_);
-        HIPCK(hipStreamEndCapture(s_, &graph_));
-        HIPCK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
-    }
     auto launch_slots = [&](long slots) {
         if (G > 0) {
             const long reps = (slots + G - 1) / G;
@@ -671,7 +686,7 @@ Solution Engine<V>::run(SolveInfo* info) {
         sync();
     }
     if (info) {
-        info->setup_ms = (t_setup1 - t_setup0) * 1000.0;
+        info->setup_ms = setup_ms_;
         info->solve_ms = secs * 1000.0;
         info->attempts = static_cast<long>(st.attempts);
         info->restarts = restarts;
@@ -690,35 +705,78 @@ Solution Engine<V>::run(SolveInfo* info) {
     return sol;
 }
 
+template <typename V>
+AttemptTiming Engine<V>::profile(int attempts) {
+    reset();
+    LsState st{};
+    st.eta = eta0_;
+    st.omega = omega0_;
+    st.k = 1;
+    st.maxit = 1LL << 60;
+    st.fresh = 1;
+    st.no_freeze = 1;
+    write_state(st);
+    sync();
+    hipEvent_t e[3];
+    for (auto& ev : e) HIPCK(hipEventCreate(&ev));
+    AttemptTiming t;
+    t.nblk_primal = rT_.nblk;
+    t.nblk_dual = rA_.nblk;
+    double a = 0.0, b = 0.0;
+    const int warm = 3;
+    for (int it = 0; it < warm + attempts; ++it) {
+        HIPCK(hipEventRecord(e[0], s_));
+        dev::launch_primal(ib_, s_);
+        HIPCK(hipEventRecord(e[1], s_));
+        dev::launch_dual(ib_, s_);
+        HIPCK(hipEventRecord(e[2], s_));
+        dev::launch_decide(ib_, s_);
+        HIPCK(hipEventSynchronize(e[2]));
+        float ms1 = 0.0f, ms2 = 0.0f;
+        HIPCK(hipEventElapsedTime(&ms1, e[0], e[1]));
+        HIPCK(hipEventElapsedTime(&ms2, e[1], e[2]));
+        if (it >= warm) {
+            a += ms1;
+            b += ms2;
+        }
+    }
+    for (auto& ev : e) (void)hipEventDestroy(ev);
+    t.ms_primal = a / attempts;
+    t.ms_dual = b / attempts;
+    return t;
+}
+
 }  // namespace
 
-Solution solve_ex(const io::Model& model, const Options& options,
-                  const EngineConfig& cfg, SolveInfo* info) {
-    Solution sol;
+std::unique_ptr<EngineBase> make_engine(const io::Model& model,
+                                        const EngineConfig& cfg) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg.device) {
         (void)hipGetLastError();
-        sol.status = Status::Error;
-        sol.message = "PDHG engine unavailable: no HIP GPU device visible "
-                      "(this build runs the engine on MI355X via HIP, not CUDA)";
-        return sol;
+        throw std::runtime_error(
+            "PDHG engine unavailable: no HIP GPU device visible "
+            "(this build runs the engine on MI355X via HIP, not CUDA)");
     }
     if (model.m < 0 || model.n < 0 || static_cast<int>(model.ap.size()) != model.m + 1 ||
-        static_cast<int>(model.cp.size()) != model.n + 1) {
-        sol.status = Status::Error;
-        sol.message = "malformed model: CSR/CSC pointer sizes do not match m, n";
-        return sol;
-    }
+        static_cast<int>(model.cp.size()) != model.n + 1)
+        throw std::invalid_argument("malformed model: CSR/CSC pointer sizes do not match m, n");
+    if (cfg.precision == 32) return std::make_unique<Engine<float>>(model, cfg);
+    return std::make_unique<Engine<double>>(model, cfg);
+}
+
+AttemptTiming profile_attempts(const io::Model& model, const EngineConfig& cfg,
+                               int attempts) {
+    return make_engine(model, cfg)->profile(attempts < 1 ? 1 : attempts);
+}
+
+Solution solve_ex(const io::Model& model, const Options& options,
+                  const EngineConfig& cfg, SolveInfo* info) {
     try {
-        if (cfg.precision == 32) {
-            Engine<float> e(model, options, cfg);
-            return e.run(info);
-        }
-        Engine<double> e(model, options, cfg);
-        return e.run(info);
+        auto e = make_engine(model, cfg);
+        return e->run(options, info);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[pdhg] %s\n", e.what());
-        sol = Solution{};
+        Solution sol;
         sol.status = Status::Error;
         sol.message = e.what();
         return sol;

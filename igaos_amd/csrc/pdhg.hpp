@@ -4,6 +4,8 @@
 // solve_with_engine, src/api/engine.hpp:32-38), backed by HIP kernels on
 // one MI355X.  solve_ex adds engine-private knobs without touching Options.
 
+#include <memory>
+
 #include "igaos/options.h"
 #include "igaos/solution.h"
 #include "model.hpp"
@@ -32,8 +34,35 @@ struct SolveInfo {
 };
 
 Solution solve(const io::Model& model, const Options& options);
+
+// A prepared engine: construction uploads the model and runs the setup
+// (Ruiz scaling, power iteration, graph capture) once; every run() is a
+// fresh solve from x = y = 0 on the HBM-resident data.  The model must
+// outlive the engine.
+struct AttemptTiming;
+class EngineBase {
+  public:
+    virtual ~EngineBase() = default;
+    virtual Solution run(const Options& options, SolveInfo* info) = 0;
+    virtual AttemptTiming profile(int attempts) = 0;
+    virtual double setup_ms() const = 0;
+};
+// Throws std::runtime_error when no HIP device is visible or a HIP call
+// fails, std::invalid_argument for a malformed model.
+std::unique_ptr<EngineBase> make_engine(const io::Model& model,
+                                        const EngineConfig& cfg);
 Solution solve_ex(const io::Model& model, const Options& options,
                   const EngineConfig& cfg, SolveInfo* info);
+
+// Times `attempts` line-search attempts (k_primal, k_dual) with HIP events
+// on the engine's stream after the usual setup; returns the mean device
+// milliseconds per launch of each kernel.  Checks and restarts are off.
+struct AttemptTiming {
+    double ms_primal = 0.0, ms_dual = 0.0;
+    int nblk_primal = 0, nblk_dual = 0;
+};
+AttemptTiming profile_attempts(const io::Model& model, const EngineConfig& cfg,
+                               int attempts);
 
 // Host-side KKT evaluation and dense dual repair in original units
 // (src/pdhg/pdhg.cu:173-254, 288-448); used at check time only.

@@ -32,7 +32,7 @@ struct LsState {
     int cur;             // buffer index holding the current x, y
     int frozen;          // iteration k-1 ended on a check boundary or MAXIT
     int mix_pending;     // the last accepted step still needs averaging
-    int pad_;
+    int no_freeze;       // timing mode: never freeze at check boundaries
     double dots[3];      // dots of the last attempt (diagnostics)
 };
 
@@ -63,7 +63,7 @@ __host__ __device__ inline void ls_decide(LsState& s, double d0, double d1,
         s.k += 1;
         s.retries = 0;
         s.fresh = 1;
-        if (done % CHECK_EVERY == 0 || done >= s.maxit) s.frozen = 1;
+        if (!s.no_freeze && (done % CHECK_EVERY == 0 || done >= s.maxit)) s.frozen = 1;
     } else {
         s.eta = fmax(eta_new, 1e-12);
         s.fresh = 0;

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(BS) void k_primal(IterBufs<V> b) {
         acc += static_cast<double>(d) * static_cast<double>(d);
     }
     acc = block_sum(acc, red);
-    if (threadIdx.x == 0) store_agent(b.partT + blockIdx.x, acc);
+    if (threadIdx.x == 0) b.partT[blockIdx.x] = acc;
 }
 
 template <typename V>
@@ -155,7 +155,6 @@ __global__ __launch_bounds__(BS) void k_dual(IterBufs<V> b) {
     __shared__ V s_rows[ROWS_MAX];
     __shared__ V red_v[BS / WAVE];
     __shared__ double red[BS / WAVE];
-    __shared__ int s_flag;
     const LsState st = *b.st;
     if (st.frozen) return;
     const int cur = st.cur;
@@ -193,15 +192,21 @@ __global__ __launch_bounds__(BS) void k_dual(IterBufs<V> b) {
     }
     a1 = block_sum(a1, red);
     a2 = block_sum(a2, red);
-    const int nA = gridDim.x;
     if (threadIdx.x == 0) {
-        store_agent(b.partA + blockIdx.x, a1);
-        store_agent(b.partA + nA + blockIdx.x, a2);
+        b.partA[2 * blockIdx.x] = a1;
+        b.partA[2 * blockIdx.x + 1] = a2;
     }
-    if (!elect_last(b.ticket, &s_flag)) return;
-    const double d0 = fixed_sum(b.partT, b.rbT.nblk, red);
-    const double d1 = fixed_sum(b.partA, nA, red);
-    const double d2 = fixed_sum(b.partA + nA, nA, red);
+}
+
+// Line-search decision: fixed-order sums of the attempt's partials, then
+// the accept/reject rule on the device-resident state.
+template <typename V>
+__global__ __launch_bounds__(FIN) void k_decide(IterBufs<V> b, int nA) {
+    __shared__ double red[FIN / WAVE];
+    if (b.st->frozen) return;
+    const double d0 = fin_sum(b.partT, b.rbT.nblk, 1, red);
+    const double d1 = fin_sum(b.partA, nA, 2, red);
+    const double d2 = fin_sum(b.partA + 1, nA, 2, red);
     if (threadIdx.x == 0) {
         LsState s = *b.st;
         pdhg::ls_decide(s, d0, d1, d2);
@@ -225,8 +230,27 @@ template <typename V>
 static void attempt_impl(const IterBufs<V>& b, hipStream_t s) {
     if (b.rbT.nblk > 0) k_primal<V><<<b.rbT.nblk, BS, 0, s>>>(b);
     k_dual<V><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
+    k_decide<V><<<1, FIN, 0, s>>>(b, grid_of(b.rbA.nblk));
 }
 void launch_attempt(const IterBufs<double>& b, hipStream_t s) { attempt_impl(b, s); }
+void launch_primal(const IterBufs<double>& b, hipStream_t s) {
+    if (b.rbT.nblk > 0) k_primal<double><<<b.rbT.nblk, BS, 0, s>>>(b);
+}
+void launch_primal(const IterBufs<float>& b, hipStream_t s) {
+    if (b.rbT.nblk > 0) k_primal<float><<<b.rbT.nblk, BS, 0, s>>>(b);
+}
+void launch_dual(const IterBufs<double>& b, hipStream_t s) {
+    k_dual<double><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
+}
+void launch_dual(const IterBufs<float>& b, hipStream_t s) {
+    k_dual<float><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
+}
+void launch_decide(const IterBufs<double>& b, hipStream_t s) {
+    k_decide<double><<<1, FIN, 0, s>>>(b, grid_of(b.rbA.nblk));
+}
+void launch_decide(const IterBufs<float>& b, hipStream_t s) {
+    k_decide<float><<<1, FIN, 0, s>>>(b, grid_of(b.rbA.nblk));
+}
 void launch_attempt(const IterBufs<float>& b, hipStream_t s) { attempt_impl(b, s); }
 
 template <typename V>
@@ -288,10 +312,10 @@ __global__ __launch_bounds__(BS) void k_kkt_cols(KktData d, KktIter<V> it) {
     unb = block_max(unb, red);
     if (threadIdx.x == 0) {
         double* p = d.partC + (static_cast<long>(q) * gridDim.x + blockIdx.x) * 4;
-        store_agent(p + 0, dv);
-        store_agent(p + 1, op);
-        store_agent(p + 2, od);
-        store_agent(p + 3, unb);
+        p[0] = dv;
+        p[1] = op;
+        p[2] = od;
+        p[3] = unb;
     }
 }
 
@@ -300,7 +324,6 @@ __global__ __launch_bounds__(BS) void k_kkt_rows(KktData d, KktIter<V> it) {
     __shared__ double s_prod[NNZ_MAX];
     __shared__ double s_rows[ROWS_MAX];
     __shared__ double red[BS / WAVE];
-    __shared__ int s_flag;
     const int q = blockIdx.y;
     int r0, r1;
     block_rows(d.rbA, blockIdx.x, r0, r1);
@@ -329,36 +352,30 @@ __global__ __launch_bounds__(BS) void k_kkt_rows(KktData d, KktIter<V> it) {
     pa = block_max(pa, red);
     prow = block_max(prow, red);
     ymax = block_max(ymax, red);
-    const int nb = gridDim.x;
     if (threadIdx.x == 0) {
-        double* p = d.partR + (static_cast<long>(q) * nb + blockIdx.x) * 4;
-        store_agent(p + 0, pv);
-        store_agent(p + 1, pa);
-        store_agent(p + 2, prow);
-        store_agent(p + 3, ymax);
-    }
-    if (!elect_last(d.tickets + 0, &s_flag)) return;
-    for (int qq = 0; qq < 2; ++qq) {
-        double r[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int k = 0; k < 4; ++k) {
-            double s = 0.0;
-            for (int bb = threadIdx.x; bb < nb; bb += BS)
-                s = fmax(s, load_agent(d.partR + (static_cast<long>(qq) * nb + bb) * 4 + k));
-            r[k] = block_max(s, red);
-        }
-        if (threadIdx.x == 0)
-            for (int k = 0; k < 4; ++k) store_agent(d.rowres + qq * 4 + k, r[k]);
+        double* p = d.partR + (static_cast<long>(q) * gridDim.x + blockIdx.x) * 4;
+        p[0] = pv;
+        p[1] = pa;
+        p[2] = prow;
+        p[3] = ymax;
     }
 }
 
+// Row-pass maxima of both iterates -> rowres[q*4 + k].
+__global__ __launch_bounds__(FIN) void k_kkt_rowfin(KktData d, int nb) {
+    __shared__ double red[FIN / WAVE];
+    for (int q = 0; q < 2; ++q)
+        for (int k = 0; k < 4; ++k) {
+            const double r = fin_max(d.partR + static_cast<long>(q) * nb * 4 + k, nb, 4, red);
+            if (threadIdx.x == 0) d.rowres[q * 4 + k] = r;
+        }
+}
+
 template <typename V>
-__global__ __launch_bounds__(BS) void k_kkt_final(KktData d, KktIter<V> it,
-                                                  int nbC, Metrics* out) {
+__global__ __launch_bounds__(BS) void k_kkt_bound(KktData d, KktIter<V> it) {
     __shared__ double red[BS / WAVE];
-    __shared__ int s_flag;
     const int q = blockIdx.y;
-    const double ymax_q = d.rowres[q * 4 + 3];
-    const double yz = 1e-8 * (1.0 + ymax_q);
+    const double yz = 1e-8 * (1.0 + d.rowres[q * 4 + 3]);
     const GatherRef<V> gy{it.yref[q], it.yd[q]};
     double od = 0.0, unb = 0.0;
     for (long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x; i < d.m;
@@ -374,37 +391,30 @@ __global__ __launch_bounds__(BS) void k_kkt_final(KktData d, KktIter<V> it,
     }
     od = block_sum(od, red);
     unb = block_max(unb, red);
-    const int nb = gridDim.x;
     if (threadIdx.x == 0) {
-        double* p = d.partR2 + (static_cast<long>(q) * nb + blockIdx.x) * 2;
-        store_agent(p + 0, od);
-        store_agent(p + 1, unb);
+        double* p = d.partR2 + (static_cast<long>(q) * gridDim.x + blockIdx.x) * 2;
+        p[0] = od;
+        p[1] = unb;
     }
-    if (!elect_last(d.tickets + 1, &s_flag)) return;
-    for (int qq = 0; qq < 2; ++qq) {
-        double dv = 0.0, op = 0.0, odc = 0.0, unbc = 0.0, odr = 0.0, unbr = 0.0;
-        for (int bb = threadIdx.x; bb < nbC; bb += BS) {
-            const double* p = d.partC + (static_cast<long>(qq) * nbC + bb) * 4;
-            dv = fmax(dv, load_agent(p + 0));
-            op += load_agent(p + 1);
-            odc += load_agent(p + 2);
-            unbc = fmax(unbc, load_agent(p + 3));
-        }
-        for (int bb = threadIdx.x; bb < nb; bb += BS) {
-            const double* p = d.partR2 + (static_cast<long>(qq) * nb + bb) * 2;
-            odr += load_agent(p + 0);
-            unbr = fmax(unbr, load_agent(p + 1));
-        }
-        dv = block_max(dv, red);
-        op = block_sum(op, red);
-        odc = block_sum(odc, red);
-        unbc = block_max(unbc, red);
-        odr = block_sum(odr, red);
-        unbr = block_max(unbr, red);
+}
+
+// Final metrics of both iterates (pdhg.cu:195-253).
+__global__ __launch_bounds__(FIN) void k_kkt_done(KktData d, int nbC, int nbB,
+                                                  Metrics* out) {
+    __shared__ double red[FIN / WAVE];
+    for (int q = 0; q < 2; ++q) {
+        const double* pc = d.partC + static_cast<long>(q) * nbC * 4;
+        const double* pb = d.partR2 + static_cast<long>(q) * nbB * 2;
+        const double dv = fin_max(pc + 0, nbC, 4, red);
+        const double op = fin_sum(pc + 1, nbC, 4, red);
+        const double odc = fin_sum(pc + 2, nbC, 4, red);
+        const double unbc = fin_max(pc + 3, nbC, 4, red);
+        const double odr = fin_sum(pb + 0, nbB, 2, red);
+        const double unbr = fin_max(pb + 1, nbB, 2, red);
         if (threadIdx.x == 0) {
-            const double pv = load_agent(d.rowres + qq * 4 + 0);
-            const double pa = load_agent(d.rowres + qq * 4 + 1);
-            const double prow = load_agent(d.rowres + qq * 4 + 2);
+            const double pv = d.rowres[q * 4 + 0];
+            const double pa = d.rowres[q * 4 + 1];
+            const double prow = d.rowres[q * 4 + 2];
             Metrics mt;
             mt.pinf = fmax(pv, 0.0) / (1.0 + pa);
             mt.prow = prow;
@@ -415,7 +425,7 @@ __global__ __launch_bounds__(BS) void k_kkt_final(KktData d, KktIter<V> it,
             mt.od = bounded ? odv : -__builtin_huge_val();
             mt.gap = bounded ? fmax(0.0, op - odv) / (1.0 + fabs(op))
                              : __builtin_huge_val();
-            out[qq] = mt;
+            out[q] = mt;
         }
     }
 }
@@ -432,9 +442,12 @@ static void kkt_impl(const KktData& d, const KktIter<V>& it, Metrics* out,
                      hipStream_t s) {
     const int nbC = grid_of(d.rbT.nblk);
     const int nbR = grid_of(d.rbA.nblk);
+    const int nbB = kkt_rows2_blocks(d.m);
     k_kkt_cols<V><<<dim3(nbC, 2), BS, 0, s>>>(d, it);
     k_kkt_rows<V><<<dim3(nbR, 2), BS, 0, s>>>(d, it);
-    k_kkt_final<V><<<dim3(kkt_rows2_blocks(d.m), 2), BS, 0, s>>>(d, it, nbC, out);
+    k_kkt_rowfin<<<1, FIN, 0, s>>>(d, nbR);
+    k_kkt_bound<V><<<dim3(nbB, 2), BS, 0, s>>>(d, it);
+    k_kkt_done<<<1, FIN, 0, s>>>(d, nbC, nbB, out);
 }
 void launch_kkt(const KktData& d, const KktIter<double>& it, Metrics* out,
                 hipStream_t s) { kkt_impl(d, it, out, s); }
@@ -577,11 +590,8 @@ int reduce_blocks(long n) {
 
 template <typename V>
 __global__ __launch_bounds__(BS) void k_sumsq_diff(const V* a, const V* b,
-                                                   long n, double* part,
-                                                   unsigned int* ticket,
-                                                   double* out) {
+                                                   long n, double* part) {
     __shared__ double red[BS / WAVE];
-    __shared__ int s_flag;
     double s = 0.0;
     for (long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x; i < n;
          i += static_cast<long>(gridDim.x) * BS) {
@@ -590,28 +600,33 @@ __global__ __launch_bounds__(BS) void k_sumsq_diff(const V* a, const V* b,
         s += d * d;
     }
     s = block_sum(s, red);
-    if (threadIdx.x == 0) store_agent(part + blockIdx.x, s);
-    if (!elect_last(ticket, &s_flag)) return;
-    const double t = fixed_sum(part, gridDim.x, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(FIN) void k_fin(const double* part, int cnt,
+                                             int is_max, double* out) {
+    __shared__ double red[FIN / WAVE];
+    const double t = is_max ? fin_max(part, cnt, 1, red) : fin_sum(part, cnt, 1, red);
     if (threadIdx.x == 0) *out = t;
 }
 
 void sumsq_diff(const double* a, const double* b, long n, double* part,
-                unsigned int* ticket, double* out, hipStream_t s) {
-    k_sumsq_diff<double><<<reduce_blocks(n), BS, 0, s>>>(a, b, n, part, ticket, out);
+                unsigned int*, double* out, hipStream_t s) {
+    const int g = reduce_blocks(n);
+    k_sumsq_diff<double><<<g, BS, 0, s>>>(a, b, n, part);
+    k_fin<<<1, FIN, 0, s>>>(part, g, 0, out);
 }
 void sumsq_diff(const float* a, const float* b, long n, double* part,
-                unsigned int* ticket, double* out, hipStream_t s) {
-    k_sumsq_diff<float><<<reduce_blocks(n), BS, 0, s>>>(a, b, n, part, ticket, out);
+                unsigned int*, double* out, hipStream_t s) {
+    const int g = reduce_blocks(n);
+    k_sumsq_diff<float><<<g, BS, 0, s>>>(a, b, n, part);
+    k_fin<<<1, FIN, 0, s>>>(part, g, 0, out);
 }
 
 __global__ __launch_bounds__(BS) void k_sumsq_q(const double* srmin,
                                                 const double* srmax, long m,
-                                                double* part,
-                                                unsigned int* ticket,
-                                                double* out) {
+                                                double* part) {
     __shared__ double red[BS / WAVE];
-    __shared__ int s_flag;
     double s = 0.0;
     for (long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x; i < m;
          i += static_cast<long>(gridDim.x) * BS) {
@@ -621,35 +636,30 @@ __global__ __launch_bounds__(BS) void k_sumsq_q(const double* srmin,
         s += q * q;
     }
     s = block_sum(s, red);
-    if (threadIdx.x == 0) store_agent(part + blockIdx.x, s);
-    if (!elect_last(ticket, &s_flag)) return;
-    const double t = fixed_sum(part, gridDim.x, red);
-    if (threadIdx.x == 0) *out = t;
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 void sumsq_q(const double* srmin, const double* srmax, long m, double* part,
-             unsigned int* ticket, double* out, hipStream_t s) {
-    k_sumsq_q<<<reduce_blocks(m), BS, 0, s>>>(srmin, srmax, m, part, ticket, out);
+             unsigned int*, double* out, hipStream_t s) {
+    const int g = reduce_blocks(m);
+    k_sumsq_q<<<g, BS, 0, s>>>(srmin, srmax, m, part);
+    k_fin<<<1, FIN, 0, s>>>(part, g, 0, out);
 }
 
 __global__ __launch_bounds__(BS) void k_absmax(const double* v, long n,
-                                               double* part,
-                                               unsigned int* ticket,
-                                               double* out) {
+                                               double* part) {
     __shared__ double red[BS / WAVE];
-    __shared__ int s_flag;
     double s = 0.0;
     for (long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x; i < n;
          i += static_cast<long>(gridDim.x) * BS)
         s = fmax(s, fabs(v[i]));
     s = block_max(s, red);
-    if (threadIdx.x == 0) store_agent(part + blockIdx.x, s);
-    if (!elect_last(ticket, &s_flag)) return;
-    const double t = fixed_max(part, gridDim.x, red);
-    if (threadIdx.x == 0) *out = t;
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-void absmax(const double* v, long n, double* part, unsigned int* ticket,
+void absmax(const double* v, long n, double* part, unsigned int*,
             double* out, hipStream_t s) {
-    k_absmax<<<reduce_blocks(n), BS, 0, s>>>(v, n, part, ticket, out);
+    const int g = reduce_blocks(n);
+    k_absmax<<<g, BS, 0, s>>>(v, n, part);
+    k_fin<<<1, FIN, 0, s>>>(part, g, 1, out);
 }
 
 // ------------------------------------------------------ elementwise --

@@ -45,7 +45,7 @@ struct IterBufs {
     const V *gref = nullptr, *lo = nullptr, *hi = nullptr;   // primal data
     const V *rlo = nullptr, *rhi = nullptr, *yoff = nullptr; // dual data
     double* partT = nullptr;  // nblkT partials of |dx|^2
-    double* partA = nullptr;  // 2*nblkA partials of |dy|^2, dy'Adx
+    double* partA = nullptr;  // nblkA pairs (|dy|^2, dy'Adx)
     pdhg::LsState* st = nullptr;
     unsigned int* ticket = nullptr;
 };
@@ -54,6 +54,13 @@ struct IterBufs {
 // step + line-search decision (both no-ops once the state is frozen).
 void launch_attempt(const IterBufs<double>& b, hipStream_t s);
 void launch_attempt(const IterBufs<float>& b, hipStream_t s);
+// The two kernels of an attempt, separately (for timing).
+void launch_primal(const IterBufs<double>& b, hipStream_t s);
+void launch_primal(const IterBufs<float>& b, hipStream_t s);
+void launch_dual(const IterBufs<double>& b, hipStream_t s);
+void launch_dual(const IterBufs<float>& b, hipStream_t s);
+void launch_decide(const IterBufs<double>& b, hipStream_t s);
+void launch_decide(const IterBufs<float>& b, hipStream_t s);
 // Apply the averaging / A x commit of the last accepted step.
 void launch_flush(const IterBufs<double>& b, int cur, double w, hipStream_t s);
 void launch_flush(const IterBufs<float>& b, int cur, double w, hipStream_t s);

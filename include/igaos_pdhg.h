@@ -115,6 +115,28 @@ int igaos_pdhg_solve(const igaos_model* model, const igaos_options* opt,
                      const igaos_engine_config* cfg, double* x, double* y,
                      igaos_solve_info* info, char* msg, int msglen);
 
+/* Prepared engine (no reference counterpart; the reference re-uploads and
+ * re-scales on every call).  igaos_pdhg_create uploads the model and runs
+ * the setup of pdhg.cu:467-663 (Ruiz, scaling, power iteration) once;
+ * every igaos_pdhg_run is then a fresh solve from x = y = 0 on the
+ * HBM-resident data with the same results as igaos_pdhg_solve.  The model
+ * must outlive the engine.  create returns NULL on failure. */
+typedef struct igaos_engine igaos_engine;
+igaos_engine* igaos_pdhg_create(const igaos_model* model,
+                                const igaos_engine_config* cfg);
+int igaos_pdhg_run(igaos_engine* engine, const igaos_options* opt, double* x,
+                   double* y, igaos_solve_info* info, char* msg, int msglen);
+/* Setup time of the engine in milliseconds. */
+double igaos_pdhg_setup_ms(const igaos_engine* engine);
+void igaos_pdhg_destroy(igaos_engine* engine);
+/* Timing (no reference counterpart): runs `attempts` line-search attempts
+ * without checks and reports the mean device time per launch of the two
+ * attempt kernels, measured with HIP events on the engine's stream, and
+ * their grid sizes.  A later igaos_pdhg_run starts afresh. */
+int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
+                                double* ms_primal, double* ms_dual,
+                                int* blocks_primal, int* blocks_dual);
+
 /* ---- kernel layer (device pointers, HIP stream or NULL) ----------------- */
 /* Host: row-block layout of a CSR row pointer (host) for the SpMV
  * kernels.  Writes at most cap entries; returns the number of entries

@@ -90,6 +90,12 @@ def load():
         lib.orc_dual_repair.argtypes = [C.POINTER(_Model), _dp, _dp, _dp,
                                         C.c_double, _dp]
         lib.orc_set_mutation.argtypes = [C.c_int]
+        lib.orc_avg_weight.restype = C.c_double
+        lib.orc_avg_weight.argtypes = [C.c_long]
+        lib.orc_mix_weights.argtypes = [C.c_int, _dp]
+        lib.orc_ls_decide.restype = C.c_int
+        lib.orc_ls_decide.argtypes = [C.c_long, _dp, C.c_double, _dp,
+                                      C.POINTER(C.c_int)]
         _lib = lib
     return _lib
 
@@ -234,6 +240,23 @@ def evaluate(rmin, rmax, cl, cu, c, x, y, Ax, d) -> Metrics:
     arrs = [_d(a) for a in (rmin, rmax, cl, cu, c, x, y, Ax, d)]
     return lib.orc_evaluate(arrs[0].shape[0], arrs[2].shape[0],
                             *[_pd(a) for a in arrs])
+
+
+def mix_weights(count):
+    """Weight of each of `count` accepted steps in the running average."""
+    out = np.zeros(count)
+    load().orc_mix_weights(int(count), _pd(out))
+    return out
+
+
+def ls_decide(k, eta, omega, dots, retries=0):
+    """One line-search decision: returns (accept, new_eta, new_retries)."""
+    e = C.c_double(eta)
+    r = C.c_int(retries)
+    d = _d(dots)
+    acc = load().orc_ls_decide(int(k), C.byref(e), float(omega), _pd(d),
+                               C.byref(r))
+    return bool(acc), e.value, r.value
 
 
 def dual_repair(lp, x, Ax, y_it, col_tol):

@@ -206,6 +206,53 @@ void orc_dots(int n, int m, const double* dx, const double* dy,
     free(part);
 }
 
+
+/* Averaging weight of the hn-th accepted step of an epoch, pdhg.cu:795. */
+double orc_avg_weight(long hn) {
+    if (g_mut == 5) return 1.0 / ((double)hn + 1.0);
+    return 2.0 / ((double)hn + 2.0);
+}
+
+/* kern_mix2's running average, pdhg.cu:127-141. */
+static void mix_into(int len, double w, const double* v, double* avg) {
+    for (int i = 0; i < len; ++i) avg[i] += w * (v[i] - avg[i]);
+}
+
+/* Averages `count` basis vectors e_0..e_{count-1} with the same recurrence
+ * the solve uses; out[j] is then the weight of step j in the average. */
+void orc_mix_weights(int count, double* out) {
+    double* e = (double*)calloc((size_t)(count > 0 ? count : 1), sizeof(double));
+    for (int j = 0; j < count; ++j) out[j] = 0.0;
+    for (long h = 0; h < count; ++h) {
+        for (int j = 0; j < count; ++j) e[j] = (j == h) ? 1.0 : 0.0;
+        mix_into(count, orc_avg_weight(h), e, out);
+    }
+    free(e);
+}
+
+/* Line-search controller, pdhg.cu:770-793.  Returns 1 on accept; updates
+ * eta and the retry counter exactly as the reference does. */
+int orc_ls_decide(long k, double* eta, double omega, const double dots[3],
+                  int* retries) {
+    const double INF = INFINITY;
+    const double num = omega * dots[0] + dots[1] / omega;
+    const double cross = dots[2];
+    const double etabar = (num > 0.0 && cross > 0.0) ? num / (2.0 * cross) : INF;
+    const double kk = (double)k;
+    const double eta_new = isfinite(etabar)
+                               ? dmin((1.0 - pow(kk, -0.3)) * etabar,
+                                      (1.0 + pow(kk, -0.6)) * *eta)
+                               : INF;
+    const int accept = !(*eta > etabar) || num <= 0.0 || cross <= 0.0 ||
+                       ++*retries > 20;
+    if (accept) {
+        if (isfinite(eta_new) && eta_new > 1e-12) *eta = dmin(eta_new, 1e12);
+        return 1;
+    }
+    *eta = dmax(eta_new, 1e-12);
+    return 0;
+}
+
 /* KKT evaluation in original units, pdhg.cu:173-254. */
 orc_metrics orc_evaluate(int m, int n, const double* rmin,
                          const double* rmax, const double* cl,
@@ -588,35 +635,18 @@ int orc_pdhg_solve(const orc_model* lp, const orc_options* opt,
                 adx[i] = 0.5 * (tm[i] - axcur[i]);
             }
             orc_dots(n, m, dx, dy, adx, hdots);
-            /* line-search controller, pdhg.cu:770-793 */
-            const double num = omega * hdots[0] + hdots[1] / omega;
-            const double cross = hdots[2];
-            const double etabar =
-                (num > 0.0 && cross > 0.0) ? num / (2.0 * cross) : INF;
-            const double kk = (double)k;
-            const double eta_new =
-                isfinite(etabar)
-                    ? dmin((1.0 - pow(kk, -0.3)) * etabar,
-                           (1.0 + pow(kk, -0.6)) * eta)
-                    : INF;
-            const int accept = !(eta > etabar) || num <= 0.0 ||
-                               cross <= 0.0 || ++retries > 20;
-            if (accept) {
-                if (isfinite(eta_new) && eta_new > 1e-12)
-                    eta = dmin(eta_new, 1e12);
+            if (orc_ls_decide(k, &eta, omega, hdots, &retries)) {
                 double* t1 = x; x = xnew; xnew = t1;
                 double* t2 = y; y = ynew; ynew = t2;
                 break;
             }
-            eta = dmax(eta_new, 1e-12);
         }
         if (trace_eta && k - 1 < trace_cap) trace_eta[k - 1] = eta;
         /* kern_mix2, pdhg.cu:127-141,795-799 */
-        double weta = g_mut == 5 ? 1.0 / ((double)hn + 1.0)
-                                 : 2.0 / ((double)hn + 2.0);
+        double weta = orc_avg_weight(hn);
         ++hn;
-        for (int j = 0; j < n; ++j) xavg[j] += weta * (x[j] - xavg[j]);
-        for (int i = 0; i < m; ++i) yavg[i] += weta * (y[i] - yavg[i]);
+        mix_into(n, weta, x, xavg);
+        mix_into(m, weta, y, yavg);
         for (int i = 0; i < m; ++i) axcur[i] += adx[i];
 
         if (k % CHECK != 0) continue;

@@ -65,6 +65,10 @@ orc_metrics orc_evaluate(int m, int n, const double* rmin,
                          const double* rmax, const double* cl,
                          const double* cu, const double* c, const double* x,
                          const double* y, const double* Ax, const double* d);
+double orc_avg_weight(long hn);
+void orc_mix_weights(int count, double* out);
+int orc_ls_decide(long k, double* eta, double omega, const double dots[3],
+                  int* retries);
 int orc_dual_repair(const orc_model* lp, const double* x, const double* Ax,
                     const double* y_it, double col_tol, double* y);
 /* Full host replay of pdhg::solve.  x_out (n) and y_out (m) receive the

@@ -1,0 +1,198 @@
+"""The C-ABI boundary, the MPS reader and the host-side API — CPU only.
+
+No compute is launched here: these tests check that libigaos_pdhg.so
+loads and exports every entry point include/igaos_pdhg.h declares, that
+the model reader keeps the reference's MPS semantics (the edge cases of
+tests/engine_smoke.cpp in the reference), that the Python and CLI
+surfaces keep the reference's names, defaults and error behaviour, and
+the row-block layout invariants the SpMV kernels rely on.
+"""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import igaos_amd
+from igaos_amd import _core, lpgen
+from tests.conftest import ROOT, netlib_path
+
+HEADER = os.path.join(ROOT, "include", "igaos_pdhg.h")
+CLI = os.path.join(ROOT, "igaos_amd", "igaos")
+INF = math.inf
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(igaos_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _core.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    # and the ctypes table covers them all with signatures
+    from igaos_amd import _batch_sigs
+    table = set(_core.SIGNATURES) | set(_batch_sigs.SIGNATURES)
+    assert set(syms) <= table, sorted(set(syms) - table)
+
+
+def test_abi_and_status_names():
+    L = _core.lib()
+    assert L.igaos_pdhg_abi_version() == 1
+    names = [L.igaos_status_name(i).decode() for i in range(8)]
+    assert names == ["optimal", "near-optimal", "feasible", "infeasible",
+                     "unbounded", "iteration-limit", "time-limit", "error"]
+    assert [s.name for s in igaos_amd.Status] == [
+        "Optimal", "NearOptimal", "Feasible", "Infeasible", "Unbounded",
+        "IterationLimit", "TimeLimit", "Error"]
+
+
+def test_option_defaults_match_reference():
+    import ctypes as C
+    o = _core.Options()
+    _core.lib().igaos_options_default(C.byref(o))
+    # include/igaos/options.h:5-13 of the reference
+    assert (o.time_limit_s, o.tolerance, o.max_iterations, o.seed,
+            o.presolve, o.restarts, o.verbosity) == (120.0, 1e-4, 500000, 0,
+                                                     1, 0, 0)
+
+
+def _mps(tmp_path, text):
+    p = tmp_path / "t.mps"
+    p.write_text(text)
+    return igaos_amd.read_mps(str(p))
+
+
+def test_mps_negative_up_frees_lower_bound(tmp_path):
+    m = _mps(tmp_path, "NAME T1\nROWS\n N obj\n G r1\nCOLUMNS\n x obj 1.0 r1 1.0\n"
+                       "RHS\n RHS1 r1 -5.0\nBOUNDS\n UP BND x -3.0\nENDATA\n")
+    a = m.arrays()
+    assert a["cl"][0] == -INF and a["cu"][0] == -3.0 and a["rmin"][0] == -5.0
+
+
+def test_mps_mi_ranges_objconst_fr(tmp_path):
+    m = _mps(tmp_path, "NAME T2\nROWS\n N obj\n G r1\nCOLUMNS\n x obj 1.0 r1 1.0\n"
+                       "RHS\n RHS1 r1 -2.0\nBOUNDS\n MI BND x\nENDATA\n")
+    a = m.arrays()
+    assert a["cl"][0] == -INF and a["cu"][0] == INF
+    m = _mps(tmp_path, "NAME T3\nROWS\n N obj\n G r1\nCOLUMNS\n x obj 1.0 r1 1.0\n"
+                       "RHS\n RHS1 obj -7.113 r1 2.0\nRANGES\n RNG r1 3.0\nENDATA\n")
+    a = m.arrays()
+    assert m.n_ranges_parsed == 1 and abs(m.obj_const - 7.113) < 1e-12
+    assert a["rmin"][0] == 2.0 and a["rmax"][0] == 5.0
+    m = _mps(tmp_path, "NAME T4\nROWS\n N obj\n E r1\n G r2\nCOLUMNS\n x obj 1.0 r1 1.0\n"
+                       " x r2 0.0\n y obj 2.0 r1 1.0\n y r2 1.0\nRHS\n RHS1 r1 4.0 r2 1.0\n"
+                       "BOUNDS\n FR BND y\nENDATA\n")
+    assert m.n_fr_parsed == 1 and m.nnz() == 4
+    a = m.arrays()
+    assert a["cl"][1] == -INF and a["cu"][1] == INF
+
+
+def test_mps_ranges_sign_rules_and_markers(tmp_path):
+    m = _mps(tmp_path,
+             "NAME T\nROWS\n N obj\n E e1\n E e2\n L l1\n G g1\nCOLUMNS\n"
+             " M1 'MARKER' 'INTORG'\n x obj 1.0 e1 1.0\n M2 'MARKER' 'INTEND'\n"
+             " x e2 1.0 l1 1.0\n x g1 1.0\nRHS\n R e1 1.0 e2 1.0 l1 4.0 g1 -1.0\n"
+             "RANGES\n RR e1 2.0 e2 -2.0\n RR l1 -3.0 g1 -3.0\nBOUNDS\n"
+             " BV B x\nENDATA\n")
+    a = m.arrays()
+    assert list(a["rmin"]) == [1.0, -1.0, 1.0, -1.0]
+    assert list(a["rmax"]) == [3.0, 1.0, 4.0, 2.0]
+    assert m.n_int == 1 and a["cu"][0] == 1.0
+
+
+def test_mps_afiro_stats_and_csc_is_transpose():
+    m = igaos_amd.read_mps(netlib_path("afiro"))
+    assert (m.m, m.n, m.nnz()) == (27, 32, 83)
+    lp = m.to_lp()
+    A = lp.to_scipy()
+    import scipy.sparse as sp
+    At = sp.csr_matrix((lp.acx, lp.ci, lp.cp), shape=(lp.n, lp.m))
+    assert abs(A.T - At).max() == 0
+
+
+def test_triplet_order_matches_counting_sort():
+    import ctypes as C
+    rng = np.random.default_rng(0)
+    m, n, z = 7, 9, 30
+    rows = rng.integers(0, m, z).astype(np.int32)
+    cols = rng.integers(0, n, z).astype(np.int32)
+    vals = rng.standard_normal(z)
+    ref = lpgen.LP.from_triplets(m, n, rows, cols, vals, np.zeros(n),
+                                 np.zeros(n), np.ones(n), np.zeros(m),
+                                 np.ones(m))
+    dp = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))  # noqa
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa
+    h = _core.lib().igaos_model_from_triplets(
+        m, n, z, ip(rows), ip(cols), dp(vals), dp(np.zeros(n)),
+        dp(np.zeros(n)), dp(np.ones(n)), dp(np.zeros(m)), dp(np.ones(m)), 0.0)
+    a = igaos_amd.Model(h).arrays()
+    for k in ("ap", "ai", "ax", "cp", "ci", "acx"):
+        assert np.array_equal(a[k], getattr(ref, k)), k
+
+
+def test_invalid_engine_raises_and_other_engines_report_error():
+    with pytest.raises(ValueError):
+        igaos_amd.solve(netlib_path("afiro"), engine="bogus")
+    s = igaos_amd.solve(netlib_path("afiro"), engine="simplex")
+    assert s.status == "error" and "not part of this build" in s.message
+
+
+@pytest.mark.skipif(_core.device_count() > 0, reason="host has a GPU")
+def test_no_gpu_gives_honest_error():
+    # the reference's binding test expects 'CUDA' in the message of a
+    # build without the GPU engine (demo/test_bindings.py:84-86)
+    s = igaos_amd.solve(netlib_path("afiro"), engine="pdhg")
+    assert s.status == "error" and "CUDA" in s.message
+    assert not igaos_amd.has_pdhg
+
+
+def test_cli_info_and_non_pdhg_engine():
+    out = subprocess.run([CLI, "info", netlib_path("afiro")], check=True,
+                         capture_output=True, text=True).stdout
+    assert "rows           27" in out and "nnz            83" in out
+    assert "row types      E=8 L=19 G=0 ranged=0" in out
+    r = subprocess.run([CLI, "solve", netlib_path("afiro"), "--engine", "qp"],
+                       capture_output=True, text=True)
+    import json
+    js = json.loads(r.stdout)
+    assert js["status"] == "error" and js["engine"] == "qp"
+    assert subprocess.run([CLI, "solve", netlib_path("afiro"), "--engine",
+                           "bogus"], capture_output=True).returncode == 2
+
+
+def _rowblocks(ptr):
+    import ctypes as C
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    L = _core.lib()
+    k = L.igaos_rowblocks_build(len(ptr) - 1, ptr.ctypes.data_as(C.POINTER(C.c_int)), None, 0)
+    out = np.zeros(k, np.int32)
+    L.igaos_rowblocks_build(len(ptr) - 1, ptr.ctypes.data_as(C.POINTER(C.c_int)),
+                            out.ctypes.data_as(C.POINTER(C.c_int)), k)
+    return out
+
+
+@pytest.mark.parametrize("gen", ["zipf", "eq", "empty"])
+def test_rowblock_invariants(gen):
+    if gen == "zipf":
+        lp = lpgen.gen_zipf_badly_scaled(seed=1, m=3000, n=20000)
+        ptrs = [lp.ap, lp.cp]
+    elif gen == "eq":
+        lp = lpgen.gen_random_eq(seed=1)
+        ptrs = [lp.ap, lp.cp]
+    else:
+        ptrs = [np.zeros(5, np.int32), np.array([0, 0, 3000, 3000, 3001])]
+    for ptr in ptrs:
+        rb = _rowblocks(ptr)
+        nrows = len(ptr) - 1
+        assert rb[0] == 0 and rb[-1] == nrows and np.all(np.diff(rb) > 0)
+        for a, b in zip(rb[:-1], rb[1:]):
+            nz = ptr[b] - ptr[a]
+            assert b - a <= 256
+            assert nz <= 2048 or b - a == 1

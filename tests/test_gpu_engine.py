@@ -1,0 +1,235 @@
+"""GPU parity of the MI355X PDHG engine against the oracle and fixtures.
+
+Every case calls the HIP library through its C-ABI (libigaos_pdhg.so).
+Bars: integer/index work and the Ruiz factors are bit-exact; FP64 SpMV
+agrees with the oracle's sequential row sums to 1e-14 relative; the plain
+FP64 engine matches the oracle's trajectory to 1e-9 relative over one
+check period (64 iterations; beyond that the line search amplifies
+last-bit rounding differences by about 10x every 8 iterations, see
+tests/test_oracle.py and the parity-mode tests); every solve to
+tolerance 1e-6 certifies (near-optimal) and lands within 1e-6 relative
+of the reference simplex / HiGHS objective.
+"""
+import ctypes as C
+import json
+import math
+import subprocess
+
+import numpy as np
+import pytest
+
+import igaos_amd
+import oracle
+from igaos_amd import _core, lpgen
+from tests.conftest import ROOT, netlib_path
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev(a, dtype):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype=dtype, device="cuda")
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _rowblocks(ptr):
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    L = _core.lib()
+    k = L.igaos_rowblocks_build(len(ptr) - 1, ptr.ctypes.data_as(C.POINTER(C.c_int)), None, 0)
+    out = np.zeros(k, np.int32)
+    L.igaos_rowblocks_build(len(ptr) - 1, ptr.ctypes.data_as(C.POINTER(C.c_int)),
+                            out.ctypes.data_as(C.POINTER(C.c_int)), k)
+    return out
+
+
+def _gpu_spmv(ptr, idx, val, x, f32=False):
+    rb = _rowblocks(ptr)
+    fdt = torch.float32 if f32 else torch.float64
+    d_rb, d_ptr, d_idx = (_dev(rb, torch.int32), _dev(ptr, torch.int32),
+                          _dev(idx, torch.int32))
+    d_val, d_x = _dev(val, fdt), _dev(x, fdt)
+    d_y = torch.full((len(ptr) - 1,), float("nan"), dtype=fdt, device="cuda")
+    fn = _core.lib().igaos_spmv_csr_f32 if f32 else _core.lib().igaos_spmv_csr_f64
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = fn(len(rb) - 1, _ptr(d_rb), _ptr(d_ptr), _ptr(d_idx), _ptr(d_val),
+            _ptr(d_x), _ptr(d_y), stream)
+    assert rc == 0, _core.last_error()
+    torch.cuda.synchronize()
+    return d_y.cpu().numpy().astype(np.float64)
+
+
+def _long_row_matrix(rng):
+    # rows of 0, 1, 5, 300, 2048, 2049 and 9000 nonzeros (vector mode > 2048)
+    lens = [0, 1, 5, 300, 2048, 2049, 9000, 3, 0, 64]
+    n = 12000
+    ptr = np.zeros(len(lens) + 1, np.int64)
+    ptr[1:] = np.cumsum(lens)
+    idx = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int32)
+    return ptr.astype(np.int32), idx, rng.standard_normal(ptr[-1]), n
+
+
+@pytest.mark.parametrize("case", ["random_eq", "zipf", "long_rows"])
+def test_spmv_f64_matches_oracle(case):
+    rng = np.random.default_rng(11)
+    if case == "long_rows":
+        ptr, idx, val, n = _long_row_matrix(rng)
+        mats = [(ptr, idx, val, n)]
+    else:
+        lp = (lpgen.gen_random_eq(seed=3) if case == "random_eq"
+              else lpgen.gen_zipf_badly_scaled(seed=3, m=2000, n=5000))
+        mats = [(lp.ap, lp.ai, lp.ax, lp.n), (lp.cp, lp.ci, lp.acx, lp.m)]
+    for ptr, idx, val, ncols in mats:
+        x = rng.standard_normal(ncols)
+        ref = oracle.spmv(ptr, idx, val, x)
+        got = _gpu_spmv(ptr, idx, val, x)
+        scale = np.abs(val).max() * np.abs(x).max() * np.maximum(np.diff(ptr), 1)
+        assert np.all(np.abs(got - ref) <= 1e-14 * scale), case
+
+
+def test_spmv_f32_matches_oracle():
+    rng = np.random.default_rng(12)
+    lp = lpgen.gen_random_eq(seed=4)
+    x = rng.standard_normal(lp.n).astype(np.float32).astype(np.float64)
+    val = lp.ax.astype(np.float32).astype(np.float64)
+    ref = oracle.spmv(lp.ap, lp.ai, val, x)
+    got = _gpu_spmv(lp.ap, lp.ai, val, x, f32=True)
+    bound = 2e-7 * np.abs(val).max() * np.abs(x).max() * np.diff(lp.ap)
+    assert np.all(np.abs(got - ref) <= bound)
+
+
+def test_ruiz_bit_exact():
+    lp = lpgen.gen_zipf_badly_scaled(seed=5, m=500, n=1500)
+    dr_ref, dc_ref = oracle.ruiz(lp)
+    t = {k: _dev(getattr(lp, k), torch.int32 if k in ("ap", "ai", "cp", "ci") else torch.float64)
+         for k in ("ap", "ai", "ax", "cp", "ci", "acx")}
+    dr = torch.zeros(lp.m, dtype=torch.float64, device="cuda")
+    dc = torch.zeros(lp.n, dtype=torch.float64, device="cuda")
+    rc = _core.lib().igaos_ruiz_f64(lp.m, lp.n, _ptr(t["ap"]), _ptr(t["ai"]),
+                                    _ptr(t["ax"]), _ptr(t["cp"]), _ptr(t["ci"]),
+                                    _ptr(t["acx"]), _ptr(dr), _ptr(dc),
+                                    C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(dr.cpu().numpy(), dr_ref)
+    assert np.array_equal(dc.cpu().numpy(), dc_ref)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("K", [16, 64])
+def test_fixed_iteration_trajectory_parity(K):
+    lp = lpgen.gen_random_eq(seed=0)
+    r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
+    s = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
+    assert s.iterations == r.iterations == K
+    assert s.info["attempts"] == r.attempts
+    assert s.info["restarts"] == r.restarts
+    assert s.status == r.status == "iteration-limit"
+    assert _rel(s.x, r.x) <= 1e-9 and _rel(s.y, r.y) <= 1e-9
+    assert math.isclose(s.info["lam2"], r.lam2, rel_tol=1e-12)
+    assert math.isclose(s.info["omega0"], r.omega0, rel_tol=1e-14)
+
+
+def _check_solution(s, ref_obj, tol=1e-6):
+    assert s.status == "near-optimal", (s.status, s.message)
+    assert s.pinf <= tol and s.rel_gap <= tol
+    assert abs(s.objective - ref_obj) <= tol * (1 + abs(ref_obj)), (s.objective, ref_obj)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_random_eq_solves_to_fixture(precision, objectives):
+    lp = lpgen.gen_random_eq(seed=0)
+    ent = objectives["gen_random_eq(seed=0)"]
+    s = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=60, precision=precision)
+    _check_solution(s, ent["ref_simplex"])
+    _check_solution(s, ent["scipy_highs"])
+    # the reported pair satisfies the KKT bar in the oracle's own evaluation
+    A = lp.to_scipy()
+    mt = oracle.evaluate(lp.rmin, lp.rmax, lp.cl, lp.cu, lp.c, s.x, s.y,
+                         A @ s.x, lp.c + A.T @ s.y)
+    assert mt.pinf <= 1e-6 and mt.gap <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["afiro", "sc50a", "adlittle", "sc205"])
+def test_netlib_solves_match_reference_highs(name, objectives):
+    s = igaos_amd.solve(netlib_path(name), engine="pdhg", tolerance=1e-6,
+                        time_limit=120)
+    ref = objectives[f"netlib/{name}"]["highs_ref"]
+    assert s.status == "near-optimal", s.message
+    assert abs(s.objective - ref) <= 1e-6 * (1 + abs(ref)) + 5e-10 * abs(ref)
+
+
+ENGINE_SMOKE = [
+    ("NAME T1\nROWS\n N obj\n G r1\nCOLUMNS\n x obj 1.0 r1 1.0\n"
+     "RHS\n RHS1 r1 -5.0\nBOUNDS\n UP BND x -3.0\nENDATA\n", -5.0),
+    ("NAME T2\nROWS\n N obj\n G r1\nCOLUMNS\n x obj 1.0 r1 1.0\n"
+     "RHS\n RHS1 r1 -2.0\nBOUNDS\n MI BND x\nENDATA\n", -2.0),
+    ("NAME T3\nROWS\n N obj\n G r1\nCOLUMNS\n x obj 1.0 r1 1.0\n"
+     "RHS\n RHS1 obj -7.113 r1 2.0\nRANGES\n RNG r1 3.0\nENDATA\n", 2.0 + 7.113),
+    ("NAME T4\nROWS\n N obj\n E r1\n G r2\nCOLUMNS\n x obj 1.0 r1 1.0\n"
+     " x r2 0.0\n y obj 2.0 r1 1.0\n y r2 1.0\nRHS\n RHS1 r1 4.0 r2 1.0\n"
+     "BOUNDS\n FR BND y\nENDATA\n", 5.0),
+]
+
+
+@pytest.mark.parametrize("idx", range(len(ENGINE_SMOKE)))
+def test_reference_engine_smoke_lps(idx, tmp_path):
+    text, obj = ENGINE_SMOKE[idx]
+    p = tmp_path / "t.mps"
+    p.write_text(text)
+    s = igaos_amd.solve(str(p), engine="pdhg", tolerance=1e-8, time_limit=30)
+    assert s.status == "near-optimal" and abs(s.objective - obj) < 1e-6
+
+
+def test_iteration_cap_and_time_limit():
+    lp = lpgen.gen_random_eq(seed=2, m=60, n=150)
+    s = igaos_amd.solve_lp(lp, tolerance=0.0, time_limit=1e7, max_iterations=100)
+    r = oracle.solve(lp, tolerance=0.0, time_limit=1e7, max_iterations=100)
+    assert s.iterations == r.iterations == 100 and s.status == "iteration-limit"
+    s = igaos_amd.solve_lp(lp, tolerance=0.0, time_limit=1e-9)
+    assert s.status == "time-limit" and s.iterations == 64
+    assert s.message == "wall-clock budget exhausted"
+
+
+def test_run_to_run_bit_identical():
+    lp = lpgen.gen_zipf_badly_scaled(seed=1, m=2000, n=5000)
+    a = igaos_amd.solve_lp(lp, tolerance=1e-5, time_limit=60)
+    b = igaos_amd.solve_lp(lp, tolerance=1e-5, time_limit=60)
+    assert a.iterations == b.iterations
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_badly_scaled_and_multiperiod(precision, objectives):
+    cases = [("gen_zipf_badly_scaled(seed=0,m=400,n=1000)",
+              lpgen.gen_zipf_badly_scaled(seed=0, m=400, n=1000)),
+             ("gen_multiperiod(seed=0,periods=2,rows_per=200,cols_per=320,inv=40)",
+              lpgen.gen_multiperiod(seed=0, periods=2, rows_per=200,
+                                    cols_per=320, inv=40))]
+    for key, lp in cases:
+        s = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=120,
+                               precision=precision)
+        _check_solution(s, objectives[key]["scipy_highs"])
+
+
+def test_graph_and_direct_launches_agree():
+    lp = lpgen.gen_random_eq(seed=1, m=200, n=600)
+    a = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=60, graph_slots=16)
+    b = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=60, graph_slots=0)
+    assert a.iterations == b.iterations and np.array_equal(a.x, b.x)
+
+
+def test_cli_pdhg_json():
+    out = subprocess.run([f"{ROOT}/igaos_amd/igaos", "solve", netlib_path("afiro"),
+                          "--engine", "pdhg", "--tol", "1e-6"],
+                         capture_output=True, text=True, check=True).stdout
+    js = json.loads(out)
+    assert js["status"] == "near-optimal" and js["engine"] == "pdhg"
+    assert abs(js["objective"] + 464.7531429) < 1e-3
+    assert len(js["x"]) == 32 and len(js["y"]) == 27
