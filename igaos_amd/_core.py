@@ -94,6 +94,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with igaos_amd.build.build()")
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7
+        # with the same soname as /opt/rocm's.  Loading torch first makes
+        # the library bind to that runtime, so device pointers and streams
+        # handed over from torch tensors are valid for the library.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

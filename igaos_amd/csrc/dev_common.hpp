@@ -109,48 +109,117 @@ struct GatherRef {
     }
 };
 
-// Row sums of one row block [r0, r1): s_rows[r - r0] = sum_p val[p]*x(col).
-// `s_prod` holds NNZ_MAX entries of Acc, `s_rows` ROWS_MAX entries, `red`
-// BS/WAVE entries.  Contains __syncthreads(); all threads must call it.
+// Row-block layout: rb[2b] = first row of block b, rb[2b+1] = its first
+// nonzero; the pair after the last block holds (nrows, nnz).
+struct RowBlocks {
+    const int* rb = nullptr;
+    int nblk = 0;
+};
+
+__device__ __forceinline__ void block_rows(const RowBlocks& rb, int blk,
+                                           int& r0, int& r1, int& p0,
+                                           int& p1) {
+    if (blk < rb.nblk) {
+        const int2 u = *reinterpret_cast<const int2*>(rb.rb + 2 * blk);
+        const int2 v = *reinterpret_cast<const int2*>(rb.rb + 2 * blk + 2);
+        r0 = u.x;
+        p0 = u.y;
+        r1 = v.x;
+        p1 = v.y;
+    } else {
+        r0 = r1 = p0 = p1 = 0;
+    }
+}
+
+// Row sums of one row block: s_rows[r - r0] = sum_p val[p] * x(col[p]).
+// `s_prod` holds NNZ_MAX entries of Acc, `s_rows` ROWS_MAX, `red`
+// BS/WAVE.  Contains __syncthreads(); all threads must call it.
+//
+// Multi-row block: every thread issues its (at most NNZ_MAX/BS = 8) index
+// and value loads first, then the 8 gathers, then the products into LDS,
+// so a thread has 8 independent HBM requests in flight.  2^k threads per
+// row then reduce: inside a wavefront with xor shuffles, across the 2 or 4
+// wavefronts of a block through LDS when the block holds only 1-2 rows.
+// Single row longer than NNZ_MAX: the workgroup strides over it four
+// loads deep per thread.
 template <typename Acc, typename Val, typename G>
 __device__ __forceinline__ void rowblock_sums(const int* __restrict__ ptr,
                                               const int* __restrict__ idx,
                                               const Val* __restrict__ val,
                                               const G& gx, int r0, int r1,
-                                              Acc* s_rows, Acc* s_prod,
-                                              Acc* red) {
+                                              int p0, int p1, Acc* s_rows,
+                                              Acc* s_prod, Acc* red) {
     const int tid = threadIdx.x;
     const int nrows = r1 - r0;
-    const int p0 = ptr[r0];
-    const int p1 = ptr[r1];
     const int nz = p1 - p0;
     if (nrows == 1 && nz > NNZ_MAX) {
-        Acc s = Acc(0);
-        for (int p = p0 + tid; p < p1; p += BS)
-            s += static_cast<Acc>(val[p]) * static_cast<Acc>(gx(idx[p]));
-        s = block_sum(s, red);
+        Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
+        int p = p0 + tid;
+        for (; p + 3 * BS < p1; p += 4 * BS) {
+            const int c0 = idx[p], c1 = idx[p + BS], c2 = idx[p + 2 * BS],
+                      c3 = idx[p + 3 * BS];
+            const Val v0 = val[p], v1 = val[p + BS], v2 = val[p + 2 * BS],
+                      v3 = val[p + 3 * BS];
+            s0 += static_cast<Acc>(v0) * static_cast<Acc>(gx(c0));
+            s1 += static_cast<Acc>(v1) * static_cast<Acc>(gx(c1));
+            s2 += static_cast<Acc>(v2) * static_cast<Acc>(gx(c2));
+            s3 += static_cast<Acc>(v3) * static_cast<Acc>(gx(c3));
+        }
+        for (; p < p1; p += BS) s0 += static_cast<Acc>(val[p]) * static_cast<Acc>(gx(idx[p]));
+        Acc s = block_sum((s0 + s1) + (s2 + s3), red);
         if (tid == 0) s_rows[0] = s;
         __syncthreads();
         return;
     }
-    for (int q = tid; q < nz; q += BS)
-        s_prod[q] = static_cast<Acc>(val[p0 + q]) *
-                    static_cast<Acc>(gx(idx[p0 + q]));
-    __syncthreads();
     int tpr = 1;
-    while (tpr < WAVE && tpr * 2 * nrows <= BS) tpr *= 2;
+    while (tpr < BS && tpr * 2 * nrows <= BS) tpr *= 2;
     const int row = tid / tpr;
     const int lane = tid & (tpr - 1);
-    Acc s = Acc(0);
-    if (row < nrows) {
-        const int a = ptr[r0 + row] - p0;
-        const int b = ptr[r0 + row + 1] - p0;
-        for (int q = a + lane; q < b; q += tpr) s += s_prod[q];
+    int a = 0, b = 0;
+    if (row < nrows) {  // row bounds, loaded early and independently
+        a = ptr[r0 + row] - p0;
+        b = ptr[r0 + row + 1] - p0;
     }
-    for (int o = tpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (row < nrows && lane == 0) s_rows[row] = s;
+    constexpr int U = NNZ_MAX / BS;
+    int ci[U];
+    Val vv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int q = tid + u * BS;
+        if (q < nz) {
+            ci[u] = idx[p0 + q];
+            vv[u] = val[p0 + q];
+        }
+    }
+    Acc xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int q = tid + u * BS;
+        if (q < nz) xv[u] = static_cast<Acc>(gx(ci[u]));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int q = tid + u * BS;
+        if (q < nz) s_prod[q] = static_cast<Acc>(vv[u]) * xv[u];
+    }
+    __syncthreads();
+    Acc s = Acc(0);
+    for (int q = a + lane; q < b; q += tpr) s += s_prod[q];
+    const int wl = tpr < WAVE ? tpr : WAVE;
+    for (int o = wl / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (tpr <= WAVE) {
+        if (row < nrows && lane == 0) s_rows[row] = s;
+    } else {
+        if ((tid & (WAVE - 1)) == 0) red[tid / WAVE] = s;
+        __syncthreads();
+        if (lane == 0 && row < nrows) {
+            const int wpr = tpr / WAVE;
+            Acc t = red[row * wpr];
+            for (int k = 1; k < wpr; ++k) t += red[row * wpr + k];
+            s_rows[row] = t;
+        }
+    }
     __syncthreads();
 }
 
-// Row maxima of |val| * f(row, col): used by Ruiz equilibration.
 }  // namespace igaos::dev

@@ -230,8 +230,8 @@ void Engine<V>::setup() {
     const std::vector<int> hbT = dev::build_row_blocks(lp_.cp.data(), n_);
     rbA_.upload(hbA.data(), hbA.size(), s_);
     rbT_.upload(hbT.data(), hbT.size(), s_);
-    rA_ = {rbA_.p, static_cast<int>(hbA.size()) - 1};
-    rT_ = {rbT_.p, static_cast<int>(hbT.size()) - 1};
+    rA_ = {rbA_.p, static_cast<int>(hbA.size() / 2) - 1};
+    rT_ = {rbT_.p, static_cast<int>(hbT.size() / 2) - 1};
 
     dr_.alloc(m_);
     dc_.alloc(n_);

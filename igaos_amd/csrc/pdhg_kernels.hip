@@ -48,17 +48,12 @@ std::vector<int> build_row_blocks(const int* ptr, int nrows) {
         }
         rb.push_back(r);
     }
-    return rb;
-}
-
-__device__ __forceinline__ void block_rows(const RowBlocks& rb, int blk,
-                                           int& r0, int& r1) {
-    if (blk < rb.nblk) {
-        r0 = rb.rb[blk];
-        r1 = rb.rb[blk + 1];
-    } else {
-        r0 = r1 = 0;
+    std::vector<int> out(2 * rb.size());
+    for (size_t b = 0; b < rb.size(); ++b) {
+        out[2 * b] = rb[b];
+        out[2 * b + 1] = ptr[rb[b]];
     }
+    return out;
 }
 
 static inline int grid_of(int nblk) { return nblk > 0 ? nblk : 1; }
@@ -74,9 +69,9 @@ __global__ __launch_bounds__(BS) void k_spmv(RowBlocks rb,
     __shared__ Acc s_prod[NNZ_MAX];
     __shared__ Acc s_rows[ROWS_MAX];
     __shared__ Acc red[BS / WAVE];
-    int r0, r1;
-    block_rows(rb, blockIdx.x, r0, r1);
-    rowblock_sums<Acc>(ptr, idx, val, gx, r0, r1, s_rows, s_prod, red);
+    int r0, r1, p0, p1;
+    block_rows(rb, blockIdx.x, r0, r1, p0, p1);
+    rowblock_sums<Acc>(ptr, idx, val, gx, r0, r1, p0, p1, s_rows, s_prod, red);
     for (int t = threadIdx.x; t < r1 - r0; t += BS) y[r0 + t] = s_rows[t];
 }
 
@@ -118,10 +113,10 @@ __global__ __launch_bounds__(BS) void k_primal(IterBufs<V> b) {
     const V* __restrict__ x = b.x[cur];
     V* __restrict__ xn = b.x[cur ^ 1];
     const V* __restrict__ y = b.y[cur];
-    int r0, r1;
-    block_rows(b.rbT, blockIdx.x, r0, r1);
+    int r0, r1, p0, p1;
+    block_rows(b.rbT, blockIdx.x, r0, r1, p0, p1);
     if (fresh)
-        rowblock_sums<V>(b.cp, b.ci, b.acx, GatherPlain<V>{y}, r0, r1, s_rows,
+        rowblock_sums<V>(b.cp, b.ci, b.acx, GatherPlain<V>{y}, r0, r1, p0, p1, s_rows,
                          s_prod, red_v);
     double acc = 0.0;
     for (int t = threadIdx.x; t < r1 - r0; t += BS) {
@@ -163,9 +158,9 @@ __global__ __launch_bounds__(BS) void k_dual(IterBufs<V> b) {
     const bool mix = st.fresh != 0 && st.mix_pending != 0;
     const V* __restrict__ y = b.y[cur];
     V* __restrict__ yn = b.y[cur ^ 1];
-    int r0, r1;
-    block_rows(b.rbA, blockIdx.x, r0, r1);
-    rowblock_sums<V>(b.ap, b.ai, b.ax, GatherPlain<V>{b.xbar}, r0, r1, s_rows,
+    int r0, r1, p0, p1;
+    block_rows(b.rbA, blockIdx.x, r0, r1, p0, p1);
+    rowblock_sums<V>(b.ap, b.ai, b.ax, GatherPlain<V>{b.xbar}, r0, r1, p0, p1, s_rows,
                      s_prod, red_v);
     double a1 = 0.0, a2 = 0.0;
     for (int t = threadIdx.x; t < r1 - r0; t += BS) {
@@ -276,10 +271,10 @@ __global__ __launch_bounds__(BS) void k_kkt_cols(KktData d, KktIter<V> it) {
     __shared__ double s_rows[ROWS_MAX];
     __shared__ double red[BS / WAVE];
     const int q = blockIdx.y;
-    int r0, r1;
-    block_rows(d.rbT, blockIdx.x, r0, r1);
+    int r0, r1, p0, p1;
+    block_rows(d.rbT, blockIdx.x, r0, r1, p0, p1);
     rowblock_sums<double>(d.cp, d.ci, d.sacx, GatherRef<V>{it.yref[q], it.yd[q]},
-                          r0, r1, s_rows, s_prod, red);
+                          r0, r1, p0, p1, s_rows, s_prod, red);
     const GatherRef<V> gx{it.xref[q], it.xd[q]};
     const double dz = 1e-8 * (1.0 + d.cmax);
     double dv = 0.0, op = 0.0, od = 0.0, unb = 0.0;
@@ -325,10 +320,10 @@ __global__ __launch_bounds__(BS) void k_kkt_rows(KktData d, KktIter<V> it) {
     __shared__ double s_rows[ROWS_MAX];
     __shared__ double red[BS / WAVE];
     const int q = blockIdx.y;
-    int r0, r1;
-    block_rows(d.rbA, blockIdx.x, r0, r1);
+    int r0, r1, p0, p1;
+    block_rows(d.rbA, blockIdx.x, r0, r1, p0, p1);
     rowblock_sums<double>(d.ap, d.ai, d.sax, GatherRef<V>{it.xref[q], it.xd[q]},
-                          r0, r1, s_rows, s_prod, red);
+                          r0, r1, p0, p1, s_rows, s_prod, red);
     const GatherRef<V> gy{it.yref[q], it.yd[q]};
     double pv = 0.0, pa = 0.0, prow = 0.0, ymax = 0.0;
     for (int t = threadIdx.x; t < r1 - r0; t += BS) {

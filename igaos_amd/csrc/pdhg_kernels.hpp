@@ -7,17 +7,14 @@
 
 #include <vector>
 
+#include "dev_common.hpp"
 #include "pdhg_ctrl.hpp"
 
 namespace igaos::dev {
 
-// Row-block layout of one CSR matrix (block starts, nblk + 1 entries).
-struct RowBlocks {
-    const int* rb = nullptr;
-    int nblk = 0;
-};
-
-// Host: cut rows [0, nrows) of a CSR row pointer into row blocks.
+// Host: cut rows [0, nrows) of a CSR row pointer into row blocks; returns
+// the interleaved (first row, first nonzero) pairs of the RowBlocks layout
+// (dev_common.hpp), nblk + 1 pairs.
 std::vector<int> build_row_blocks(const int* ptr, int nrows);
 
 // y = A x  (plain SpMV).

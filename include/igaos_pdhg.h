@@ -139,8 +139,11 @@ int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
 
 /* ---- kernel layer (device pointers, HIP stream or NULL) ----------------- */
 /* Host: row-block layout of a CSR row pointer (host) for the SpMV
- * kernels.  Writes at most cap entries; returns the number of entries
- * (blocks + 1), or the required count if cap is too small. */
+ * kernels: pairs (first row, first nonzero) of each block followed by the
+ * pair (nrows, nnz), i.e. 2 * (blocks + 1) ints.  A block holds at most
+ * 2048 nonzeros and 256 rows, or a single longer row.  Writes at most cap
+ * ints; returns the number of ints, or the required count if cap is too
+ * small.  The device copy must be 8-byte aligned. */
 int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap);
 /* y = A x with A in CSR (device arrays); replaces the cusparseSpMV calls
  * of src/pdhg/pdhg.cu:150-156 for A and A' (CSR of A' = CSC of A). */

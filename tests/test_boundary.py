@@ -189,7 +189,9 @@ def test_rowblock_invariants(gen):
     else:
         ptrs = [np.zeros(5, np.int32), np.array([0, 0, 3000, 3000, 3001])]
     for ptr in ptrs:
-        rb = _rowblocks(ptr)
+        pairs = _rowblocks(ptr).reshape(-1, 2)
+        rb = pairs[:, 0]
+        assert np.array_equal(pairs[:, 1], np.asarray(ptr)[rb])
         nrows = len(ptr) - 1
         assert rb[0] == 0 and rb[-1] == nrows and np.all(np.diff(rb) > 0)
         for a, b in zip(rb[:-1], rb[1:]):

@@ -55,7 +55,7 @@ def _gpu_spmv(ptr, idx, val, x, f32=False):
     d_y = torch.full((len(ptr) - 1,), float("nan"), dtype=fdt, device="cuda")
     fn = _core.lib().igaos_spmv_csr_f32 if f32 else _core.lib().igaos_spmv_csr_f64
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    rc = fn(len(rb) - 1, _ptr(d_rb), _ptr(d_ptr), _ptr(d_idx), _ptr(d_val),
+    rc = fn(len(rb) // 2 - 1, _ptr(d_rb), _ptr(d_ptr), _ptr(d_idx), _ptr(d_val),
             _ptr(d_x), _ptr(d_y), stream)
     assert rc == 0, _core.last_error()
     torch.cuda.synchronize()
