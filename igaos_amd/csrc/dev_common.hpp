@@ -180,27 +180,27 @@ __device__ __forceinline__ void rowblock_sums(const int* __restrict__ ptr,
         a = ptr[r0 + row] - p0;
         b = ptr[r0 + row + 1] - p0;
     }
-    constexpr int U = NNZ_MAX / BS;
-    int ci[U];
-    Val vv[U];
+    if (nz > 0) {
+        // Branch-free loads: slots past the block's end re-read its last
+        // nonzero (same cache line), so all 8 requests issue back to back.
+        constexpr int U = NNZ_MAX / BS;
+        const int last = nz - 1;
+        int ci[U];
+        Val vv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int q = tid + u * BS;
-        if (q < nz) {
+        for (int u = 0; u < U; ++u) {
+            const int q = min(tid + u * BS, last);
             ci[u] = idx[p0 + q];
             vv[u] = val[p0 + q];
         }
-    }
-    Acc xv[U];
+        Acc xv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int q = tid + u * BS;
-        if (q < nz) xv[u] = static_cast<Acc>(gx(ci[u]));
-    }
+        for (int u = 0; u < U; ++u) xv[u] = static_cast<Acc>(gx(ci[u]));
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int q = tid + u * BS;
-        if (q < nz) s_prod[q] = static_cast<Acc>(vv[u]) * xv[u];
+        for (int u = 0; u < U; ++u) {
+            const int q = tid + u * BS;
+            if (q < nz) s_prod[q] = static_cast<Acc>(vv[u]) * xv[u];
+        }
     }
     __syncthreads();
     Acc s = Acc(0);

@@ -130,7 +130,9 @@ def test_fixed_iteration_trajectory_parity(K):
     assert s.iterations == r.iterations == K
     assert s.info["attempts"] == r.attempts
     assert s.info["restarts"] == r.restarts
-    assert s.status == r.status == "iteration-limit"
+    # a run that stops before the first check keeps the zero-initialised
+    # metrics and reports near-optimal (pdhg.cu:734,1096); both replicate it
+    assert s.status == r.status == ("near-optimal" if K < 64 else "iteration-limit")
     assert _rel(s.x, r.x) <= 1e-9 and _rel(s.y, r.y) <= 1e-9
     assert math.isclose(s.info["lam2"], r.lam2, rel_tol=1e-12)
     assert math.isclose(s.info["omega0"], r.omega0, rel_tol=1e-14)
