@@ -201,9 +201,11 @@ def test_iteration_cap_and_time_limit():
 
 def test_run_to_run_bit_identical():
     lp = lpgen.gen_zipf_badly_scaled(seed=1, m=2000, n=5000)
-    a = igaos_amd.solve_lp(lp, tolerance=1e-5, time_limit=60)
-    b = igaos_amd.solve_lp(lp, tolerance=1e-5, time_limit=60)
-    assert a.iterations == b.iterations
+    kw = dict(tolerance=0.0, time_limit=1e7, max_iterations=3000)
+    a = igaos_amd.solve_lp(lp, **kw)
+    b = igaos_amd.solve_lp(lp, **kw)
+    assert a.iterations == b.iterations == 3000
+    assert a.info["restarts"] == b.info["restarts"] > 0
     assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
 
 
