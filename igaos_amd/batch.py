@@ -1,0 +1,145 @@
+"""Batched persistent PDHG over the C-ABI (igaos_batch_* in
+include/igaos_pdhg.h): thousands of independent LPs of uneven sizes solved
+in one launch, each with its own line search, restarts and termination.
+No reference counterpart (the reference solves one LP per call); every
+instance follows the single-LP engine's algorithm, except that the dense
+host dual repair is not run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _core
+from ._batch_sigs import BatchResult
+from .api import STATUS_NAMES, _options
+
+
+@dataclass
+class BatchSolution:
+    status: list
+    objective: np.ndarray
+    pinf: np.ndarray
+    dinf: np.ndarray
+    rel_gap: np.ndarray
+    iterations: np.ndarray
+    attempts: np.ndarray
+    restarts: np.ndarray
+    solve_ms: np.ndarray
+    x: list = field(default_factory=list)
+    y: list = field(default_factory=list)
+    kernel_ms: float = 0.0
+
+    @property
+    def solved(self) -> int:
+        return sum(s == "near-optimal" for s in self.status)
+
+
+def _cat(arrs, dtype):
+    return np.ascontiguousarray(np.concatenate(arrs) if arrs else
+                                np.zeros(0), dtype=dtype)
+
+
+class BatchSolver:
+    """Uploads a list of ``lpgen.LP`` once; ``run`` solves all of them."""
+
+    def __init__(self, lps, precision=32):
+        L = _core.lib()
+        self.count = len(lps)
+        self.precision = precision
+        self.m = np.array([lp.m for lp in lps], np.int32)
+        self.n = np.array([lp.n for lp in lps], np.int32)
+        self.nnz = np.array([lp.nnz for lp in lps], np.int32)
+        d = lambda key: _cat([getattr(lp, key) for lp in lps], np.float64)  # noqa
+        i = lambda key: _cat([getattr(lp, key) for lp in lps], np.int32)  # noqa
+        self._keep = dict(c=d("c"), cl=d("cl"), cu=d("cu"), rmin=d("rmin"),
+                          rmax=d("rmax"), ap=i("ap"), ai=i("ai"), ax=d("ax"),
+                          cp=i("cp"), ci=i("ci"), acx=d("acx"),
+                          oc=np.array([lp.obj_const for lp in lps], np.float64))
+        k = self._keep
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa
+        h = L.igaos_batch_create(self.count, ip(self.m), ip(self.n),
+                                 ip(self.nnz), dp(k["c"]), dp(k["cl"]),
+                                 dp(k["cu"]), dp(k["rmin"]), dp(k["rmax"]),
+                                 ip(k["ap"]), ip(k["ai"]), dp(k["ax"]),
+                                 ip(k["cp"]), ip(k["ci"]), dp(k["acx"]),
+                                 dp(k["oc"]), int(precision))
+        if not h:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+        self._h = C.c_void_p(h)
+        self._roff = np.concatenate([[0], np.cumsum(self.m)])
+        self._coff = np.concatenate([[0], np.cumsum(self.n)])
+
+    def schedule(self):
+        tpr = np.zeros(self.count, np.int32)
+        g = C.c_int()
+        _core.lib().igaos_batch_get_schedule(
+            self._h, tpr.ctypes.data_as(C.POINTER(C.c_int)), C.byref(g))
+        return tpr, g.value
+
+    def set_schedule(self, tpr=None, grid=0):
+        L = _core.lib()
+        t = None if tpr is None else np.ascontiguousarray(tpr, np.int32)
+        rc = L.igaos_batch_set_schedule(
+            self._h, None if t is None else t.ctypes.data_as(C.POINTER(C.c_int)),
+            int(grid))
+        if rc != 0:
+            raise ValueError(L.igaos_batch_last_error().decode())
+
+    def run(self, tolerance=1e-6, time_limit=120.0, max_iterations=500000,
+            want_xy=True, progress=None, poll_s=2.0) -> BatchSolution:
+        """Solve every instance.  ``progress(done, count)`` is called every
+        ``poll_s`` seconds while the persistent kernel runs."""
+        import time
+        L = _core.lib()
+        opt = _options(time_limit, tolerance, max_iterations, True, 0)
+        res = (BatchResult * self.count)()
+        x = np.zeros(int(self._coff[-1])) if want_xy else None
+        y = np.zeros(int(self._roff[-1])) if want_xy else None
+        if L.igaos_batch_launch(self._h, C.byref(opt)) != 0:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+        if progress is not None:
+            while True:
+                done = L.igaos_batch_progress(self._h)
+                progress(done, self.count)
+                if done >= self.count:
+                    break
+                time.sleep(poll_s)
+        rc = L.igaos_batch_wait(
+            self._h,
+            x.ctypes.data_as(C.POINTER(C.c_double)) if want_xy else None,
+            y.ctypes.data_as(C.POINTER(C.c_double)) if want_xy else None, res)
+        if rc != 0:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+        f = lambda k: np.array([getattr(r, k) for r in res])  # noqa
+        sol = BatchSolution(
+            status=[STATUS_NAMES[r.status] for r in res],
+            objective=f("objective"), pinf=f("pinf"), dinf=f("dinf"),
+            rel_gap=f("rel_gap"), iterations=f("iterations"),
+            attempts=f("attempts"), restarts=f("restarts"),
+            solve_ms=f("solve_ms"),
+            kernel_ms=L.igaos_batch_kernel_ms(self._h))
+        if want_xy:
+            sol.x = [x[self._coff[i]:self._coff[i + 1]] for i in range(self.count)]
+            sol.y = [y[self._roff[i]:self._roff[i + 1]] for i in range(self.count)]
+        return sol
+
+    def close(self):
+        if getattr(self, "_h", None) and _core._lib is not None:
+            _core._lib.igaos_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def solve_batch(lps, precision=32, **kw) -> BatchSolution:
+    """One-shot batched solve of a list of LPs."""
+    b = BatchSolver(lps, precision=precision)
+    try:
+        return b.run(**kw)
+    finally:
+        b.close()

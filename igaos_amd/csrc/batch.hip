@@ -1,0 +1,718 @@
+// Persistent batched PDHG kernel (see batch.hpp).
+//
+// One 256-thread workgroup owns one LP at a time.  Per attempt it runs the
+// two SpMV phases with 2^k threads per row (k from the LP's schedule; rows
+// strided over the workgroup, lanes of a row reduced with xor shuffles),
+// fused with the primal / dual updates exactly as the single-LP kernels do,
+// then reduces |dx|^2, |dy|^2 and dy'Adx with the workgroup tree and
+// applies the line-search rule (pdhg_ctrl.hpp) on thread 0.  Every 64
+// accepted iterations it evaluates both iterates' KKT metrics in FP64 in
+// original units and runs the check logic of src/pdhg/pdhg.cu:801-1084
+// (termination, incumbent, rollback, restart, primal weight, time limit)
+// with the instance's own state.  In FP32 mode a restart folds the anchor's
+// correction into the FP64 reference point and rebuilds the correction
+// problem's data with FP64 SpMVs (iterative refinement).
+//
+// Every phase uses a fixed thread-to-data mapping, so an LP's result does
+// not depend on which workgroup solves it or on the rest of the batch.
+
+#include "batch.hpp"
+#include "dev_common.hpp"
+
+namespace igaos::batch {
+
+using dev::WAVE;
+using pdhg::LsState;
+using pdhg::Metrics;
+
+__device__ __forceinline__ double bsum(double v, double* red) {
+    return dev::block_sum<double, NT>(v, red);
+}
+__device__ __forceinline__ double bmax(double v, double* red) {
+    return dev::block_max<double, NT>(v, red);
+}
+
+// Rows [0, nrows) of a CSR block: tpr lanes per row, f(row, sum) on the
+// row's first lane.  The outer loop trip count is uniform, so every lane
+// reaches the shuffles.
+template <typename Acc, typename Val, typename G, typename F>
+__device__ __forceinline__ void rows_apply(int nrows, const int* __restrict__ ptr,
+                                           const int* __restrict__ idx,
+                                           const Val* __restrict__ val,
+                                           const G& gx, int tpr, F&& f) {
+    const int groups = NT / tpr;
+    const int g = threadIdx.x / tpr;
+    const int lane = threadIdx.x & (tpr - 1);
+    for (int base = 0; base < nrows; base += groups) {
+        const int r = base + g;
+        Acc s = Acc(0);
+        if (r < nrows) {
+            const int p1 = ptr[r + 1];
+            for (int p = ptr[r] + lane; p < p1; p += tpr)
+                s += static_cast<Acc>(val[p]) * static_cast<Acc>(gx(idx[p]));
+        }
+        for (int o = tpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0 && r < nrows) f(r, s);
+    }
+}
+
+// Per-instance view: array pointers are formed on demand from the batch's
+// pointer table (device memory, read through scalar loads) and the
+// instance's offsets, which keeps register pressure low.
+template <typename V>
+struct LP {
+    const Arrays<V>* A;
+    int m, n, tpr;
+    long ro, co, zo, rpo, cpo;
+    static constexpr bool F32 = sizeof(V) == 4;
+    __device__ const int* ap() const { return A->ap + rpo; }
+    __device__ const int* ai() const { return A->ai + zo; }
+    __device__ const int* cp() const { return A->cp + cpo; }
+    __device__ const int* ci() const { return A->ci + zo; }
+    __device__ const double* ax() const { return A->ax + zo; }
+    __device__ const double* acx() const { return A->acx + zo; }
+    __device__ const double* c() const { return A->c + co; }
+    __device__ const double* cl() const { return A->cl + co; }
+    __device__ const double* cu() const { return A->cu + co; }
+    __device__ const double* rmin() const { return A->rmin + ro; }
+    __device__ const double* rmax() const { return A->rmax + ro; }
+    __device__ double* dr() const { return A->dr + ro; }
+    __device__ double* dc() const { return A->dc + co; }
+    __device__ double* sax() const { return A->sax + zo; }
+    __device__ double* sacx() const { return A->sacx + zo; }
+    __device__ double* sc() const { return A->sc + co; }
+    __device__ double* scl() const { return A->scl + co; }
+    __device__ double* scu() const { return A->scu + co; }
+    __device__ double* srmin() const { return A->srmin + ro; }
+    __device__ double* srmax() const { return A->srmax + ro; }
+    __device__ V* fax() const { return A->fax + zo; }
+    __device__ V* facx() const { return A->facx + zo; }
+    __device__ V* x(int k) const { return (k ? A->x1 : A->x0) + co; }
+    __device__ V* y(int k) const { return (k ? A->y1 : A->y0) + ro; }
+    __device__ V* xbar() const { return A->xbar + co; }
+    __device__ V* ata() const { return A->ata + co; }
+    __device__ V* xavg() const { return A->xavg + co; }
+    __device__ V* yavg() const { return A->yavg + ro; }
+    __device__ V* axcur() const { return A->axcur + ro; }
+    __device__ V* adx() const { return A->adx + ro; }
+    __device__ double* xref() const { return F32 ? A->xref + co : nullptr; }
+    __device__ double* yref() const { return F32 ? A->yref + ro : nullptr; }
+    __device__ double* aref() const { return F32 ? A->aref + ro : nullptr; }
+    __device__ double* atref() const { return F32 ? A->atref + co : nullptr; }
+    __device__ V* lo() const { return A->lo + co; }
+    __device__ V* hi() const { return A->hi + co; }
+    __device__ V* gref() const { return A->gref + co; }
+    __device__ V* rlo() const { return A->rlo + ro; }
+    __device__ V* rhi() const { return A->rhi + ro; }
+    __device__ V* yoff() const { return F32 ? A->yoff + ro : nullptr; }
+    __device__ double* xb() const { return A->xb + co; }
+    __device__ double* yb() const { return A->yb + ro; }
+    __device__ double* pesx() const { return A->pesx + co; }
+    __device__ double* pesy() const { return A->pesy + ro; }
+    __device__ double* xout() const { return A->xout + co; }
+    __device__ double* yout() const { return A->yout + ro; }
+};
+
+template <typename V>
+__device__ __forceinline__ LP<V> view(const Arrays<V>* A, const LPDesc& d) {
+    LP<V> L;
+    L.A = A;
+    L.m = d.m;
+    L.n = d.n;
+    L.tpr = d.tpr;
+    L.ro = d.roff;
+    L.co = d.coff;
+    L.zo = d.zoff;
+    L.rpo = d.rpo;
+    L.cpo = d.cpo;
+    return L;
+}
+
+// Shared per-workgroup control block (thread 0 writes, all read).
+struct Ctl {
+    LsState st;
+    pdhg::RestartState rs;
+    Metrics mc, ma, last, inc;
+    double eta0, omega0, inc_op, cmax, lam2;
+    long long last_rollback, restarts;
+    int lp, have_inc, have_pe, last_is_avg, stop, time_limit;
+    int inc_src, rollback, restart, anchor_avg, accepted;
+    unsigned long long t0;
+};
+
+// x = xref + xd in FP64 (xref may be null)
+template <typename V>
+__device__ __forceinline__ double full(const double* ref, const V* d, int j) {
+    return ref ? ref[j] + static_cast<double>(d[j]) : static_cast<double>(d[j]);
+}
+
+template <typename V>
+__device__ void refresh_refs(const LP<V>& L, double* red) {
+    // FP32: aref = A~ xref, atref = A~' yref (FP64), then the shifted data
+    if (!L.xref()) return;
+    rows_apply<double>(L.m, L.ap(), L.ai(), L.sax(), dev::GatherPlain<double>{L.xref()}, L.tpr,
+                       [&](int i, double s) { L.aref()[i] = s; });
+    rows_apply<double>(L.n, L.cp(), L.ci(), L.sacx(), dev::GatherPlain<double>{L.yref()}, L.tpr,
+                       [&](int j, double s) { L.atref()[j] = s; });
+    __syncthreads();
+    for (int j = threadIdx.x; j < L.n; j += NT) {
+        L.lo()[j] = static_cast<V>(L.scl()[j] - L.xref()[j]);
+        L.hi()[j] = static_cast<V>(L.scu()[j] - L.xref()[j]);
+        L.gref()[j] = static_cast<V>(L.sc()[j] + L.atref()[j]);
+    }
+    for (int i = threadIdx.x; i < L.m; i += NT) {
+        L.rlo()[i] = static_cast<V>(L.srmin()[i] - L.aref()[i]);
+        L.rhi()[i] = static_cast<V>(L.srmax()[i] - L.aref()[i]);
+        L.yoff()[i] = static_cast<V>(L.yref()[i]);
+    }
+    __syncthreads();
+}
+
+// KKT metrics in original units of the iterate (xref + xd, yref + yd)
+// (src/pdhg/pdhg.cu:173-254 through metrics_of, 694-733).
+template <typename V>
+__device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
+                           double cmax, double* red) {
+    const double* xr = L.xref();
+    const double* yr = L.yref();
+    const double dz = 1e-8 * (1.0 + cmax);
+    double dv = 0.0, op = 0.0, odc = 0.0, unbc = 0.0;
+    double pv = 0.0, pa = 0.0, prow = 0.0, ymax = 0.0;
+    const int tpr = L.tpr;
+    const int groups = NT / tpr;
+    const int g = threadIdx.x / tpr;
+    const int lane = threadIdx.x & (tpr - 1);
+    {  // column pass: d = c + A'y / dc, dinf, c'x and the column bound
+        const int* cp = L.cp();
+        const int* ci = L.ci();
+        const double* sacx = L.sacx();
+        const double* dc = L.dc();
+        const double* c = L.c();
+        const double* clp = L.cl();
+        const double* cup = L.cu();
+        const dev::GatherRef<V> gy{yr, yd};
+        for (int base = 0; base < L.n; base += groups) {
+            const int j = base + g;
+            double ata = 0.0;
+            if (j < L.n)
+                for (int p = cp[j] + lane; p < cp[j + 1]; p += tpr) ata += sacx[p] * gy(ci[p]);
+            for (int o = tpr / 2; o > 0; o >>= 1) ata += __shfl_xor(ata, o);
+            if (lane == 0 && j < L.n) {
+                const double hx = full(xr, xd, j) * dc[j];
+                const double hd = c[j] + ata / dc[j];
+                const double cl = clp[j], cu = cup[j];
+                const double bnd = fabs(__builtin_isfinite(cu) ? cu : cl);
+                const double tol = 1e-9 * (1.0 + bnd);
+                double v;
+                if (hx <= cl + tol && hd > 0) v = 0.0;
+                else if (hx >= cu - tol && hd < 0) v = 0.0;
+                else v = fabs(hd);
+                dv = fmax(dv, v);
+                op += c[j] * hx;
+                if (hd >= 0) {
+                    if (__builtin_isfinite(cl)) odc += hd * cl;
+                    else if (hd > dz) unbc = 1.0;
+                } else {
+                    if (__builtin_isfinite(cu)) odc += hd * cu;
+                    else if (-hd > dz) unbc = 1.0;
+                }
+            }
+        }
+    }
+    {  // row pass: Ax = dr (A~x~), pinf / prow numerators, max|y|
+        const int* ap = L.ap();
+        const int* ai = L.ai();
+        const double* sax = L.sax();
+        const double* dr = L.dr();
+        const double* rminp = L.rmin();
+        const double* rmaxp = L.rmax();
+        const dev::GatherRef<V> gx{xr, xd};
+        for (int base = 0; base < L.m; base += groups) {
+            const int i = base + g;
+            double tm = 0.0;
+            if (i < L.m)
+                for (int p = ap[i] + lane; p < ap[i + 1]; p += tpr) tm += sax[p] * gx(ai[p]);
+            for (int o = tpr / 2; o > 0; o >>= 1) tm += __shfl_xor(tm, o);
+            if (lane == 0 && i < L.m) {
+                const double ax = tm * dr[i];
+                const double lo = rminp[i], hi = rmaxp[i];
+                const double e = fmax(lo - ax, ax - hi);
+                pv = fmax(pv, e);
+                pa = fmax(pa, fabs(fmin(fmax(ax, lo), hi)));
+                double sc = fabs(ax);
+                if (__builtin_isfinite(lo)) sc = fmax(sc, fabs(lo));
+                if (__builtin_isfinite(hi)) sc = fmax(sc, fabs(hi));
+                prow = fmax(prow, fmax(0.0, e) / (1.0 + sc));
+                ymax = fmax(ymax, fabs(full(yr, yd, i) * (1.0 / dr[i])));
+            }
+        }
+    }
+    dv = bmax(dv, red);
+    op = bsum(op, red);
+    odc = bsum(odc, red);
+    unbc = bmax(unbc, red);
+    pv = bmax(pv, red);
+    pa = bmax(pa, red);
+    prow = bmax(prow, red);
+    ymax = bmax(ymax, red);
+    const double yz = 1e-8 * (1.0 + ymax);
+    double odr = 0.0, unbr = 0.0;
+    for (int i = threadIdx.x; i < L.m; i += NT) {
+        const double hy = full(yr, yd, i) * (1.0 / L.dr()[i]);
+        if (hy > yz) {
+            if (__builtin_isfinite(L.rmax()[i])) odr -= hy * L.rmax()[i];
+            else unbr = 1.0;
+        } else if (hy < -yz) {
+            if (__builtin_isfinite(L.rmin()[i])) odr -= hy * L.rmin()[i];
+            else unbr = 1.0;
+        }
+    }
+    odr = bsum(odr, red);
+    unbr = bmax(unbr, red);
+    Metrics mt;
+    mt.pinf = fmax(pv, 0.0) / (1.0 + pa);
+    mt.prow = prow;
+    mt.dinf = dv / (1.0 + cmax);
+    mt.op = op;
+    const bool bounded = unbc == 0.0 && unbr == 0.0;
+    mt.od = bounded ? odc + odr : -__builtin_huge_val();
+    mt.gap = bounded ? fmax(0.0, op - (odc + odr)) / (1.0 + fabs(op))
+                     : __builtin_huge_val();
+    return mt;
+}
+
+template <typename V>
+__device__ void setup(const LP<V>& L, Ctl& C, double* red) {
+    const int m = L.m, n = L.n, tid = threadIdx.x;
+    // Ruiz, 10 sweeps (pdhg.cu:469-492); one thread per row / column, so
+    // each factor is exactly the host loop's value.
+    for (int i = tid; i < m; i += NT) L.dr()[i] = 1.0;
+    for (int j = tid; j < n; j += NT) L.dc()[j] = 1.0;
+    __syncthreads();
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        for (int i = tid; i < m; i += NT) {
+            double mx = 0.0;
+            const double d = L.dr()[i];
+            for (int p = L.ap()[i]; p < L.ap()[i + 1]; ++p)
+                mx = fmax(mx, fabs(L.ax()[p]) * L.dc()[L.ai()[p]] / d);
+            if (mx > 0 && __builtin_isfinite(mx)) L.dr()[i] = d * sqrt(mx);
+        }
+        __syncthreads();
+        for (int j = tid; j < n; j += NT) {
+            double mx = 0.0;
+            const double d = L.dc()[j];
+            for (int p = L.cp()[j]; p < L.cp()[j + 1]; ++p)
+                mx = fmax(mx, fabs(L.acx()[p]) / L.dr()[L.ci()[p]] * d);
+            if (mx > 0 && __builtin_isfinite(mx)) L.dc()[j] = d / sqrt(mx);
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < m; i += NT) {
+        double d = L.dr()[i];
+        if (!(d > 0 && __builtin_isfinite(d))) L.dr()[i] = d = 1.0;
+        L.srmin()[i] = L.rmin()[i] / d;
+        L.srmax()[i] = L.rmax()[i] / d;
+    }
+    for (int j = tid; j < n; j += NT) {
+        double d = L.dc()[j];
+        if (!(d > 0 && __builtin_isfinite(d))) L.dc()[j] = d = 1.0;
+        L.sc()[j] = L.c()[j] * d / 1.0;
+        L.scl()[j] = L.cl()[j] / d;
+        L.scu()[j] = L.cu()[j] / d;
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += NT) {
+        const double d = L.dr()[i];
+        for (int p = L.ap()[i]; p < L.ap()[i + 1]; ++p) {
+            const double v = L.ax()[p] * L.dc()[L.ai()[p]] / d;
+            L.sax()[p] = v;
+            L.fax()[p] = static_cast<V>(v);
+        }
+    }
+    for (int j = tid; j < n; j += NT) {
+        const double d = L.dc()[j];
+        for (int p = L.cp()[j]; p < L.cp()[j + 1]; ++p) {
+            const double v = L.acx()[p] * d / L.dr()[L.ci()[p]];
+            L.sacx()[p] = v;
+            L.facx()[p] = static_cast<V>(v);
+        }
+    }
+    __syncthreads();
+    // power iteration (pdhg.cu:605-631); xb, pesx hold v, w and yb A v
+    double* v = L.xb();
+    double* w = L.pesx();
+    double* t = L.yb();
+    const double v0 = 1.0 / sqrt(static_cast<double>(n));
+    for (int j = tid; j < n; j += NT) v[j] = v0;
+    __syncthreads();
+    for (int it = 0; it < 30; ++it) {
+        rows_apply<double>(m, L.ap(), L.ai(), L.sax(), dev::GatherPlain<double>{v}, L.tpr,
+                           [&](int i, double s) { t[i] = s; });
+        __syncthreads();
+        double q = 0.0;
+        rows_apply<double>(n, L.cp(), L.ci(), L.sacx(), dev::GatherPlain<double>{t}, L.tpr,
+                           [&](int j, double s) { w[j] = s; q += s * s; });
+        const double nw = sqrt(bsum(q, red));
+        if (nw < 1e-300) break;
+        const double inv = 1.0 / nw;
+        for (int j = tid; j < n; j += NT) v[j] = w[j] * inv;
+        __syncthreads();
+    }
+    double q = 0.0;
+    rows_apply<double>(m, L.ap(), L.ai(), L.sax(), dev::GatherPlain<double>{v}, L.tpr,
+                       [&](int i, double s) { q += s * s; });
+    double lam2 = bsum(q, red);
+    if (!(lam2 > 0) || !__builtin_isfinite(lam2)) lam2 = 1.0;
+    // omega0 (pdhg.cu:647-663), max|c|
+    double cn = 0.0, qn = 0.0, cm = 0.0;
+    for (int j = tid; j < n; j += NT) {
+        cn += L.sc()[j] * L.sc()[j];
+        cm = fmax(cm, fabs(L.c()[j]));
+    }
+    for (int i = tid; i < m; i += NT) {
+        const double qq = __builtin_isfinite(L.srmax()[i]) ? L.srmax()[i]
+                          : (__builtin_isfinite(L.srmin()[i]) ? L.srmin()[i] : 0.0);
+        qn += qq * qq;
+    }
+    cn = bsum(cn, red);
+    qn = bsum(qn, red);
+    cm = bmax(cm, red);
+    if (tid == 0) {
+        C.lam2 = lam2;
+        C.eta0 = 1.2 / sqrt(lam2);
+        double om = 1.0;
+        if (cn > 1e-24 && qn > 1e-24) om = fmin(fmax(sqrt(cn / qn), 0.3), 3.0);
+        C.omega0 = om;
+        C.cmax = cm;
+    }
+    // fresh iterates: x = y = 0
+    for (int j = tid; j < n; j += NT) {
+        L.x(0)[j] = L.x(1)[j] = L.xbar()[j] = L.ata()[j] = L.xavg()[j] = V(0);
+        if (L.xref()) L.xref()[j] = 0.0;
+    }
+    for (int i = tid; i < m; i += NT) {
+        L.y(0)[i] = L.y(1)[i] = L.yavg()[i] = L.axcur()[i] = L.adx()[i] = V(0);
+        if (L.yref()) L.yref()[i] = 0.0;
+    }
+    __syncthreads();
+    refresh_refs(L, red);
+}
+
+template <typename V>
+__device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
+                          Ctl& C, double* red, LPResult& R) {
+    const int m = L.m, n = L.n, tid = threadIdx.x;
+    constexpr bool F32 = sizeof(V) == 4;
+    const double INF = __builtin_huge_val();
+    const double TOL = P.tol;
+    setup(L, C, red);
+    if (tid == 0) {
+        LsState& s = C.st;
+        s = LsState{};
+        s.eta = C.eta0;
+        s.omega = C.omega0;
+        s.k = 1;
+        s.maxit = P.maxit;
+        s.no_freeze = 1;
+        C.rs = pdhg::RestartState{-1.0, INF, 0};
+        C.last = Metrics{0, 0, 0, 0, 0, 0};
+        C.inc = Metrics{0, 0, 0, 0, 0, 0};
+        C.last_is_avg = 0;
+        C.have_inc = 0;
+        C.have_pe = 0;
+        C.inc_op = 0.0;
+        C.last_rollback = 0;
+        C.restarts = 0;
+        C.stop = P.maxit < 1;
+        C.time_limit = 0;
+        C.t0 = wall_clock64();
+    }
+    __syncthreads();
+    long long kdone = 0;
+    while (!C.stop) {
+        // ---- one accepted iteration (pdhg.cu:748-799)
+        bool fresh = true;
+        for (;;) {
+            const int cur = C.st.cur;
+            const V tau = static_cast<V>(C.st.eta / C.st.omega);
+            const V sig = static_cast<V>(C.st.eta * C.st.omega);
+            const V* x = L.x(cur);
+            V* xn = L.x(cur ^ 1);
+            const V* y = L.y(cur);
+            V* yn = L.y(cur ^ 1);
+            double pacc = 0.0;
+            auto primal = [&](int j, V a) {
+                const V xo = x[j];
+                const V g = L.gref()[j] + a;
+                const V v = fmin(fmax(xo - tau * g, L.lo()[j]), L.hi()[j]);
+                xn[j] = v;
+                L.xbar()[j] = V(2) * v - xo;
+                const double d = static_cast<double>(v - xo);
+                pacc += d * d;
+            };
+            if (fresh) {
+                rows_apply<V>(n, L.cp(), L.ci(), L.facx(), dev::GatherPlain<V>{y}, L.tpr,
+                              [&](int j, V a) { L.ata()[j] = a; primal(j, a); });
+            } else {
+                for (int j = tid; j < n; j += NT) primal(j, L.ata()[j]);
+            }
+            __syncthreads();
+            double a1 = 0.0, a2 = 0.0;
+            rows_apply<V>(m, L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.xbar()}, L.tpr,
+                          [&](int i, V tm) {
+                const V yo = y[i];
+                const V tt = yo + sig * tm;
+                V u = L.yoff() ? tt + L.yoff()[i] : tt;
+                u = u / sig;
+                const V yv = tt - sig * fmin(fmax(u, L.rlo()[i]), L.rhi()[i]);
+                yn[i] = yv;
+                const V dy = yv - yo;
+                const V ad = V(0.5) * (tm - L.axcur()[i]);
+                L.adx()[i] = ad;
+                a1 += static_cast<double>(dy) * static_cast<double>(dy);
+                a2 += static_cast<double>(dy) * static_cast<double>(ad);
+            });
+            pacc = bsum(pacc, red);
+            a1 = bsum(a1, red);
+            a2 = bsum(a2, red);
+            if (tid == 0) {
+                pdhg::ls_decide(C.st, pacc, a1, a2);
+                C.accepted = C.st.mix_pending;
+            }
+            __syncthreads();
+            if (C.accepted) break;
+            fresh = false;
+        }
+        // averaging + A x commit of the accepted step (pdhg.cu:795-799)
+        {
+            const int cur = C.st.cur;
+            const V w = static_cast<V>(C.st.mix_w);
+            for (int j = tid; j < n; j += NT) L.xavg()[j] += w * (L.x(cur)[j] - L.xavg()[j]);
+            for (int i = tid; i < m; i += NT) {
+                L.yavg()[i] += w * (L.y(cur)[i] - L.yavg()[i]);
+                L.axcur()[i] += L.adx()[i];
+            }
+            kdone = C.st.k - 1;
+        }
+        __syncthreads();
+        if (kdone % pdhg::CHECK_EVERY != 0) {
+            if (kdone >= P.maxit) break;
+            continue;
+        }
+        // ---- check (pdhg.cu:801-1084)
+        const int cur = C.st.cur;
+        const Metrics mc = metrics(L, L.x(cur), L.y(cur), C.cmax, red);
+        const Metrics ma = metrics(L, L.xavg(), L.yavg(), C.cmax, red);
+        if (tid == 0) {
+            const double ec = mc.finite_res() ? mc.err() : INF;
+            const double ea = ma.finite_res() ? ma.err() : INF;
+            C.last = (ec <= ea) ? mc : ma;
+            C.last_is_avg = (ea < ec);
+            C.inc_src = -1;
+            C.rollback = 0;
+            C.restart = 0;
+            if (pdhg::passes(mc, TOL)) {
+                C.last = mc;
+                C.last_is_avg = 0;
+                C.stop = 1;
+            } else if (pdhg::passes(ma, TOL)) {
+                C.last = ma;
+                C.last_is_avg = 1;
+                C.stop = 1;
+            } else {
+                for (int cc = 0; cc < 2; ++cc) {
+                    const Metrics& t = cc ? ma : mc;
+                    if (t.pinf <= 1e-4 && __builtin_isfinite(t.op) &&
+                        (!C.have_inc || t.op < C.inc_op)) {
+                        C.have_inc = 1;
+                        C.inc_op = t.op;
+                        C.inc = t;
+                        C.inc_src = cc;
+                    }
+                }
+                if (mc.pinf >= 0.5 && C.have_inc && kdone - C.last_rollback > 2000) {
+                    C.rollback = 1;
+                    C.st.eta = fmin(C.st.eta, 0.5 * C.eta0);
+                    C.st.hn = 0;
+                    C.rs.epoch_start = kdone;
+                    C.rs.epoch_kkt = -1.0;
+                    C.rs.prev_cand = INF;
+                    C.last_rollback = kdone;
+                }
+                const double cand = fmin(ec, ea);
+                if (pdhg::restart_decide(C.rs, cand, kdone)) {
+                    C.restart = 1;
+                    C.anchor_avg = ea < ec;
+                    C.restarts += 1;
+                    C.st.hn = 0;
+                    C.rs.epoch_start = kdone;
+                    C.rs.epoch_kkt = cand;
+                    C.rs.prev_cand = INF;
+                }
+                const double el = static_cast<double>(wall_clock64() - C.t0) / P.ticks_per_s;
+                if (el > P.time_limit_s) C.time_limit = 1;
+            }
+        }
+        __syncthreads();
+        if (C.stop) break;  // a pair passed the termination test
+        if (C.inc_src >= 0) {
+            const V* sx = C.inc_src ? L.xavg() : L.x(cur);
+            const V* sy = C.inc_src ? L.yavg() : L.y(cur);
+            for (int j = tid; j < n; j += NT) L.xb()[j] = full(L.xref(), sx, j);
+            for (int i = tid; i < m; i += NT) L.yb()[i] = full(L.yref(), sy, i);
+        }
+        __syncthreads();
+        if (C.rollback) {
+            if constexpr (F32) {
+                for (int j = tid; j < n; j += NT) {
+                    L.xref()[j] = L.xb()[j];
+                    L.x(0)[j] = L.x(1)[j] = L.xavg()[j] = V(0);
+                }
+                for (int i = tid; i < m; i += NT) {
+                    L.yref()[i] = L.yb()[i];
+                    L.y(0)[i] = L.y(1)[i] = L.yavg()[i] = L.axcur()[i] = V(0);
+                }
+                __syncthreads();
+                refresh_refs(L, red);
+            } else {
+                for (int j = tid; j < n; j += NT) L.x(cur)[j] = L.xavg()[j] = static_cast<V>(L.xb()[j]);
+                for (int i = tid; i < m; i += NT) L.y(cur)[i] = L.yavg()[i] = static_cast<V>(L.yb()[i]);
+                __syncthreads();
+                rows_apply<V>(m, L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.x(cur)}, L.tpr,
+                              [&](int i, V s) { L.axcur()[i] = s; });
+            }
+            __syncthreads();
+        }
+        if (C.restart) {
+            if constexpr (F32) {
+                const V* ax = C.anchor_avg ? L.xavg() : L.x(cur);
+                const V* ay = C.anchor_avg ? L.yavg() : L.y(cur);
+                for (int j = tid; j < n; j += NT) L.xref()[j] += static_cast<double>(ax[j]);
+                for (int i = tid; i < m; i += NT) L.yref()[i] += static_cast<double>(ay[i]);
+                __syncthreads();
+                for (int j = tid; j < n; j += NT) L.x(0)[j] = L.x(1)[j] = L.xavg()[j] = V(0);
+                for (int i = tid; i < m; i += NT) L.y(0)[i] = L.y(1)[i] = L.yavg()[i] = L.axcur()[i] = V(0);
+                __syncthreads();
+                refresh_refs(L, red);
+            } else {
+                if (C.anchor_avg) {
+                    for (int j = tid; j < n; j += NT) L.x(cur)[j] = L.xavg()[j];
+                    for (int i = tid; i < m; i += NT) L.y(cur)[i] = L.yavg()[i];
+                } else {
+                    for (int j = tid; j < n; j += NT) L.xavg()[j] = L.x(cur)[j];
+                    for (int i = tid; i < m; i += NT) L.yavg()[i] = L.y(cur)[i];
+                }
+                __syncthreads();
+                rows_apply<V>(m, L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.x(cur)}, L.tpr,
+                              [&](int i, V s) { L.axcur()[i] = s; });
+                __syncthreads();
+            }
+            // primal weight on the epoch-start points (scaled units)
+            double dxn = 0.0, dyn = 0.0;
+            for (int j = tid; j < n; j += NT) {
+                const double v = F32 ? L.xref()[j] : static_cast<double>(L.x(cur)[j]);
+                const double d = v - L.pesx()[j];
+                dxn += d * d;
+                L.pesx()[j] = v;
+            }
+            for (int i = tid; i < m; i += NT) {
+                const double v = F32 ? L.yref()[i] : static_cast<double>(L.y(cur)[i]);
+                const double d = v - L.pesy()[i];
+                dyn += d * d;
+                L.pesy()[i] = v;
+            }
+            dxn = bsum(dxn, red);
+            dyn = bsum(dyn, red);
+            if (tid == 0) {
+                if (C.have_pe) C.st.omega = pdhg::primal_weight(C.st.omega, dxn, dyn);
+                C.have_pe = 1;
+            }
+            __syncthreads();
+        }
+        if (C.time_limit) break;  // checked last, as in pdhg.cu:1077-1083
+        if (kdone >= P.maxit) break;
+    }
+    // ---- exit (pdhg.cu:1090-1141)
+    __syncthreads();
+    int src;
+    if (tid == 0) {
+        Metrics last = C.last;
+        int st;
+        if (C.time_limit) st = 6;
+        else st = pdhg::certified(last, TOL) ? 1 : 5;
+        src = C.last_is_avg ? 1 : 0;
+        if (!pdhg::certified(last, TOL) && C.have_inc && C.inc.pinf <= TOL &&
+            __builtin_isfinite(C.inc.op) && (C.inc.op < last.op || last.pinf > TOL)) {
+            last = C.inc;
+            src = 2;
+        }
+        R.status = st;
+        R.last_is_avg = src == 1;
+        R.objective = last.op + D.obj_const;
+        R.pinf = last.pinf;
+        R.dinf = last.dinf;
+        R.gap = last.gap;
+        R.prow = last.prow;
+        R.iterations = kdone;
+        R.attempts = C.st.attempts;
+        R.restarts = C.restarts;
+        R.solve_ms = static_cast<double>(wall_clock64() - C.t0) / P.ticks_per_s * 1e3;
+        R.eta = C.st.eta;
+        R.omega = C.st.omega;
+        R.lam2 = C.lam2;
+        C.inc_src = src;
+    }
+    __syncthreads();
+    src = C.inc_src;
+    const int cur = C.st.cur;
+    for (int j = tid; j < n; j += NT) {
+        const double sx = src == 2 ? L.xb()[j]
+                          : full(L.xref(), src == 1 ? L.xavg() : L.x(cur), j);
+        L.xout()[j] = sx * L.dc()[j];
+    }
+    for (int i = tid; i < m; i += NT) {
+        const double sy = src == 2 ? L.yb()[i]
+                          : full(L.yref(), src == 1 ? L.yavg() : L.y(cur), i);
+        L.yout()[i] = 1.0 * sy / L.dr()[i];
+    }
+    __syncthreads();
+}
+
+template <typename V>
+__global__ __launch_bounds__(NT) void k_batch(const Arrays<V>* A, Params P) {
+    __shared__ Ctl C;
+    __shared__ double red[NT / WAVE];
+    unsigned int done = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int q = atomicAdd(P.queue, 1);
+            C.lp = q < P.count ? P.order[q] : -1;
+        }
+        __syncthreads();
+        const int id = C.lp;
+        if (id < 0) break;
+        const LPDesc D = P.lps[id];
+        const LP<V> L = view(A, D);
+        solve_one(L, P, D, C, red, P.res[id]);
+        __syncthreads();
+        if (threadIdx.x == 0 && P.progress)  // this workgroup's own slot
+            __hip_atomic_store(P.progress + blockIdx.x, ++done, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+void launch_batch(const Arrays<float>* a, const Params& p, int grid, hipStream_t s) {
+    k_batch<float><<<grid, NT, 0, s>>>(a, p);
+}
+void launch_batch(const Arrays<double>* a, const Params& p, int grid, hipStream_t s) {
+    k_batch<double><<<grid, NT, 0, s>>>(a, p);
+}
+
+int batch_occupancy(bool f32) {
+    int nb = 0;
+    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<float>, NT, 0);
+    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<double>, NT, 0);
+    return nb > 0 ? nb : 1;
+}
+
+}  // namespace igaos::batch

@@ -1,0 +1,77 @@
+#pragma once
+// Batched persistent PDHG: thousands of independent LPs of uneven sizes
+// solved in one launch (no reference counterpart; the reference solves one
+// LP per call).  Each workgroup pulls LPs from a queue (largest first) and
+// runs a whole solve for each: Ruiz scaling, power iteration, the adaptive
+// line search, averaging, FP64 KKT checks every 64 iterations, restarts
+// with primal-weight updates, incumbent and rollback -- the control flow
+// of the single-LP engine (src/pdhg/pdhg.cu:457-1156) with per-instance
+// state and termination.  The dense host dual repair (pdhg.cu:288-448) is
+// not run in batched mode.
+
+#include <hip/hip_runtime.h>
+
+#include "pdhg_ctrl.hpp"
+
+namespace igaos::batch {
+
+constexpr int NT = 256;  // threads per workgroup (one LP at a time)
+
+struct LPDesc {
+    int m, n, nnz;
+    int tpr;     // threads per row in the SpMV phases (1, 2, 4 or 8)
+    long roff;   // offset into m-sized arrays
+    long coff;   // offset into n-sized arrays
+    long zoff;   // offset into nnz-sized arrays
+    long rpo;    // offset into the concatenated row pointers (m+1 per LP)
+    long cpo;    // offset into the concatenated column pointers (n+1 per LP)
+    double obj_const;
+};
+
+struct LPResult {
+    int status;
+    int last_is_avg;
+    double objective, pinf, dinf, gap, prow;
+    long long iterations, attempts, restarts;
+    double solve_ms;
+    double eta, omega, lam2;
+};
+
+// Concatenated arrays of the batch (device pointers).  V is the iteration
+// precision; in FP64 mode the V* data arrays alias the FP64 ones and the
+// refinement arrays are unused.
+template <typename V>
+struct Arrays {
+    const int *ap, *ai, *cp, *ci;     // local indices, per-LP pointer arrays
+    const double *ax, *acx;           // original values
+    const double *c, *cl, *cu, *rmin, *rmax;
+    double *dr, *dc, *sax, *sacx, *sc, *scl, *scu, *srmin, *srmax;
+    V *fax, *facx;
+    V *x0, *x1, *y0, *y1, *xbar, *ata, *xavg, *yavg, *axcur, *adx;
+    double *xref, *yref, *aref, *atref;
+    V *lo, *hi, *gref, *rlo, *rhi, *yoff;
+    double *xb, *yb, *pesx, *pesy;
+    double *xout, *yout;
+};
+
+struct Params {
+    const LPDesc* lps;
+    const int* order;     // queue order (largest first)
+    int count;
+    int* queue;           // zeroed counter
+    LPResult* res;
+    double tol;
+    long long maxit;
+    double time_limit_s;
+    double ticks_per_s;   // wall-clock counter rate
+    unsigned int* progress;  // optional, pinned host memory: LPs finished per workgroup
+};
+
+// `a` points to a device copy of the pointer table.
+void launch_batch(const Arrays<float>* a, const Params& p, int grid,
+                  hipStream_t s);
+void launch_batch(const Arrays<double>* a, const Params& p, int grid,
+                  hipStream_t s);
+int batch_occupancy(bool f32);
+
+}  // namespace igaos::batch

@@ -1,0 +1,363 @@
+// Host side of the batched persistent PDHG solver: packs a batch of LPs
+// into concatenated device arrays, picks a per-LP SpMV schedule, launches
+// the persistent kernel once per run and returns per-instance results.
+// C entry points: igaos_batch_* in include/igaos_pdhg.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "igaos_pdhg.h"
+
+namespace {
+
+#define BCK(expr)                                                            \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess)                                                \
+            throw std::runtime_error(std::string("HIP error in batched PDHG: ") + \
+                                     hipGetErrorString(e_) + " at " #expr);  \
+    } while (0)
+
+struct Mem {
+    std::vector<void*> ptrs;
+    ~Mem() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    T* alloc(size_t k) {
+        void* p = nullptr;
+        BCK(hipMalloc(&p, sizeof(T) * (k > 0 ? k : 1)));
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    template <typename T>
+    T* upload(const T* h, size_t k) {
+        T* d = alloc<T>(k);
+        if (k) BCK(hipMemcpy(d, h, sizeof(T) * k, hipMemcpyHostToDevice));
+        return d;
+    }
+};
+
+using igaos::batch::LPDesc;
+using igaos::batch::LPResult;
+
+}  // namespace
+
+struct igaos_batch {
+    int count = 0;
+    int precision = 32;
+    long sum_m = 0, sum_n = 0, sum_z = 0;
+    std::vector<LPDesc> desc;
+    std::vector<int> order;
+    Mem mem;
+    LPDesc* d_desc = nullptr;
+    int* d_order = nullptr;
+    int* d_queue = nullptr;
+    LPResult* d_res = nullptr;
+    igaos::batch::Arrays<float> a32{};
+    igaos::batch::Arrays<double> a64{};
+    igaos::batch::Arrays<float>* d_a32 = nullptr;   // device copies
+    igaos::batch::Arrays<double>* d_a64 = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    unsigned int* h_prog = nullptr;  // pinned, coherent: per-workgroup progress
+    unsigned int* d_prog = nullptr;
+    bool running = false;
+    double last_ms = 0.0;
+    int grid = 0;
+    ~igaos_batch() {
+        if (running && e1) (void)hipEventSynchronize(e1);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (h_prog) (void)hipHostFree(h_prog);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+namespace {
+
+thread_local std::string g_berr;
+
+int bfail(const std::string& w) {
+    g_berr = w;
+    return -1;
+}
+
+// SpMV lanes per row from the LP's mean row length (A and A' rows).
+int default_tpr(int m, int n, int nnz) {
+    const double len = m > 0 ? static_cast<double>(nnz) / m : 0.0;
+    const double lenT = n > 0 ? static_cast<double>(nnz) / n : 0.0;
+    const double mean = 0.5 * (len + lenT);
+    if (mean <= 6.0) return 1;
+    if (mean <= 12.0) return 2;
+    if (mean <= 24.0) return 4;
+    return 8;
+}
+
+template <typename V>
+void fill_arrays(igaos_batch& B, igaos::batch::Arrays<V>& A, const int* ap,
+                 const int* ai, const double* ax, const int* cp,
+                 const int* ci, const double* acx, const double* c,
+                 const double* cl, const double* cu, const double* rmin,
+                 const double* rmax) {
+    Mem& M = B.mem;
+    const long m = B.sum_m, n = B.sum_n, z = B.sum_z;
+    A.ap = M.upload(ap, m + B.count);
+    A.ai = M.upload(ai, z);
+    A.ax = M.upload(ax, z);
+    A.cp = M.upload(cp, n + B.count);
+    A.ci = M.upload(ci, z);
+    A.acx = M.upload(acx, z);
+    A.c = M.upload(c, n);
+    A.cl = M.upload(cl, n);
+    A.cu = M.upload(cu, n);
+    A.rmin = M.upload(rmin, m);
+    A.rmax = M.upload(rmax, m);
+    A.dr = M.alloc<double>(m);
+    A.dc = M.alloc<double>(n);
+    A.sax = M.alloc<double>(z);
+    A.sacx = M.alloc<double>(z);
+    A.sc = M.alloc<double>(n);
+    A.scl = M.alloc<double>(n);
+    A.scu = M.alloc<double>(n);
+    A.srmin = M.alloc<double>(m);
+    A.srmax = M.alloc<double>(m);
+    constexpr bool F32 = sizeof(V) == 4;
+    if constexpr (F32) {
+        A.fax = M.alloc<V>(z);
+        A.facx = M.alloc<V>(z);
+        A.xref = M.alloc<double>(n);
+        A.yref = M.alloc<double>(m);
+        A.aref = M.alloc<double>(m);
+        A.atref = M.alloc<double>(n);
+        A.lo = M.alloc<V>(n);
+        A.hi = M.alloc<V>(n);
+        A.gref = M.alloc<V>(n);
+        A.rlo = M.alloc<V>(m);
+        A.rhi = M.alloc<V>(m);
+        A.yoff = M.alloc<V>(m);
+    } else {
+        A.fax = A.sax;
+        A.facx = A.sacx;
+        A.xref = A.yref = A.aref = A.atref = nullptr;
+        A.lo = A.scl;
+        A.hi = A.scu;
+        A.gref = A.sc;
+        A.rlo = A.srmin;
+        A.rhi = A.srmax;
+        A.yoff = nullptr;
+    }
+    for (V** p : {&A.x0, &A.x1, &A.xbar, &A.ata, &A.xavg}) *p = M.alloc<V>(n);
+    for (V** p : {&A.y0, &A.y1, &A.yavg, &A.axcur, &A.adx}) *p = M.alloc<V>(m);
+    A.xb = M.alloc<double>(n);
+    A.yb = M.alloc<double>(m);
+    A.pesx = M.alloc<double>(n);
+    A.pesy = M.alloc<double>(m);
+    A.xout = M.alloc<double>(n);
+    A.yout = M.alloc<double>(m);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* igaos_batch_last_error(void) { return g_berr.c_str(); }
+
+igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
+                                const int* nnz, const double* c,
+                                const double* cl, const double* cu,
+                                const double* rmin, const double* rmax,
+                                const int* ap, const int* ai,
+                                const double* ax, const int* cp,
+                                const int* ci, const double* acx,
+                                const double* obj_const, int precision) {
+    if (count <= 0 || !m || !n || !nnz) {
+        bfail("empty batch");
+        return nullptr;
+    }
+    if (precision != 32 && precision != 64) {
+        bfail("precision must be 32 or 64");
+        return nullptr;
+    }
+    try {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+            (void)hipGetLastError();
+            throw std::runtime_error("batched PDHG unavailable: no HIP GPU device visible");
+        }
+        auto B = std::make_unique<igaos_batch>();
+        B->count = count;
+        B->precision = precision;
+        B->desc.resize(count);
+        long ro = 0, co = 0, zo = 0, rpo = 0, cpo = 0;
+        for (int i = 0; i < count; ++i) {
+            if (m[i] < 0 || n[i] < 0 || nnz[i] < 0) throw std::invalid_argument("negative size");
+            LPDesc& d = B->desc[i];
+            d.m = m[i];
+            d.n = n[i];
+            d.nnz = nnz[i];
+            d.tpr = default_tpr(m[i], n[i], nnz[i]);
+            d.roff = ro;
+            d.coff = co;
+            d.zoff = zo;
+            d.rpo = rpo;
+            d.cpo = cpo;
+            d.obj_const = obj_const ? obj_const[i] : 0.0;
+            if (ap[rpo + m[i]] != nnz[i] || cp[cpo + n[i]] != nnz[i])
+                throw std::invalid_argument("pointer arrays do not match nnz of instance " +
+                                            std::to_string(i));
+            ro += m[i];
+            co += n[i];
+            zo += nnz[i];
+            rpo += m[i] + 1;
+            cpo += n[i] + 1;
+        }
+        B->sum_m = ro;
+        B->sum_n = co;
+        B->sum_z = zo;
+        // largest first, so the long solves start early (LPT order)
+        B->order.resize(count);
+        std::iota(B->order.begin(), B->order.end(), 0);
+        std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) {
+            return B->desc[a].nnz + B->desc[a].m + B->desc[a].n >
+                   B->desc[b].nnz + B->desc[b].m + B->desc[b].n;
+        });
+        BCK(hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking));
+        if (precision == 32) {
+            fill_arrays(*B, B->a32, ap, ai, ax, cp, ci, acx, c, cl, cu, rmin, rmax);
+            B->d_a32 = B->mem.upload(&B->a32, 1);
+        } else {
+            fill_arrays(*B, B->a64, ap, ai, ax, cp, ci, acx, c, cl, cu, rmin, rmax);
+            B->d_a64 = B->mem.upload(&B->a64, 1);
+        }
+        B->d_desc = B->mem.upload(B->desc.data(), count);
+        B->d_order = B->mem.upload(B->order.data(), count);
+        B->d_queue = B->mem.alloc<int>(1);
+        B->d_res = B->mem.alloc<LPResult>(count);
+        hipDeviceProp_t prop;
+        BCK(hipGetDeviceProperties(&prop, 0));
+        const int occ = igaos::batch::batch_occupancy(precision == 32);
+        B->grid = std::min(count, prop.multiProcessorCount * occ);
+        BCK(hipEventCreate(&B->e0));
+        BCK(hipEventCreate(&B->e1));
+        BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_prog),
+                          sizeof(unsigned int) * count,
+                          hipHostMallocMapped | hipHostMallocCoherent));
+        BCK(hipHostGetDevicePointer(reinterpret_cast<void**>(&B->d_prog), B->h_prog, 0));
+        return B.release();
+    } catch (const std::exception& e) {
+        bfail(e.what());
+        return nullptr;
+    }
+}
+
+int igaos_batch_set_schedule(igaos_batch* B, const int* tpr, int grid) {
+    if (!B) return bfail("null batch");
+    try {
+        if (tpr) {
+            for (int i = 0; i < B->count; ++i) {
+                const int t = tpr[i];
+                if (t != 1 && t != 2 && t != 4 && t != 8) return bfail("tpr must be 1, 2, 4 or 8");
+                B->desc[i].tpr = t;
+            }
+            BCK(hipMemcpy(B->d_desc, B->desc.data(), sizeof(LPDesc) * B->count,
+                          hipMemcpyHostToDevice));
+        }
+        if (grid > 0) B->grid = std::min(grid, B->count);
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
+
+int igaos_batch_get_schedule(const igaos_batch* B, int* tpr, int* grid) {
+    if (!B) return bfail("null batch");
+    if (tpr)
+        for (int i = 0; i < B->count; ++i) tpr[i] = B->desc[i].tpr;
+    if (grid) *grid = B->grid;
+    return 0;
+}
+
+int igaos_batch_launch(igaos_batch* B, const igaos_options* opt) {
+    if (!B || !opt) return bfail("null batch or options");
+    if (B->running) return bfail("batch already running");
+    try {
+        igaos::batch::Params P{};
+        P.lps = B->d_desc;
+        P.order = B->d_order;
+        P.count = B->count;
+        P.queue = B->d_queue;
+        P.res = B->d_res;
+        P.tol = opt->tolerance;
+        // pdhg.cu:669-671: the iteration cap applies only without a budget
+        P.maxit = (opt->time_limit_s > 0 && opt->time_limit_s < 1e6) ? 20000000LL
+                                                                     : opt->max_iterations;
+        P.time_limit_s = opt->time_limit_s;
+        int khz = 0;
+        BCK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
+        P.ticks_per_s = static_cast<double>(khz) * 1e3;
+        P.progress = B->d_prog;
+        std::memset(B->h_prog, 0, sizeof(unsigned int) * B->count);
+        BCK(hipMemsetAsync(B->d_queue, 0, sizeof(int), B->s));
+        BCK(hipEventRecord(B->e0, B->s));
+        if (B->precision == 32) igaos::batch::launch_batch(B->d_a32, P, B->grid, B->s);
+        else igaos::batch::launch_batch(B->d_a64, P, B->grid, B->s);
+        BCK(hipGetLastError());
+        BCK(hipEventRecord(B->e1, B->s));
+        B->running = true;
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
+
+int igaos_batch_progress(const igaos_batch* B) {
+    if (!B) return bfail("null batch");
+    long done = 0;
+    const volatile unsigned int* p = B->h_prog;
+    for (int g = 0; g < B->grid; ++g) done += p[g];
+    return static_cast<int>(done);
+}
+
+int igaos_batch_wait(igaos_batch* B, double* x, double* y,
+                     igaos_batch_result* res) {
+    if (!B) return bfail("null batch");
+    if (!B->running) return bfail("batch not launched");
+    static_assert(sizeof(igaos_batch_result) == sizeof(LPResult), "result layout");
+    try {
+        B->running = false;
+        BCK(hipEventSynchronize(B->e1));
+        float ms = 0.0f;
+        BCK(hipEventElapsedTime(&ms, B->e0, B->e1));
+        B->last_ms = ms;
+        const double* xo = B->precision == 32 ? B->a32.xout : B->a64.xout;
+        const double* yo = B->precision == 32 ? B->a32.yout : B->a64.yout;
+        if (x) BCK(hipMemcpy(x, xo, sizeof(double) * B->sum_n, hipMemcpyDeviceToHost));
+        if (y) BCK(hipMemcpy(y, yo, sizeof(double) * B->sum_m, hipMemcpyDeviceToHost));
+        if (res)
+            BCK(hipMemcpy(res, B->d_res, sizeof(LPResult) * B->count, hipMemcpyDeviceToHost));
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
+
+int igaos_batch_run(igaos_batch* B, const igaos_options* opt, double* x,
+                    double* y, igaos_batch_result* res) {
+    const int rc = igaos_batch_launch(B, opt);
+    if (rc != 0) return rc;
+    return igaos_batch_wait(B, x, y, res);
+}
+
+double igaos_batch_kernel_ms(const igaos_batch* B) { return B ? B->last_ms : 0.0; }
+
+void igaos_batch_destroy(igaos_batch* B) { delete B; }
+
+}  // extern "C"

@@ -137,6 +137,55 @@ int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
                                 double* ms_primal, double* ms_dual,
                                 int* blocks_primal, int* blocks_dual);
 
+/* ---- batched persistent solve (no reference counterpart) ----------------
+ * Thousands of independent LPs solved in one persistent launch: each
+ * workgroup takes instances from a queue (largest first) and runs the whole
+ * PDHG solve of pdhg.cu:457-1156 for each, with per-instance line search,
+ * restarts, termination and results (the dense dual repair is not run).
+ * Host arrays are the instances' arrays concatenated in order: c, cl, cu
+ * by n_i; rmin, rmax by m_i; ap (m_i + 1 entries per instance) and cp
+ * (n_i + 1) hold instance-local offsets; ai, ci instance-local indices. */
+typedef struct igaos_batch igaos_batch;
+/* Per-instance result; status uses the IGAOS_* codes. */
+typedef struct {
+    int status;
+    int last_is_avg;
+    double objective, pinf, dinf, rel_gap, prow;
+    long long iterations, attempts, restarts;
+    double solve_ms;
+    double eta, omega, lam2;
+} igaos_batch_result;
+/* precision: 32 = FP32 iterations + FP64 refinement, 64 = plain FP64.
+ * Uploads the batch; returns NULL on failure. */
+igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
+                                const int* nnz, const double* c,
+                                const double* cl, const double* cu,
+                                const double* rmin, const double* rmax,
+                                const int* ap, const int* ai,
+                                const double* ax, const int* cp,
+                                const int* ci, const double* acx,
+                                const double* obj_const, int precision);
+/* Solves every instance from x = y = 0 with the options' tolerance,
+ * max_iterations / time_limit_s (per instance, as in igaos_pdhg_solve).
+ * x (host, sum n_i), y (host, sum m_i) and res (count) may be NULL. */
+int igaos_batch_run(igaos_batch* batch, const igaos_options* opt, double* x,
+                    double* y, igaos_batch_result* res);
+/* The same as launch + wait: launch returns once the persistent kernel is
+ * enqueued; progress returns how many instances have finished so far (read
+ * from pinned host memory, no synchronisation); wait blocks and copies. */
+int igaos_batch_launch(igaos_batch* batch, const igaos_options* opt);
+int igaos_batch_progress(const igaos_batch* batch);
+int igaos_batch_wait(igaos_batch* batch, double* x, double* y,
+                     igaos_batch_result* res);
+/* Schedule: SpMV lanes per row of each instance (1, 2, 4 or 8; NULL keeps
+ * the current ones) and the persistent grid size (<= 0 keeps it). */
+int igaos_batch_set_schedule(igaos_batch* batch, const int* tpr, int grid);
+int igaos_batch_get_schedule(const igaos_batch* batch, int* tpr, int* grid);
+/* Device time of the last run's persistent kernel (HIP events), ms. */
+double igaos_batch_kernel_ms(const igaos_batch* batch);
+const char* igaos_batch_last_error(void);
+void igaos_batch_destroy(igaos_batch* batch);
+
 /* ---- kernel layer (device pointers, HIP stream or NULL) ----------------- */
 /* Host: row-block layout of a CSR row pointer (host) for the SpMV
  * kernels: pairs (first row, first nonzero) of each block followed by the

@@ -1,0 +1,103 @@
+"""GPU tests of the batched persistent PDHG solver (igaos_batch_* C-ABI).
+
+Bars: every instance of a batch certifies (near-optimal) at tolerance 1e-6
+with its objective within 1e-6 relative of the reference simplex / HiGHS
+fixture; in FP64 mode an instance follows the oracle's trajectory to 1e-9
+over one check period and ends with the same status as the single-LP
+engine; an instance's result does not depend on the batch around it.
+"""
+import numpy as np
+import pytest
+
+import igaos_amd
+import oracle
+from igaos_amd import lpgen
+from igaos_amd.batch import BatchSolver, solve_batch
+from tests.conftest import netlib_path
+
+pytestmark = pytest.mark.gpu
+
+
+def _fixture_obj(objectives, key):
+    ent = objectives[key]
+    return ent.get("ref_simplex", ent["scipy_highs"])
+
+
+def _c4_key(i):
+    return f"gen_batch_instance(seed=0,index={i})"
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_batch_solves_match_fixtures(precision, objectives):
+    lps = [lpgen.gen_batch_instance(0, i) for i in range(16)]
+    sol = solve_batch(lps, precision=precision, tolerance=1e-6, time_limit=120)
+    for i, lp in enumerate(lps):
+        ref = _fixture_obj(objectives, _c4_key(i))
+        assert sol.status[i] == "near-optimal", (i, sol.status[i])
+        assert sol.pinf[i] <= 1e-6 and sol.rel_gap[i] <= 1e-6
+        assert abs(sol.objective[i] - ref) <= 1e-6 * (1 + abs(ref)), (i, sol.objective[i], ref)
+        # the returned pair passes the oracle's KKT evaluation too
+        A = lp.to_scipy()
+        mt = oracle.evaluate(lp.rmin, lp.rmax, lp.cl, lp.cu, lp.c, sol.x[i],
+                             sol.y[i], A @ sol.x[i], lp.c + A.T @ sol.y[i])
+        assert mt.pinf <= 1e-6 and mt.gap <= 1e-6
+
+
+def test_batch_fp64_trajectory_matches_oracle():
+    lps = [lpgen.gen_batch_instance(0, i, m_hi=600) for i in range(6)]
+    sol = solve_batch(lps, precision=64, tolerance=0.0, time_limit=1e7,
+                      max_iterations=64)
+    for i, lp in enumerate(lps):
+        r = oracle.solve(lp, tolerance=0.0, time_limit=1e7, max_iterations=64)
+        assert sol.iterations[i] == 64 and sol.attempts[i] == r.attempts
+        err = max(np.abs(sol.x[i] - r.x).max() / max(1, np.abs(r.x).max()),
+                  np.abs(sol.y[i] - r.y).max() / max(1, np.abs(r.y).max()))
+        assert err <= 1e-9, (i, err)
+
+
+def test_batch_fp64_matches_single_engine():
+    lps = [lpgen.gen_batch_instance(3, i, m_hi=800) for i in range(4)]
+    sol = solve_batch(lps, precision=64, tolerance=1e-6, time_limit=60)
+    for i, lp in enumerate(lps):
+        s = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=60)
+        assert sol.status[i] == s.status == "near-optimal"
+        assert abs(sol.objective[i] - s.objective) <= 1e-6 * (1 + abs(s.objective))
+
+
+def test_instance_independent_of_batch():
+    lps = [lpgen.gen_batch_instance(1, i, m_hi=400) for i in range(10)]
+    whole = solve_batch(lps, precision=32, tolerance=1e-6, time_limit=60)
+    alone = solve_batch([lps[4]], precision=32, tolerance=1e-6, time_limit=60)
+    assert whole.iterations[4] == alone.iterations[0]
+    assert np.array_equal(whole.x[4], alone.x[0])
+    assert np.array_equal(whole.y[4], alone.y[0])
+
+
+def test_batch_general_rows_and_bounds(objectives):
+    # Netlib instances: inequality rows, free and fixed columns, RANGES
+    names = ["afiro", "sc50a", "adlittle"]
+    lps = [igaos_amd.read_mps(netlib_path(n)).to_lp() for n in names]
+    sol = solve_batch(lps, precision=32, tolerance=1e-6, time_limit=120)
+    for i, n in enumerate(names):
+        ref = objectives[f"netlib/{n}"]["highs_ref"]
+        assert sol.status[i] == "near-optimal", (n, sol.status[i])
+        assert abs(sol.objective[i] - ref) <= 1e-6 * (1 + abs(ref)) + 5e-10 * abs(ref)
+
+
+def test_batch_iteration_cap_and_schedule():
+    lps = [lpgen.gen_batch_instance(2, i, m_hi=300) for i in range(5)]
+    b = BatchSolver(lps, precision=32)
+    tpr, grid = b.schedule()
+    assert set(tpr) <= {1, 2, 4, 8} and 1 <= grid <= 5
+    s1 = b.run(tolerance=0.0, time_limit=1e7, max_iterations=100)
+    assert all(s == "iteration-limit" for s in s1.status)
+    assert list(s1.iterations) == [100] * 5
+    # another lanes-per-row schedule changes the summation order only
+    b.set_schedule(np.full(5, 4))
+    s2 = b.run(tolerance=1e-6, time_limit=60)
+    b.set_schedule(np.full(5, 1))
+    s3 = b.run(tolerance=1e-6, time_limit=60)
+    for i in range(5):
+        assert s2.status[i] == s3.status[i] == "near-optimal"
+        assert abs(s2.objective[i] - s3.objective[i]) <= 1e-6 * (1 + abs(s3.objective[i]))
+    b.close()
