@@ -1,27 +1,31 @@
-"""Benchmark of the MI355X PDHG engine (one JSON line on stdout).
+"""Benchmark of the MI355X PDHG engines (one JSON line on stdout).
 
-Default workload: the large badly scaled LP — n = 1,000,000 columns,
-m = 400,000 rows (half =, half <=), column counts clip(Zipf(1.5), 2, 2000)
-taken literally (68.5M nonzeros), values U[-1,1]*10^U[-3,3], bounds
-[0, U[1,100]], seed 0 (igaos_amd.lpgen.gen_zipf_badly_scaled).
+Default workload ("batch"): 8,192 independent random LPs (seed 0) with
+m log-uniform in [32, 4096], n = round(m * U[1.5, 3]), 4-8 nonzeros per
+column with values N(0,1), rhs from a feasible x0 ~ U[0,1], box bounds
+[0, 10], c ~ N(0,1) (igaos_amd.lpgen.gen_batch).  A step is one persistent
+launch that solves every instance to relative KKT 1e-6 with FP32 iterations
+and FP64 refinement (per-instance budget --time-limit seconds); ``value``
+is instances certified near-optimal per second of that launch, with the
+batch already resident in HBM.  Beside it: the plain FP64 single-LP
+engine run instance by instance on a fixed sample (every 512th instance),
+timed with its own HBM-resident setup excluded, extrapolated to LPs/s.
 
-A step is one full solve to relative KKT 1e-6 (status near-optimal) with
-the FP32 + FP64-refinement engine, starting from x = y = 0 with the model
-already resident in HBM (upload, Ruiz scaling and the power iteration run
-once, untimed, as in the reference's solve_time_ms, src/pdhg/pdhg.cu:692).
-``value`` is LPs solved per second.  The plain FP64 engine's solve of the
-same LP is timed outside the timed region and reported beside it.
+``roofline`` (bound: HBM) uses the algorithmic byte model per accepted
+iteration of an instance, B = 2 nnz (w_a + 4) + 4 (m + n + 2) +
+w (7n + 8m) with w_a = w = 4 (FP32), summed over every instance's attempts
+of the launch and divided by the launch's duration (HIP events on the
+batch stream).  ``traffic`` is read from a committed rocprofv3 --pmc
+summary for the same command when present.
 
-``roofline`` is taken from the dominant kernel of an attempt, timed with
-HIP events on the engine's stream (igaos_pdhg_profile_attempts).  Its
-algorithmic bytes per launch follow the per-iteration byte model of
-B = 2 nnz (w_a + 4) + 4 (m + n + 2) + w (7n + 8m), split per kernel:
-  k_primal (A'y + primal step): nnz (w_a + 4) + 4 (n + 1) + 7 n w
-  k_dual   (A xbar + dual step): nnz (w_a + 4) + 4 (m + 1) + 8 m w
-with w_a, w the value / vector widths (4 for FP32, 8 for FP64).
+``--workload zipf`` instead measures the single-LP attempt kernels on the
+badly scaled LP (n = 1,000,000, m = 400,000, clip(Zipf(1.5), 2, 2000)
+nonzeros per column, 68.5M nonzeros) in FP64 and FP32, plus a bounded
+FP32 solve (the LP needs well over 10^4 iterations to reach 1e-6).
+
 ``cpu_baseline`` is the oracle (a single-threaded C host replay of the
-reference engine) timed on a bounded sample: setup plus 64 iterations of
-the same LP, extrapolated to solves/s at the GPU solve's iteration count.
+reference engine) on a bounded sample of the same workload, named in
+``sample``.
 """
 from __future__ import annotations
 
@@ -43,65 +47,208 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(name):
-    from igaos_amd import lpgen
-    if name == "zipf":
-        return lpgen.gen_zipf_badly_scaled(seed=0), dict(
-            workload="badly scaled Zipf LP, m=400000 n=1000000 seed 0",
-            m=400000, n=1000000)
-    if name == "random_eq":
-        return lpgen.gen_random_eq(seed=0), dict(
-            workload="random sparse equality LP, m=1000 n=3000 seed 0",
-            m=1000, n=3000)
-    if name == "multiperiod":
-        return lpgen.gen_multiperiod(seed=0), dict(
-            workload="multi-period blending LP, 96 periods of 5000x8000 seed 0",
-            m=480000, n=768000)
-    raise SystemExit(f"unknown workload {name}")
+def bytes_per_iteration(m, n, nnz, w_a, w):
+    return 2 * nnz * (w_a + 4) + 4 * (m + n + 2) + w * (7 * n + 8 * m)
 
 
-def kernel_bytes(lp, w_a, w):
-    nnz = lp.nnz
-    primal = nnz * (w_a + 4) + 4 * (lp.n + 1) + 7 * lp.n * w
-    dual = nnz * (w_a + 4) + 4 * (lp.m + 1) + 8 * lp.m * w
-    return primal, dual
-
-
-def roofline_of(lp, eng, precision, attempts):
-    t = eng.profile_attempts(attempts)
-    w = 4 if precision == 32 else 8
-    bp, bd = kernel_bytes(lp, w, w)
-    gp = bp / (t["ms_primal"] * 1e-3) / 1e9
-    gd = bd / (t["ms_dual"] * 1e-3) / 1e9
-    if t["ms_dual"] >= t["ms_primal"]:
-        name, ms, b, g = "k_dual", t["ms_dual"], bd, gd
-    else:
-        name, ms, b, g = "k_primal", t["ms_primal"], bp, gp
-    return dict(kernel=name, ms=ms, bytes=b, gbs=g, primal_gbs=gp,
-                dual_gbs=gd, ms_primal=t["ms_primal"], ms_dual=t["ms_dual"])
-
-
-def load_traffic(kernel, workload):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.json")
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", f"{name}_pmc.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(lp, gpu_iters):
-    import oracle
-    t0 = time.perf_counter()
-    r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=64)
-    wall = time.perf_counter() - t0
-    its_per_s = r.iterations / (r.solve_time_ms * 1e-3)
-    return {"value": its_per_s / max(gpu_iters, 1), "unit": "LPs/s",
-            "cores": 1, "kind": "port",
-            "sample": (f"oracle host replay, 1 thread: setup + 64 iterations "
-                       f"({wall:.1f} s wall, {its_per_s:.2f} it/s), "
-                       f"extrapolated at the GPU solve's {gpu_iters} "
-                       f"iterations")}
+# ---------------------------------------------------------------- batch --
+def bench_batch(args):
+    import numpy as np
+    import igaos_amd
+    from igaos_amd import lpgen
+    from igaos_amd.batch import BatchSolver
+
+    count = args.count
+    t = time.perf_counter()
+    lps = lpgen.gen_batch(0, count)
+    nnz = np.array([lp.nnz for lp in lps])
+    ms = np.array([lp.m for lp in lps])
+    ns = np.array([lp.n for lp in lps])
+    log(f"[bench] {count} LPs generated in {time.perf_counter() - t:.1f} s: "
+        f"sum m={ms.sum()} n={ns.sum()} nnz={nnz.sum()}")
+    b = BatchSolver(lps, precision=32)
+    tpr, grid = b.schedule()
+    log(f"[bench] batch resident on the GPU, persistent grid {grid}")
+    t0 = [0.0]
+
+    def prog(done, total):
+        log(f"[bench]   {done}/{total} instances done at "
+            f"{time.perf_counter() - t0[0]:.1f} s")
+
+    for w in range(args.warmup):
+        t0[0] = time.perf_counter()
+        s = b.run(tolerance=args.tolerance, time_limit=args.time_limit,
+                  want_xy=False, progress=prog, poll_s=10.0)
+        log(f"[bench] warmup {w}: {s.solved}/{count} solved")
+    solved = 0
+    elapsed = 0.0
+    kernel_ms = 0.0
+    sol = None
+    for k in range(args.steps):
+        t0[0] = time.perf_counter()
+        sol = b.run(tolerance=args.tolerance, time_limit=args.time_limit,
+                    want_xy=False, progress=prog, poll_s=10.0)
+        elapsed += time.perf_counter() - t0[0]
+        kernel_ms += sol.kernel_ms
+        solved += sol.solved
+        log(f"[bench] step {k}: {sol.solved}/{count} solved, launch "
+            f"{sol.kernel_ms:.0f} ms")
+    b.close()
+    value = solved / elapsed
+    its = np.array(sol.iterations)
+    att = np.array(sol.attempts)
+    alg = float((att * bytes_per_iteration(ms, ns, nnz, 4, 4)).sum())
+    launch_ms = kernel_ms / args.steps
+    achieved = alg / (launch_ms * 1e-3) / 1e9
+
+    # plain FP64 single-LP engine, instance by instance, on a fixed sample
+    sample = list(range(0, count, max(1, count // args.fp64_sample)))
+    t64 = 0.0
+    solved64 = 0
+    for i in sample:
+        e = igaos_amd.Engine(lps[i], precision=64)
+        s = e.run(time_limit=args.time_limit, tolerance=args.tolerance)
+        e.close()
+        t64 += s.solve_time_ms * 1e-3
+        solved64 += s.status == "near-optimal"
+    fp64_rate = solved64 / t64 if t64 > 0 else 0.0
+    log(f"[bench] plain FP64 per instance: {solved64}/{len(sample)} solved "
+        f"in {t64:.1f} s -> {fp64_rate:.2f} LPs/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        cs = sample[: args.cpu_sample]
+        tc = 0.0
+        sc = 0
+        wall = time.perf_counter()
+        for i in cs:
+            r = oracle.solve(lps[i], tolerance=args.tolerance,
+                             time_limit=args.cpu_time_limit)
+            tc += r.solve_time_ms * 1e-3
+            sc += r.status == "near-optimal"
+        wall = time.perf_counter() - wall
+        cpu = {"value": sc / tc if tc > 0 else 0.0, "unit": "LPs/s",
+               "cores": 1, "kind": "port",
+               "sample": (f"oracle host replay (1 thread) on instances "
+                          f"{cs} of the batch, {sc}/{len(cs)} certified in "
+                          f"{tc:.1f} s of solve time ({wall:.1f} s wall incl. "
+                          f"setup), budget {args.cpu_time_limit} s each")}
+
+    return {
+        "metric": METRIC, "value": value, "unit": "LPs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1000.0 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64refine", "data": "synthetic",
+        "config": {"workload": (f"batch of {count} random LPs, m log-uniform "
+                                f"[32,4096], n=m*U[1.5,3], 4-8 nnz/col, seed 0"),
+                   "instances": count, "sum_m": int(ms.sum()),
+                   "sum_n": int(ns.sum()), "sum_nnz": int(nnz.sum()),
+                   "tolerance": args.tolerance,
+                   "time_limit_per_instance_s": args.time_limit,
+                   "persistent_grid": int(grid)},
+        "solved": solved, "instances": count * args.steps,
+        "iterations_median": float(np.median(its)),
+        "iterations_p99": float(np.percentile(its, 99)),
+        "launch_ms": launch_ms,
+        "plain_fp64_per_instance": {
+            "value": fp64_rate, "unit": "LPs/s", "sample": sample,
+            "solved": solved64, "solve_seconds": round(t64, 3)},
+        "batched_vs_plain_fp64": (value / fp64_rate) if fp64_rate > 0 else None,
+        "roofline": {"bound": "hbm", "kernel": "k_batch<float>",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": load_traffic("k_batch_batch8192"),
+                     "bytes_per_launch": alg,
+                     "ms_per_launch": round(launch_ms, 3)},
+        "cpu_baseline": cpu,
+    }
+
+
+# ----------------------------------------------------------------- zipf --
+def bench_zipf(args):
+    import numpy as np
+    import igaos_amd
+    from igaos_amd import lpgen
+
+    t = time.perf_counter()
+    lp = lpgen.gen_zipf_badly_scaled(seed=0)
+    log(f"[bench] zipf LP nnz={lp.nnz} generated in "
+        f"{time.perf_counter() - t:.1f} s")
+    kern = {}
+    for prec in (64, 32):
+        e = igaos_amd.Engine(lp, precision=prec)
+        tm = e.profile_attempts(args.attempts)
+        w = 8 if prec == 64 else 4
+        bp = lp.nnz * (w + 4) + 4 * (lp.n + 1) + 7 * lp.n * w
+        bd = lp.nnz * (w + 4) + 4 * (lp.m + 1) + 8 * lp.m * w
+        kern[prec] = dict(ms_primal=tm["ms_primal"], ms_dual=tm["ms_dual"],
+                          gbs_primal=bp / tm["ms_primal"] / 1e6,
+                          gbs_dual=bd / tm["ms_dual"] / 1e6,
+                          bytes_primal=bp, bytes_dual=bd)
+        log(f"[bench] FP{prec}: k_primal {tm['ms_primal']:.3f} ms "
+            f"({kern[prec]['gbs_primal']:.0f} GB/s), k_dual "
+            f"{tm['ms_dual']:.3f} ms ({kern[prec]['gbs_dual']:.0f} GB/s)")
+        if prec == 32:
+            torch_sync()
+            t0 = time.perf_counter()
+            s = e.run(time_limit=args.time_limit, tolerance=args.tolerance,
+                      verbosity=1)
+            el = time.perf_counter() - t0
+            log(f"[bench] FP32 solve: {s.status} {s.iterations} it {el:.1f} s")
+        e.close()
+    k64 = kern[64]
+    dom = "k_dual" if k64["ms_dual"] >= k64["ms_primal"] else "k_primal"
+    ms_dom = k64["ms_dual" if dom == "k_dual" else "ms_primal"]
+    b_dom = k64["bytes_dual" if dom == "k_dual" else "bytes_primal"]
+    achieved = b_dom / ms_dom / 1e6
+    solved = 1 if s.status == "near-optimal" else 0
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        t0 = time.perf_counter()
+        r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=64)
+        wall = time.perf_counter() - t0
+        ips = r.iterations / (r.solve_time_ms * 1e-3)
+        cpu = {"value": ips, "unit": "iterations/s", "cores": 1,
+               "kind": "port",
+               "sample": (f"oracle host replay (1 thread): setup + 64 "
+                          f"iterations of the same LP, {wall:.1f} s wall")}
+    return {
+        "metric": METRIC, "value": solved / el, "unit": "LPs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": el * 1000.0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "badly scaled Zipf LP, m=400000 n=1000000 "
+                               "nnz=68528436 seed 0",
+                   "tolerance": args.tolerance,
+                   "time_limit_s": args.time_limit},
+        "fp32_solve": {"status": s.status, "iterations": s.iterations,
+                       "iterations_per_s": s.iterations / el,
+                       "pinf": s.pinf, "dinf": s.dinf, "rel_gap": s.rel_gap},
+        "attempt_kernels": kern,
+        "roofline": {"bound": "hbm", "kernel": f"{dom}<double>",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": load_traffic(f"{dom}_zipf"),
+                     "bytes_per_launch": b_dom,
+                     "ms_per_launch": round(ms_dom, 4)},
+        "cpu_baseline": cpu,
+    }
+
+
+def torch_sync():
+    import torch
+    torch.cuda.synchronize()
 
 
 def main():
@@ -109,107 +256,26 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--workload", default="zipf",
-                    choices=["zipf", "random_eq", "multiperiod"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "zipf"])
+    ap.add_argument("--count", type=int, default=8192)
     ap.add_argument("--tolerance", type=float, default=1e-6)
-    ap.add_argument("--time-limit", type=float, default=300.0)
+    ap.add_argument("--time-limit", type=float, default=60.0)
     ap.add_argument("--attempts", type=int, default=20)
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp64-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=6)
+    ap.add_argument("--cpu-time-limit", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU "
-                          "sharding is a design only (see DESIGN.md)",
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
+                                   "sharding is described in DESIGN.md only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-
     import torch
     import igaos_amd
-    assert torch.cuda.is_available() and igaos_amd.has_pdhg
-    torch.cuda.init()
-
-    t = time.perf_counter()
-    lp, cfg = make_workload(args.workload)
-    log(f"[bench] workload {cfg['workload']}: nnz={lp.nnz} "
-        f"(generated in {time.perf_counter() - t:.1f} s)")
-
-    # warm the HIP runtime on a tiny LP (not a step)
-    from igaos_amd import lpgen
-    igaos_amd.solve_lp(lpgen.gen_random_eq(seed=2, m=60, n=150),
-                       tolerance=1e-6, time_limit=10)
-
-    eng = igaos_amd.Engine(lp, precision=32)
-    log(f"[bench] FP32+refine engine ready, setup {eng.setup_ms:.0f} ms")
-    for w in range(args.warmup):
-        s = eng.run(time_limit=args.time_limit, tolerance=args.tolerance)
-        log(f"[bench] warmup {w}: {s.status} {s.iterations} it "
-            f"{s.solve_time_ms:.0f} ms")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sols = []
-    for k in range(args.steps):
-        sols.append(eng.run(time_limit=args.time_limit,
-                            tolerance=args.tolerance))
-        log(f"[bench] step {k}: {sols[-1].status} {sols[-1].iterations} it "
-            f"obj {sols[-1].objective:.10g} {sols[-1].solve_time_ms:.0f} ms")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    solved = sum(s.status == "near-optimal" for s in sols)
-    ms_per_step = elapsed * 1000.0 / args.steps
-    value = solved / elapsed
-    s32 = sols[-1]
-    rf32 = roofline_of(lp, eng, 32, args.attempts)
-    log(f"[bench] FP32 attempt: k_primal {rf32['ms_primal']:.3f} ms "
-        f"({rf32['primal_gbs']:.0f} GB/s), k_dual {rf32['ms_dual']:.3f} ms "
-        f"({rf32['dual_gbs']:.0f} GB/s)")
-    eng.close()
-
-    fp64 = None
-    if not args.no_fp64:
-        e64 = igaos_amd.Engine(lp, precision=64)
-        t64 = time.perf_counter()
-        s64 = e64.run(time_limit=args.time_limit, tolerance=args.tolerance)
-        t64 = time.perf_counter() - t64
-        rf64 = roofline_of(lp, e64, 64, args.attempts)
-        e64.close()
-        fp64 = dict(status=s64.status, iterations=s64.iterations,
-                    seconds=round(t64, 3),
-                    solves_per_s=(1.0 / t64) if s64.status == "near-optimal" else 0.0,
-                    objective=s64.objective,
-                    k_primal_gbs=round(rf64["primal_gbs"], 1),
-                    k_dual_gbs=round(rf64["dual_gbs"], 1),
-                    hbm_frac_dominant=round(rf64["gbs"] / PEAK_HBM_GBS, 4))
-        log(f"[bench] FP64 plain: {s64.status} {s64.iterations} it "
-            f"{t64:.2f} s; k_primal {rf64['primal_gbs']:.0f} GB/s, "
-            f"k_dual {rf64['dual_gbs']:.0f} GB/s")
-
-    cpu = None if args.no_cpu_baseline else cpu_baseline(lp, s32.iterations)
-    traffic = load_traffic(rf32["kernel"], args.workload)
-    line = {
-        "metric": METRIC, "value": value, "unit": "LPs/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64refine",
-        "data": "synthetic",
-        "config": {"workload": cfg["workload"], "m": lp.m, "n": lp.n,
-                   "nnz": lp.nnz, "tolerance": args.tolerance,
-                   "engine": "FP32 inner + FP64 refinement, device line search, hipGraph attempts"},
-        "iterations": s32.iterations,
-        "iterations_per_s": s32.iterations / (s32.solve_time_ms * 1e-3),
-        "objective": s32.objective,
-        "status": s32.status,
-        "setup_ms": round(s32.info["setup_ms"], 1),
-        "plain_fp64": fp64,
-        "fp32_vs_fp64_speedup": (round(fp64["seconds"] / (ms_per_step / 1000), 3)
-                                 if fp64 else None),
-        "roofline": {"bound": "hbm", "kernel": rf32["kernel"],
-                     "achieved": round(rf32["gbs"], 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(rf32["gbs"] / PEAK_HBM_GBS, 4),
-                     "traffic": traffic,
-                     "bytes_per_launch": rf32["bytes"],
-                     "ms_per_launch": round(rf32["ms"], 4)},
-        "cpu_baseline": cpu,
-    }
+    if not (torch.cuda.is_available() and igaos_amd.has_pdhg):
+        raise SystemExit("bench.py needs a HIP GPU")
+    line = bench_batch(args) if args.workload == "batch" else bench_zipf(args)
     print(json.dumps(line), flush=True)
 
 

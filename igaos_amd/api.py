@@ -78,9 +78,10 @@ class Model:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h and _core._lib is not None:
-            _core._lib.igaos_model_free(h)
-            self._h = None
+        lib = getattr(_core, "_lib", None) if _core is not None else None
+        if h and lib is not None:
+            lib.igaos_model_free(h)
+        self._h = None
 
     def nnz(self):
         return self._nnz
@@ -231,9 +232,11 @@ class Engine:
                     blocks_primal=ga.value, blocks_dual=gb.value)
 
     def close(self):
-        if getattr(self, "_h", None) and _core._lib is not None:
-            _core._lib.igaos_pdhg_destroy(self._h)
-            self._h = None
+        h = getattr(self, "_h", None)
+        lib = getattr(_core, "_lib", None) if _core is not None else None
+        if h and lib is not None:
+            lib.igaos_pdhg_destroy(h)
+        self._h = None
 
     def __del__(self):
         self.close()

@@ -128,9 +128,11 @@ class BatchSolver:
         return sol
 
     def close(self):
-        if getattr(self, "_h", None) and _core._lib is not None:
-            _core._lib.igaos_batch_destroy(self._h)
-            self._h = None
+        h = getattr(self, "_h", None)
+        lib = getattr(_core, "_lib", None) if _core is not None else None
+        if h and lib is not None:
+            lib.igaos_batch_destroy(h)
+        self._h = None
 
     def __del__(self):
         self.close()
