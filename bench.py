@@ -74,8 +74,14 @@ def bench_batch(args):
     ns = np.array([lp.n for lp in lps])
     log(f"[bench] {count} LPs generated in {time.perf_counter() - t:.1f} s: "
         f"sum m={ms.sum()} n={ns.sum()} nnz={nnz.sum()}")
+    from igaos_amd import autotune
+    t = time.perf_counter()
+    sched = autotune.tune(lps, log=log)
+    log(f"[bench] autotuned in {time.perf_counter() - t:.1f} s: tpr by class "
+        f"{sched.tpr} (nnz edges {sched.edges}), waves/SIMD {sched.waves}")
     b = BatchSolver(lps, precision=32)
-    tpr, grid = b.schedule()
+    autotune.apply(b, sched)
+    tpr, grid, waves = b.schedule()
     log(f"[bench] batch resident on the GPU, persistent grid {grid}")
     t0 = [0.0]
 
@@ -155,7 +161,10 @@ def bench_batch(args):
                    "sum_n": int(ns.sum()), "sum_nnz": int(nnz.sum()),
                    "tolerance": args.tolerance,
                    "time_limit_per_instance_s": args.time_limit,
-                   "persistent_grid": int(grid)},
+                   "persistent_grid": int(grid),
+                   "schedule": {"tpr_by_class": sched.tpr,
+                                "nnz_class_edges": sched.edges,
+                                "waves_per_simd": sched.waves}},
         "solved": solved, "instances": count * args.steps,
         "iterations_median": float(np.median(its)),
         "iterations_p99": float(np.percentile(its, 99)),

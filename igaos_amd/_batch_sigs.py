@@ -26,8 +26,8 @@ SIGNATURES = {
     "igaos_batch_launch": (C.c_int, [_vp, _vp]),
     "igaos_batch_progress": (C.c_int, [_vp]),
     "igaos_batch_wait": (C.c_int, [_vp, _dp, _dp, C.POINTER(BatchResult)]),
-    "igaos_batch_set_schedule": (C.c_int, [_vp, _ip, C.c_int]),
-    "igaos_batch_get_schedule": (C.c_int, [_vp, _ip, _ip]),
+    "igaos_batch_set_schedule": (C.c_int, [_vp, _ip, C.c_int, C.c_int]),
+    "igaos_batch_get_schedule": (C.c_int, [_vp, _ip, _ip, _ip]),
     "igaos_batch_kernel_ms": (C.c_double, [_vp]),
     "igaos_batch_last_error": (C.c_char_p, []),
     "igaos_batch_destroy": (None, [_vp]),

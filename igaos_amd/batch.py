@@ -74,18 +74,20 @@ class BatchSolver:
         self._coff = np.concatenate([[0], np.cumsum(self.n)])
 
     def schedule(self):
+        """(lanes per row of each instance, persistent grid, waves/SIMD)."""
         tpr = np.zeros(self.count, np.int32)
-        g = C.c_int()
+        g, w = C.c_int(), C.c_int()
         _core.lib().igaos_batch_get_schedule(
-            self._h, tpr.ctypes.data_as(C.POINTER(C.c_int)), C.byref(g))
-        return tpr, g.value
+            self._h, tpr.ctypes.data_as(C.POINTER(C.c_int)), C.byref(g),
+            C.byref(w))
+        return tpr, g.value, w.value
 
-    def set_schedule(self, tpr=None, grid=0):
+    def set_schedule(self, tpr=None, grid=0, waves=0):
         L = _core.lib()
         t = None if tpr is None else np.ascontiguousarray(tpr, np.int32)
         rc = L.igaos_batch_set_schedule(
             self._h, None if t is None else t.ctypes.data_as(C.POINTER(C.c_int)),
-            int(grid))
+            int(grid), int(waves))
         if rc != 0:
             raise ValueError(L.igaos_batch_last_error().decode())
 

@@ -678,8 +678,12 @@ __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
     __syncthreads();
 }
 
-template <typename V>
-__global__ __launch_bounds__(NT) void k_batch(const Arrays<V>* A, Params P) {
+// W = waves per SIMD the register allocation must allow: 2 leaves the
+// compiler its natural allocation; 4 caps it at 128 VGPRs (some spills,
+// twice the resident workgroups).  The autotuner picks per batch.
+template <typename V, int W>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W, 8)))
+void k_batch(const Arrays<V>* A, Params P) {
     __shared__ Ctl C;
     __shared__ double red[NT / WAVE];
     unsigned int done = 0;
@@ -701,17 +705,26 @@ __global__ __launch_bounds__(NT) void k_batch(const Arrays<V>* A, Params P) {
     }
 }
 
-void launch_batch(const Arrays<float>* a, const Params& p, int grid, hipStream_t s) {
-    k_batch<float><<<grid, NT, 0, s>>>(a, p);
+void launch_batch(const Arrays<float>* a, const Params& p, int grid, int waves,
+                  hipStream_t s) {
+    if (waves >= 4) k_batch<float, 4><<<grid, NT, 0, s>>>(a, p);
+    else k_batch<float, 2><<<grid, NT, 0, s>>>(a, p);
 }
-void launch_batch(const Arrays<double>* a, const Params& p, int grid, hipStream_t s) {
-    k_batch<double><<<grid, NT, 0, s>>>(a, p);
+void launch_batch(const Arrays<double>* a, const Params& p, int grid,
+                  int waves, hipStream_t s) {
+    if (waves >= 4) k_batch<double, 4><<<grid, NT, 0, s>>>(a, p);
+    else k_batch<double, 2><<<grid, NT, 0, s>>>(a, p);
 }
 
-int batch_occupancy(bool f32) {
+int batch_occupancy(bool f32, int waves) {
     int nb = 0;
-    if (f32) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<float>, NT, 0);
-    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<double>, NT, 0);
+    if (f32) {
+        if (waves >= 4) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<float, 4>, NT, 0);
+        else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<float, 2>, NT, 0);
+    } else {
+        if (waves >= 4) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<double, 4>, NT, 0);
+        else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<double, 2>, NT, 0);
+    }
     return nb > 0 ? nb : 1;
 }
 

@@ -67,11 +67,12 @@ struct Params {
     unsigned int* progress;  // optional, pinned host memory: LPs finished per workgroup
 };
 
-// `a` points to a device copy of the pointer table.
+// `a` points to a device copy of the pointer table; `waves` (2 or 4) picks
+// the register-allocation variant of the persistent kernel.
 void launch_batch(const Arrays<float>* a, const Params& p, int grid,
-                  hipStream_t s);
+                  int waves, hipStream_t s);
 void launch_batch(const Arrays<double>* a, const Params& p, int grid,
-                  hipStream_t s);
-int batch_occupancy(bool f32);
+                  int waves, hipStream_t s);
+int batch_occupancy(bool f32, int waves);
 
 }  // namespace igaos::batch

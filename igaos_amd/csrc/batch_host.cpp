@@ -72,6 +72,8 @@ struct igaos_batch {
     bool running = false;
     double last_ms = 0.0;
     int grid = 0;
+    int waves = 2;
+    int cus = 1;
     ~igaos_batch() {
         if (running && e1) (void)hipEventSynchronize(e1);
         if (e0) (void)hipEventDestroy(e0);
@@ -243,8 +245,9 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
         B->d_res = B->mem.alloc<LPResult>(count);
         hipDeviceProp_t prop;
         BCK(hipGetDeviceProperties(&prop, 0));
-        const int occ = igaos::batch::batch_occupancy(precision == 32);
-        B->grid = std::min(count, prop.multiProcessorCount * occ);
+        B->cus = prop.multiProcessorCount;
+        const int occ = igaos::batch::batch_occupancy(precision == 32, B->waves);
+        B->grid = std::min(count, B->cus * occ);
         BCK(hipEventCreate(&B->e0));
         BCK(hipEventCreate(&B->e1));
         BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_prog),
@@ -258,9 +261,18 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
     }
 }
 
-int igaos_batch_set_schedule(igaos_batch* B, const int* tpr, int grid) {
+int igaos_batch_set_schedule(igaos_batch* B, const int* tpr, int grid,
+                             int waves) {
     if (!B) return bfail("null batch");
+    if (B->running) return bfail("batch is running");
     try {
+        if (waves == 2 || waves == 4) {
+            B->waves = waves;
+            const int occ = igaos::batch::batch_occupancy(B->precision == 32, waves);
+            B->grid = std::min(B->count, B->cus * occ);
+        } else if (waves != 0) {
+            return bfail("waves must be 0, 2 or 4");
+        }
         if (tpr) {
             for (int i = 0; i < B->count; ++i) {
                 const int t = tpr[i];
@@ -277,11 +289,13 @@ int igaos_batch_set_schedule(igaos_batch* B, const int* tpr, int grid) {
     return 0;
 }
 
-int igaos_batch_get_schedule(const igaos_batch* B, int* tpr, int* grid) {
+int igaos_batch_get_schedule(const igaos_batch* B, int* tpr, int* grid,
+                             int* waves) {
     if (!B) return bfail("null batch");
     if (tpr)
         for (int i = 0; i < B->count; ++i) tpr[i] = B->desc[i].tpr;
     if (grid) *grid = B->grid;
+    if (waves) *waves = B->waves;
     return 0;
 }
 
@@ -307,8 +321,8 @@ int igaos_batch_launch(igaos_batch* B, const igaos_options* opt) {
         std::memset(B->h_prog, 0, sizeof(unsigned int) * B->count);
         BCK(hipMemsetAsync(B->d_queue, 0, sizeof(int), B->s));
         BCK(hipEventRecord(B->e0, B->s));
-        if (B->precision == 32) igaos::batch::launch_batch(B->d_a32, P, B->grid, B->s);
-        else igaos::batch::launch_batch(B->d_a64, P, B->grid, B->s);
+        if (B->precision == 32) igaos::batch::launch_batch(B->d_a32, P, B->grid, B->waves, B->s);
+        else igaos::batch::launch_batch(B->d_a64, P, B->grid, B->waves, B->s);
         BCK(hipGetLastError());
         BCK(hipEventRecord(B->e1, B->s));
         B->running = true;

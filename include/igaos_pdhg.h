@@ -178,9 +178,14 @@ int igaos_batch_progress(const igaos_batch* batch);
 int igaos_batch_wait(igaos_batch* batch, double* x, double* y,
                      igaos_batch_result* res);
 /* Schedule: SpMV lanes per row of each instance (1, 2, 4 or 8; NULL keeps
- * the current ones) and the persistent grid size (<= 0 keeps it). */
-int igaos_batch_set_schedule(igaos_batch* batch, const int* tpr, int grid);
-int igaos_batch_get_schedule(const igaos_batch* batch, int* tpr, int* grid);
+ * the current ones), the persistent grid size (<= 0 keeps it) and the
+ * kernel variant by resident waves per SIMD (2 or 4; 0 keeps it; setting
+ * it resets the grid to the variant's full occupancy unless grid > 0
+ * follows in a later call). */
+int igaos_batch_set_schedule(igaos_batch* batch, const int* tpr, int grid,
+                             int waves);
+int igaos_batch_get_schedule(const igaos_batch* batch, int* tpr, int* grid,
+                             int* waves);
 /* Device time of the last run's persistent kernel (HIP events), ms. */
 double igaos_batch_kernel_ms(const igaos_batch* batch);
 const char* igaos_batch_last_error(void);

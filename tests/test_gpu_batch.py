@@ -87,8 +87,8 @@ def test_batch_general_rows_and_bounds(objectives):
 def test_batch_iteration_cap_and_schedule():
     lps = [lpgen.gen_batch_instance(2, i, m_hi=300) for i in range(5)]
     b = BatchSolver(lps, precision=32)
-    tpr, grid = b.schedule()
-    assert set(tpr) <= {1, 2, 4, 8} and 1 <= grid <= 5
+    tpr, grid, waves = b.schedule()
+    assert set(tpr) <= {1, 2, 4, 8} and 1 <= grid <= 5 and waves in (2, 4)
     s1 = b.run(tolerance=0.0, time_limit=1e7, max_iterations=100)
     assert all(s == "iteration-limit" for s in s1.status)
     assert list(s1.iterations) == [100] * 5
@@ -101,3 +101,18 @@ def test_batch_iteration_cap_and_schedule():
         assert s2.status[i] == s3.status[i] == "near-optimal"
         assert abs(s2.objective[i] - s3.objective[i]) <= 1e-6 * (1 + abs(s3.objective[i]))
     b.close()
+
+
+def test_autotuner_picks_a_valid_schedule(tmp_path):
+    from igaos_amd import autotune
+    lps = [lpgen.gen_batch_instance(4, i, m_hi=700) for i in range(24)]
+    cache = str(tmp_path / "tune.json")
+    sched = autotune.tune(lps, classes=2, per_class=6, iterations=64, cache=cache)
+    assert set(sched.tpr) <= set(autotune.TPR) and sched.waves in autotune.WAVES
+    again = autotune.tune(lps, classes=2, per_class=6, iterations=64, cache=cache)
+    assert again.tpr == sched.tpr and again.waves == sched.waves
+    b = BatchSolver(lps, precision=32)
+    autotune.apply(b, sched)
+    s = b.run(tolerance=1e-6, time_limit=60)
+    b.close()
+    assert s.solved == len(lps)
