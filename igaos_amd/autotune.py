@@ -1,8 +1,9 @@
 """Autotuner of the batched solver's schedule (no reference counterpart).
 
 Two choices are tuned:
-  * lanes per row (tpr in {1, 2, 4, 8}) of the SpMV phases, per size class
-    of instances (classes are quantile bins of nonzeros);
+  * the SpMV schedule of each size class of instances (classes are
+    quantile bins of nonzeros): 0 = row-block streaming through LDS, or
+    1, 2, 4 lanes per row;
   * the persistent kernel variant by resident waves per SIMD (2: natural
     register allocation; 4: 128-VGPR cap with some spills, twice the
     resident workgroups).
@@ -25,7 +26,7 @@ import numpy as np
 
 from .batch import BatchSolver
 
-TPR = (1, 2, 4, 8)
+TPR = (0, 1, 2, 4)
 WAVES = (2, 4)
 
 

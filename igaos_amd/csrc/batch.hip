@@ -32,6 +32,14 @@ __device__ __forceinline__ double bmax(double v, double* red) {
     return dev::block_max<double, NT>(v, red);
 }
 
+// LDS scratch of a workgroup: row-block products and row sums (as FP64 or
+// reinterpreted as FP32) and the reduction slots.
+struct Lds {
+    double prod[dev::NNZ_MAX];
+    double rows[dev::ROWS_MAX];
+    double red[NT / WAVE];
+};
+
 // Rows [0, nrows) of a CSR block: tpr lanes per row, f(row, sum) on the
 // row's first lane.  The outer loop trip count is uniform, so every lane
 // reaches the shuffles.
@@ -62,13 +70,15 @@ __device__ __forceinline__ void rows_apply(int nrows, const int* __restrict__ pt
 template <typename V>
 struct LP {
     const Arrays<V>* A;
-    int m, n, tpr;
-    long ro, co, zo, rpo, cpo;
+    int m, n, tpr, nblkA, nblkT;
+    long ro, co, zo, rpo, cpo, rbao, rbto;
     static constexpr bool F32 = sizeof(V) == 4;
     __device__ const int* ap() const { return A->ap + rpo; }
     __device__ const int* ai() const { return A->ai + zo; }
     __device__ const int* cp() const { return A->cp + cpo; }
     __device__ const int* ci() const { return A->ci + zo; }
+    __device__ const int* rbA() const { return A->rbA + rbao; }
+    __device__ const int* rbT() const { return A->rbT + rbto; }
     __device__ const double* ax() const { return A->ax + zo; }
     __device__ const double* acx() const { return A->acx + zo; }
     __device__ const double* c() const { return A->c + co; }
@@ -120,12 +130,50 @@ __device__ __forceinline__ LP<V> view(const Arrays<V>* A, const LPDesc& d) {
     L.m = d.m;
     L.n = d.n;
     L.tpr = d.tpr;
+    L.nblkA = d.nblkA;
+    L.nblkT = d.nblkT;
+    L.rbao = d.rbao;
+    L.rbto = d.rbto;
     L.ro = d.roff;
     L.co = d.coff;
     L.zo = d.zoff;
     L.rpo = d.rpo;
     L.cpo = d.cpo;
     return L;
+}
+
+// Row-block streaming over one instance matrix: each block of <= 2048
+// nonzeros is staged through LDS with 8 independent loads per thread
+// (dev::rowblock_sums), then f(row, sum) runs on consecutive threads.
+template <typename Acc, typename Val, typename G, typename F>
+__device__ __forceinline__ void blocks_apply(const int* rb, int nblk,
+                                             const int* ptr, const int* idx,
+                                             const Val* val, const G& gx,
+                                             Lds& S, F&& f) {
+    Acc* prod = reinterpret_cast<Acc*>(S.prod);
+    Acc* rows = reinterpret_cast<Acc*>(S.rows);
+    Acc* red = reinterpret_cast<Acc*>(S.red);
+    for (int b = 0; b < nblk; ++b) {
+        const int r0 = rb[2 * b], p0 = rb[2 * b + 1];
+        const int r1 = rb[2 * b + 2], p1 = rb[2 * b + 3];
+        dev::rowblock_sums<Acc>(ptr, idx, val, gx, r0, r1, p0, p1, rows, prod, red);
+        for (int t = threadIdx.x; t < r1 - r0; t += NT) f(r0 + t, rows[t]);
+        __syncthreads();
+    }
+}
+
+// y = A x (rows of A) and A'y (rows of A') with the instance's schedule.
+template <typename Acc, typename V, typename Val, typename G, typename F>
+__device__ __forceinline__ void spmv_A(const LP<V>& L, const Val* val,
+                                       const G& gx, Lds& S, F&& f) {
+    if (L.tpr == 0) blocks_apply<Acc>(L.rbA(), L.nblkA, L.ap(), L.ai(), val, gx, S, f);
+    else rows_apply<Acc>(L.m, L.ap(), L.ai(), val, gx, L.tpr, f);
+}
+template <typename Acc, typename V, typename Val, typename G, typename F>
+__device__ __forceinline__ void spmv_T(const LP<V>& L, const Val* val,
+                                       const G& gx, Lds& S, F&& f) {
+    if (L.tpr == 0) blocks_apply<Acc>(L.rbT(), L.nblkT, L.cp(), L.ci(), val, gx, S, f);
+    else rows_apply<Acc>(L.n, L.cp(), L.ci(), val, gx, L.tpr, f);
 }
 
 // Shared per-workgroup control block (thread 0 writes, all read).
@@ -147,12 +195,12 @@ __device__ __forceinline__ double full(const double* ref, const V* d, int j) {
 }
 
 template <typename V>
-__device__ void refresh_refs(const LP<V>& L, double* red) {
+__device__ void refresh_refs(const LP<V>& L, Lds& S) {
     // FP32: aref = A~ xref, atref = A~' yref (FP64), then the shifted data
     if (!L.xref()) return;
-    rows_apply<double>(L.m, L.ap(), L.ai(), L.sax(), dev::GatherPlain<double>{L.xref()}, L.tpr,
+    spmv_A<double>(L, L.sax(), dev::GatherPlain<double>{L.xref()}, S,
                        [&](int i, double s) { L.aref()[i] = s; });
-    rows_apply<double>(L.n, L.cp(), L.ci(), L.sacx(), dev::GatherPlain<double>{L.yref()}, L.tpr,
+    spmv_T<double>(L, L.sacx(), dev::GatherPlain<double>{L.yref()}, S,
                        [&](int j, double s) { L.atref()[j] = s; });
     __syncthreads();
     for (int j = threadIdx.x; j < L.n; j += NT) {
@@ -172,89 +220,63 @@ __device__ void refresh_refs(const LP<V>& L, double* red) {
 // (src/pdhg/pdhg.cu:173-254 through metrics_of, 694-733).
 template <typename V>
 __device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
-                           double cmax, double* red) {
+                           double cmax, Lds& S) {
     const double* xr = L.xref();
     const double* yr = L.yref();
     const double dz = 1e-8 * (1.0 + cmax);
     double dv = 0.0, op = 0.0, odc = 0.0, unbc = 0.0;
     double pv = 0.0, pa = 0.0, prow = 0.0, ymax = 0.0;
-    const int tpr = L.tpr;
-    const int groups = NT / tpr;
-    const int g = threadIdx.x / tpr;
-    const int lane = threadIdx.x & (tpr - 1);
     {  // column pass: d = c + A'y / dc, dinf, c'x and the column bound
-        const int* cp = L.cp();
-        const int* ci = L.ci();
-        const double* sacx = L.sacx();
         const double* dc = L.dc();
         const double* c = L.c();
         const double* clp = L.cl();
         const double* cup = L.cu();
-        const dev::GatherRef<V> gy{yr, yd};
-        for (int base = 0; base < L.n; base += groups) {
-            const int j = base + g;
-            double ata = 0.0;
-            if (j < L.n)
-                for (int p = cp[j] + lane; p < cp[j + 1]; p += tpr) ata += sacx[p] * gy(ci[p]);
-            for (int o = tpr / 2; o > 0; o >>= 1) ata += __shfl_xor(ata, o);
-            if (lane == 0 && j < L.n) {
-                const double hx = full(xr, xd, j) * dc[j];
-                const double hd = c[j] + ata / dc[j];
-                const double cl = clp[j], cu = cup[j];
-                const double bnd = fabs(__builtin_isfinite(cu) ? cu : cl);
-                const double tol = 1e-9 * (1.0 + bnd);
-                double v;
-                if (hx <= cl + tol && hd > 0) v = 0.0;
-                else if (hx >= cu - tol && hd < 0) v = 0.0;
-                else v = fabs(hd);
-                dv = fmax(dv, v);
-                op += c[j] * hx;
-                if (hd >= 0) {
-                    if (__builtin_isfinite(cl)) odc += hd * cl;
-                    else if (hd > dz) unbc = 1.0;
-                } else {
-                    if (__builtin_isfinite(cu)) odc += hd * cu;
-                    else if (-hd > dz) unbc = 1.0;
-                }
+        spmv_T<double>(L, L.sacx(), dev::GatherRef<V>{yr, yd}, S, [&](int j, double ata) {
+            const double hx = full(xr, xd, j) * dc[j];
+            const double hd = c[j] + ata / dc[j];
+            const double cl = clp[j], cu = cup[j];
+            const double bnd = fabs(__builtin_isfinite(cu) ? cu : cl);
+            const double tol = 1e-9 * (1.0 + bnd);
+            double v;
+            if (hx <= cl + tol && hd > 0) v = 0.0;
+            else if (hx >= cu - tol && hd < 0) v = 0.0;
+            else v = fabs(hd);
+            dv = fmax(dv, v);
+            op += c[j] * hx;
+            if (hd >= 0) {
+                if (__builtin_isfinite(cl)) odc += hd * cl;
+                else if (hd > dz) unbc = 1.0;
+            } else {
+                if (__builtin_isfinite(cu)) odc += hd * cu;
+                else if (-hd > dz) unbc = 1.0;
             }
-        }
+        });
     }
     {  // row pass: Ax = dr (A~x~), pinf / prow numerators, max|y|
-        const int* ap = L.ap();
-        const int* ai = L.ai();
-        const double* sax = L.sax();
         const double* dr = L.dr();
         const double* rminp = L.rmin();
         const double* rmaxp = L.rmax();
-        const dev::GatherRef<V> gx{xr, xd};
-        for (int base = 0; base < L.m; base += groups) {
-            const int i = base + g;
-            double tm = 0.0;
-            if (i < L.m)
-                for (int p = ap[i] + lane; p < ap[i + 1]; p += tpr) tm += sax[p] * gx(ai[p]);
-            for (int o = tpr / 2; o > 0; o >>= 1) tm += __shfl_xor(tm, o);
-            if (lane == 0 && i < L.m) {
-                const double ax = tm * dr[i];
-                const double lo = rminp[i], hi = rmaxp[i];
-                const double e = fmax(lo - ax, ax - hi);
-                pv = fmax(pv, e);
-                pa = fmax(pa, fabs(fmin(fmax(ax, lo), hi)));
-                double sc = fabs(ax);
-                if (__builtin_isfinite(lo)) sc = fmax(sc, fabs(lo));
-                if (__builtin_isfinite(hi)) sc = fmax(sc, fabs(hi));
-                prow = fmax(prow, fmax(0.0, e) / (1.0 + sc));
-                ymax = fmax(ymax, fabs(full(yr, yd, i) * (1.0 / dr[i])));
-            }
-        }
+        spmv_A<double>(L, L.sax(), dev::GatherRef<V>{xr, xd}, S, [&](int i, double tm) {
+            const double ax = tm * dr[i];
+            const double lo = rminp[i], hi = rmaxp[i];
+            const double e = fmax(lo - ax, ax - hi);
+            pv = fmax(pv, e);
+            pa = fmax(pa, fabs(fmin(fmax(ax, lo), hi)));
+            double sc = fabs(ax);
+            if (__builtin_isfinite(lo)) sc = fmax(sc, fabs(lo));
+            if (__builtin_isfinite(hi)) sc = fmax(sc, fabs(hi));
+            prow = fmax(prow, fmax(0.0, e) / (1.0 + sc));
+            ymax = fmax(ymax, fabs(full(yr, yd, i) * (1.0 / dr[i])));
+        });
     }
-    dv = bmax(dv, red);
-    op = bsum(op, red);
-    odc = bsum(odc, red);
-    unbc = bmax(unbc, red);
-    pv = bmax(pv, red);
-    pa = bmax(pa, red);
-    prow = bmax(prow, red);
-    ymax = bmax(ymax, red);
+    dv = bmax(dv, S.red);
+    op = bsum(op, S.red);
+    odc = bsum(odc, S.red);
+    unbc = bmax(unbc, S.red);
+    pv = bmax(pv, S.red);
+    pa = bmax(pa, S.red);
+    prow = bmax(prow, S.red);
+    ymax = bmax(ymax, S.red);
     const double yz = 1e-8 * (1.0 + ymax);
     double odr = 0.0, unbr = 0.0;
     for (int i = threadIdx.x; i < L.m; i += NT) {
@@ -267,8 +289,8 @@ __device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
             else unbr = 1.0;
         }
     }
-    odr = bsum(odr, red);
-    unbr = bmax(unbr, red);
+    odr = bsum(odr, S.red);
+    unbr = bmax(unbr, S.red);
     Metrics mt;
     mt.pinf = fmax(pv, 0.0) / (1.0 + pa);
     mt.prow = prow;
@@ -282,7 +304,7 @@ __device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
 }
 
 template <typename V>
-__device__ void setup(const LP<V>& L, Ctl& C, double* red) {
+__device__ void setup(const LP<V>& L, Ctl& C, Lds& S) {
     const int m = L.m, n = L.n, tid = threadIdx.x;
     // Ruiz, 10 sweeps (pdhg.cu:469-492); one thread per row / column, so
     // each factor is exactly the host loop's value.
@@ -346,22 +368,22 @@ __device__ void setup(const LP<V>& L, Ctl& C, double* red) {
     for (int j = tid; j < n; j += NT) v[j] = v0;
     __syncthreads();
     for (int it = 0; it < 30; ++it) {
-        rows_apply<double>(m, L.ap(), L.ai(), L.sax(), dev::GatherPlain<double>{v}, L.tpr,
+        spmv_A<double>(L, L.sax(), dev::GatherPlain<double>{v}, S,
                            [&](int i, double s) { t[i] = s; });
         __syncthreads();
         double q = 0.0;
-        rows_apply<double>(n, L.cp(), L.ci(), L.sacx(), dev::GatherPlain<double>{t}, L.tpr,
+        spmv_T<double>(L, L.sacx(), dev::GatherPlain<double>{t}, S,
                            [&](int j, double s) { w[j] = s; q += s * s; });
-        const double nw = sqrt(bsum(q, red));
+        const double nw = sqrt(bsum(q, S.red));
         if (nw < 1e-300) break;
         const double inv = 1.0 / nw;
         for (int j = tid; j < n; j += NT) v[j] = w[j] * inv;
         __syncthreads();
     }
     double q = 0.0;
-    rows_apply<double>(m, L.ap(), L.ai(), L.sax(), dev::GatherPlain<double>{v}, L.tpr,
+    spmv_A<double>(L, L.sax(), dev::GatherPlain<double>{v}, S,
                        [&](int i, double s) { q += s * s; });
-    double lam2 = bsum(q, red);
+    double lam2 = bsum(q, S.red);
     if (!(lam2 > 0) || !__builtin_isfinite(lam2)) lam2 = 1.0;
     // omega0 (pdhg.cu:647-663), max|c|
     double cn = 0.0, qn = 0.0, cm = 0.0;
@@ -374,9 +396,9 @@ __device__ void setup(const LP<V>& L, Ctl& C, double* red) {
                           : (__builtin_isfinite(L.srmin()[i]) ? L.srmin()[i] : 0.0);
         qn += qq * qq;
     }
-    cn = bsum(cn, red);
-    qn = bsum(qn, red);
-    cm = bmax(cm, red);
+    cn = bsum(cn, S.red);
+    qn = bsum(qn, S.red);
+    cm = bmax(cm, S.red);
     if (tid == 0) {
         C.lam2 = lam2;
         C.eta0 = 1.2 / sqrt(lam2);
@@ -395,17 +417,17 @@ __device__ void setup(const LP<V>& L, Ctl& C, double* red) {
         if (L.yref()) L.yref()[i] = 0.0;
     }
     __syncthreads();
-    refresh_refs(L, red);
+    refresh_refs(L, S);
 }
 
 template <typename V>
 __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
-                          Ctl& C, double* red, LPResult& R) {
+                          Ctl& C, Lds& S, LPResult& R) {
     const int m = L.m, n = L.n, tid = threadIdx.x;
     constexpr bool F32 = sizeof(V) == 4;
     const double INF = __builtin_huge_val();
     const double TOL = P.tol;
-    setup(L, C, red);
+    setup(L, C, S);
     if (tid == 0) {
         LsState& s = C.st;
         s = LsState{};
@@ -451,14 +473,14 @@ __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
                 pacc += d * d;
             };
             if (fresh) {
-                rows_apply<V>(n, L.cp(), L.ci(), L.facx(), dev::GatherPlain<V>{y}, L.tpr,
+                spmv_T<V>(L, L.facx(), dev::GatherPlain<V>{y}, S,
                               [&](int j, V a) { L.ata()[j] = a; primal(j, a); });
             } else {
                 for (int j = tid; j < n; j += NT) primal(j, L.ata()[j]);
             }
             __syncthreads();
             double a1 = 0.0, a2 = 0.0;
-            rows_apply<V>(m, L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.xbar()}, L.tpr,
+            spmv_A<V>(L, L.fax(), dev::GatherPlain<V>{L.xbar()}, S,
                           [&](int i, V tm) {
                 const V yo = y[i];
                 const V tt = yo + sig * tm;
@@ -472,9 +494,9 @@ __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
                 a1 += static_cast<double>(dy) * static_cast<double>(dy);
                 a2 += static_cast<double>(dy) * static_cast<double>(ad);
             });
-            pacc = bsum(pacc, red);
-            a1 = bsum(a1, red);
-            a2 = bsum(a2, red);
+            pacc = bsum(pacc, S.red);
+            a1 = bsum(a1, S.red);
+            a2 = bsum(a2, S.red);
             if (tid == 0) {
                 pdhg::ls_decide(C.st, pacc, a1, a2);
                 C.accepted = C.st.mix_pending;
@@ -501,8 +523,8 @@ __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
         }
         // ---- check (pdhg.cu:801-1084)
         const int cur = C.st.cur;
-        const Metrics mc = metrics(L, L.x(cur), L.y(cur), C.cmax, red);
-        const Metrics ma = metrics(L, L.xavg(), L.yavg(), C.cmax, red);
+        const Metrics mc = metrics(L, L.x(cur), L.y(cur), C.cmax, S);
+        const Metrics ma = metrics(L, L.xavg(), L.yavg(), C.cmax, S);
         if (tid == 0) {
             const double ec = mc.finite_res() ? mc.err() : INF;
             const double ea = ma.finite_res() ? ma.err() : INF;
@@ -573,12 +595,12 @@ __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
                     L.y(0)[i] = L.y(1)[i] = L.yavg()[i] = L.axcur()[i] = V(0);
                 }
                 __syncthreads();
-                refresh_refs(L, red);
+                refresh_refs(L, S);
             } else {
                 for (int j = tid; j < n; j += NT) L.x(cur)[j] = L.xavg()[j] = static_cast<V>(L.xb()[j]);
                 for (int i = tid; i < m; i += NT) L.y(cur)[i] = L.yavg()[i] = static_cast<V>(L.yb()[i]);
                 __syncthreads();
-                rows_apply<V>(m, L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.x(cur)}, L.tpr,
+                spmv_A<V>(L, L.fax(), dev::GatherPlain<V>{L.x(cur)}, S,
                               [&](int i, V s) { L.axcur()[i] = s; });
             }
             __syncthreads();
@@ -593,7 +615,7 @@ __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
                 for (int j = tid; j < n; j += NT) L.x(0)[j] = L.x(1)[j] = L.xavg()[j] = V(0);
                 for (int i = tid; i < m; i += NT) L.y(0)[i] = L.y(1)[i] = L.yavg()[i] = L.axcur()[i] = V(0);
                 __syncthreads();
-                refresh_refs(L, red);
+                refresh_refs(L, S);
             } else {
                 if (C.anchor_avg) {
                     for (int j = tid; j < n; j += NT) L.x(cur)[j] = L.xavg()[j];
@@ -603,7 +625,7 @@ __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
                     for (int i = tid; i < m; i += NT) L.yavg()[i] = L.y(cur)[i];
                 }
                 __syncthreads();
-                rows_apply<V>(m, L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.x(cur)}, L.tpr,
+                spmv_A<V>(L, L.fax(), dev::GatherPlain<V>{L.x(cur)}, S,
                               [&](int i, V s) { L.axcur()[i] = s; });
                 __syncthreads();
             }
@@ -621,8 +643,8 @@ __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
                 dyn += d * d;
                 L.pesy()[i] = v;
             }
-            dxn = bsum(dxn, red);
-            dyn = bsum(dyn, red);
+            dxn = bsum(dxn, S.red);
+            dyn = bsum(dyn, S.red);
             if (tid == 0) {
                 if (C.have_pe) C.st.omega = pdhg::primal_weight(C.st.omega, dxn, dyn);
                 C.have_pe = 1;
@@ -685,7 +707,7 @@ template <typename V, int W>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W, 8)))
 void k_batch(const Arrays<V>* A, Params P) {
     __shared__ Ctl C;
-    __shared__ double red[NT / WAVE];
+    __shared__ Lds S;
     unsigned int done = 0;
     for (;;) {
         if (threadIdx.x == 0) {
@@ -697,7 +719,7 @@ void k_batch(const Arrays<V>* A, Params P) {
         if (id < 0) break;
         const LPDesc D = P.lps[id];
         const LP<V> L = view(A, D);
-        solve_one(L, P, D, C, red, P.res[id]);
+        solve_one(L, P, D, C, S, P.res[id]);
         __syncthreads();
         if (threadIdx.x == 0 && P.progress)  // this workgroup's own slot
             __hip_atomic_store(P.progress + blockIdx.x, ++done, __ATOMIC_RELAXED,

@@ -19,7 +19,9 @@ constexpr int NT = 256;  // threads per workgroup (one LP at a time)
 
 struct LPDesc {
     int m, n, nnz;
-    int tpr;     // threads per row in the SpMV phases (1, 2, 4 or 8)
+    int tpr;     // SpMV schedule: 0 = row-block streaming, else lanes per row
+    int nblkA, nblkT;  // row blocks of A and A' (row-block streaming)
+    long rbao, rbto;   // offsets of the instance's block tables (ints)
     long roff;   // offset into m-sized arrays
     long coff;   // offset into n-sized arrays
     long zoff;   // offset into nnz-sized arrays
@@ -43,6 +45,7 @@ struct LPResult {
 template <typename V>
 struct Arrays {
     const int *ap, *ai, *cp, *ci;     // local indices, per-LP pointer arrays
+    const int *rbA, *rbT;             // per-LP row-block tables (pairs)
     const double *ax, *acx;           // original values
     const double *c, *cl, *cu, *rmin, *rmax;
     double *dr, *dc, *sax, *sacx, *sc, *scl, *scu, *srmin, *srmax;

@@ -14,6 +14,7 @@
 
 #include "batch.hpp"
 #include "igaos_pdhg.h"
+#include "pdhg_kernels.hpp"
 
 namespace {
 
@@ -56,6 +57,7 @@ struct igaos_batch {
     long sum_m = 0, sum_n = 0, sum_z = 0;
     std::vector<LPDesc> desc;
     std::vector<int> order;
+    std::vector<int> rbA, rbT;  // concatenated per-instance row-block tables
     Mem mem;
     LPDesc* d_desc = nullptr;
     int* d_order = nullptr;
@@ -92,17 +94,6 @@ int bfail(const std::string& w) {
     return -1;
 }
 
-// SpMV lanes per row from the LP's mean row length (A and A' rows).
-int default_tpr(int m, int n, int nnz) {
-    const double len = m > 0 ? static_cast<double>(nnz) / m : 0.0;
-    const double lenT = n > 0 ? static_cast<double>(nnz) / n : 0.0;
-    const double mean = 0.5 * (len + lenT);
-    if (mean <= 6.0) return 1;
-    if (mean <= 12.0) return 2;
-    if (mean <= 24.0) return 4;
-    return 8;
-}
-
 template <typename V>
 void fill_arrays(igaos_batch& B, igaos::batch::Arrays<V>& A, const int* ap,
                  const int* ai, const double* ax, const int* cp,
@@ -111,6 +102,8 @@ void fill_arrays(igaos_batch& B, igaos::batch::Arrays<V>& A, const int* ap,
                  const double* rmax) {
     Mem& M = B.mem;
     const long m = B.sum_m, n = B.sum_n, z = B.sum_z;
+    A.rbA = M.upload(B.rbA.data(), B.rbA.size());
+    A.rbT = M.upload(B.rbT.data(), B.rbT.size());
     A.ap = M.upload(ap, m + B.count);
     A.ai = M.upload(ai, z);
     A.ax = M.upload(ax, z);
@@ -205,7 +198,7 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
             d.m = m[i];
             d.n = n[i];
             d.nnz = nnz[i];
-            d.tpr = default_tpr(m[i], n[i], nnz[i]);
+            d.tpr = 0;  // row-block streaming; the autotuner may change it
             d.roff = ro;
             d.coff = co;
             d.zoff = zo;
@@ -215,6 +208,14 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
             if (ap[rpo + m[i]] != nnz[i] || cp[cpo + n[i]] != nnz[i])
                 throw std::invalid_argument("pointer arrays do not match nnz of instance " +
                                             std::to_string(i));
+            const std::vector<int> ba = igaos::dev::build_row_blocks(ap + rpo, m[i]);
+            const std::vector<int> bt = igaos::dev::build_row_blocks(cp + cpo, n[i]);
+            d.rbao = static_cast<long>(B->rbA.size());
+            d.rbto = static_cast<long>(B->rbT.size());
+            d.nblkA = static_cast<int>(ba.size() / 2) - 1;
+            d.nblkT = static_cast<int>(bt.size() / 2) - 1;
+            B->rbA.insert(B->rbA.end(), ba.begin(), ba.end());
+            B->rbT.insert(B->rbT.end(), bt.begin(), bt.end());
             ro += m[i];
             co += n[i];
             zo += nnz[i];
@@ -276,7 +277,8 @@ int igaos_batch_set_schedule(igaos_batch* B, const int* tpr, int grid,
         if (tpr) {
             for (int i = 0; i < B->count; ++i) {
                 const int t = tpr[i];
-                if (t != 1 && t != 2 && t != 4 && t != 8) return bfail("tpr must be 1, 2, 4 or 8");
+                if (t != 0 && t != 1 && t != 2 && t != 4 && t != 8)
+                    return bfail("schedule must be 0 (row blocks) or 1, 2, 4, 8 lanes per row");
                 B->desc[i].tpr = t;
             }
             BCK(hipMemcpy(B->d_desc, B->desc.data(), sizeof(LPDesc) * B->count,

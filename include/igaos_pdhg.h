@@ -177,8 +177,9 @@ int igaos_batch_launch(igaos_batch* batch, const igaos_options* opt);
 int igaos_batch_progress(const igaos_batch* batch);
 int igaos_batch_wait(igaos_batch* batch, double* x, double* y,
                      igaos_batch_result* res);
-/* Schedule: SpMV lanes per row of each instance (1, 2, 4 or 8; NULL keeps
- * the current ones), the persistent grid size (<= 0 keeps it) and the
+/* Schedule: SpMV mode of each instance (0 = row-block streaming through
+ * LDS, the default; 1, 2, 4 or 8 = lanes per row; NULL keeps the current
+ * ones), the persistent grid size (<= 0 keeps it) and the
  * kernel variant by resident waves per SIMD (2 or 4; 0 keeps it; setting
  * it resets the grid to the variant's full occupancy unless grid > 0
  * follows in a later call). */

@@ -88,14 +88,14 @@ def test_batch_iteration_cap_and_schedule():
     lps = [lpgen.gen_batch_instance(2, i, m_hi=300) for i in range(5)]
     b = BatchSolver(lps, precision=32)
     tpr, grid, waves = b.schedule()
-    assert set(tpr) <= {1, 2, 4, 8} and 1 <= grid <= 5 and waves in (2, 4)
+    assert set(tpr) <= {0, 1, 2, 4, 8} and 1 <= grid <= 5 and waves in (2, 4)
     s1 = b.run(tolerance=0.0, time_limit=1e7, max_iterations=100)
     assert all(s == "iteration-limit" for s in s1.status)
     assert list(s1.iterations) == [100] * 5
-    # another lanes-per-row schedule changes the summation order only
+    # another SpMV schedule changes the summation order only
     b.set_schedule(np.full(5, 4))
     s2 = b.run(tolerance=1e-6, time_limit=60)
-    b.set_schedule(np.full(5, 1))
+    b.set_schedule(np.full(5, 0))
     s3 = b.run(tolerance=1e-6, time_limit=60)
     for i in range(5):
         assert s2.status[i] == s3.status[i] == "near-optimal"
