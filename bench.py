@@ -274,6 +274,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=6)
     ap.add_argument("--cpu-time-limit", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device to device (a known-size "
+                         "kernel for scaling rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
@@ -284,6 +287,12 @@ def main():
     import igaos_amd
     if not (torch.cuda.is_available() and igaos_amd.has_pdhg):
         raise SystemExit("bench.py needs a HIP GPU")
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     line = bench_batch(args) if args.workload == "batch" else bench_zipf(args)
     print(json.dumps(line), flush=True)
 
