@@ -17,6 +17,12 @@ class BatchResult(C.Structure):
                 ("omega", C.c_double), ("lam2", C.c_double)]
 
 
+class AdjointInfo(C.Structure):
+    _fields_ = [("basic_cols", C.c_int), ("active_rows", C.c_int),
+                ("iters_p", C.c_int), ("iters_q", C.c_int),
+                ("res_p", C.c_double), ("res_q", C.c_double)]
+
+
 SIGNATURES = {
     "igaos_batch_create": (_vp, [C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp,
                                  _dp, _ip, _ip, _dp, _ip, _ip, _dp, _dp,
@@ -31,4 +37,8 @@ SIGNATURES = {
     "igaos_batch_kernel_ms": (C.c_double, [_vp]),
     "igaos_batch_last_error": (C.c_char_p, []),
     "igaos_batch_destroy": (None, [_vp]),
+    "igaos_batch_update": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "igaos_batch_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_int,
+                                      _dp, _dp, _dp, _dp, _dp, _dp,
+                                      C.POINTER(AdjointInfo)]),
 }

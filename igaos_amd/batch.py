@@ -72,6 +72,8 @@ class BatchSolver:
         self._h = C.c_void_p(h)
         self._roff = np.concatenate([[0], np.cumsum(self.m)])
         self._coff = np.concatenate([[0], np.cumsum(self.n)])
+        self._zoff = np.concatenate([[0], np.cumsum(self.nnz.astype(np.int64))])
+        self._lps = list(lps)
 
     def schedule(self):
         """(lanes per row of each instance, persistent grid, waves/SIMD)."""
@@ -134,6 +136,60 @@ class BatchSolver:
             sol.x = [x[self._coff[i]:self._coff[i + 1]] for i in range(self.count)]
             sol.y = [y[self._roff[i]:self._roff[i + 1]] for i in range(self.count)]
         return sol
+
+    def update(self, c=None, rmin=None, rmax=None, ax=None, cl=None, cu=None):
+        """Replace numeric data in place (concatenated over instances; ax in
+        CSR order, the CSC copy follows).  The next ``run`` re-scales."""
+        L = _core.lib()
+        if ax is not None and not hasattr(self, "_perm"):
+            self._perm = np.concatenate(
+                [lp.csc_perm() + z for lp, z in zip(self._lps, self._zoff[:-1])])
+        arr = {}
+        for key, v, size in (("c", c, self._coff[-1]), ("cl", cl, self._coff[-1]),
+                             ("cu", cu, self._coff[-1]), ("rmin", rmin, self._roff[-1]),
+                             ("rmax", rmax, self._roff[-1]), ("ax", ax, self._zoff[-1])):
+            if v is not None:
+                v = np.ascontiguousarray(v, dtype=np.float64)
+                if v.shape != (size,):
+                    raise ValueError(f"{key} must have shape ({size},), got {v.shape}")
+                arr[key] = v
+        if "ax" in arr:
+            arr["acx"] = np.ascontiguousarray(arr["ax"][self._perm])
+        dp = lambda k: arr[k].ctypes.data_as(C.POINTER(C.c_double)) if k in arr else None  # noqa
+        rc = L.igaos_batch_update(self._h, dp("c"), dp("cl"), dp("cu"), dp("rmin"),
+                                  dp("rmax"), dp("ax"), dp("acx"))
+        if rc != 0:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+
+    def adjoint(self, gx=None, gy=None, gobj=None, tolerance=1e-10,
+                max_iterations=None) -> dict:
+        """Gradients through the last run's solutions (include/igaos_pdhg.h,
+        igaos_batch_adjoint): returns gc, grmin, grmax, gcl, gcu, gax
+        (concatenated, gax in CSR order) and per-instance CGLS info."""
+        from ._batch_sigs import AdjointInfo
+        L = _core.lib()
+        n, m, z = int(self._coff[-1]), int(self._roff[-1]), int(self._zoff[-1])
+        if max_iterations is None:
+            max_iterations = int(min(200000, 20 * (self.m.max() + self.n.max())))
+        ins = {}
+        for key, v, size in (("gx", gx, n), ("gy", gy, m), ("gobj", gobj, self.count)):
+            if v is not None:
+                ins[key] = np.ascontiguousarray(np.broadcast_to(
+                    np.asarray(v, dtype=np.float64), (size,)))
+        out = dict(gc=np.zeros(n), grmin=np.zeros(m), grmax=np.zeros(m),
+                   gcl=np.zeros(n), gcu=np.zeros(n), gax=np.zeros(z))
+        info = (AdjointInfo * self.count)()
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+        rc = L.igaos_batch_adjoint(
+            self._h, *[dp(ins[k]) if k in ins else None for k in ("gx", "gy", "gobj")],
+            float(tolerance), int(max_iterations),
+            *[dp(out[k]) for k in ("gc", "grmin", "grmax", "gcl", "gcu", "gax")], info)
+        if rc != 0:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+        f = lambda k: np.array([getattr(r, k) for r in info])  # noqa
+        out["info"] = {k: f(k) for k in ("basic_cols", "active_rows", "iters_p",
+                                         "iters_q", "res_p", "res_q")}
+        return out
 
     def close(self):
         h = getattr(self, "_h", None)

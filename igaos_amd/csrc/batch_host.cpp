@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint.hpp"
 #include "batch.hpp"
 #include "igaos_pdhg.h"
 #include "pdhg_kernels.hpp"
@@ -76,6 +77,10 @@ struct igaos_batch {
     int grid = 0;
     int waves = 2;
     int cus = 1;
+    bool solved = false;  // a run finished and the data are unchanged since
+    Mem amem;             // adjoint workspace, allocated on first use
+    igaos::batch::AdjArrays adj{};
+    igaos::batch::AdjArrays* d_adj = nullptr;
     ~igaos_batch() {
         if (running && e1) (void)hipEventSynchronize(e1);
         if (e0) (void)hipEventDestroy(e0);
@@ -349,6 +354,7 @@ int igaos_batch_wait(igaos_batch* B, double* x, double* y,
     static_assert(sizeof(igaos_batch_result) == sizeof(LPResult), "result layout");
     try {
         B->running = false;
+        B->solved = false;
         BCK(hipEventSynchronize(B->e1));
         float ms = 0.0f;
         BCK(hipEventElapsedTime(&ms, B->e0, B->e1));
@@ -359,6 +365,7 @@ int igaos_batch_wait(igaos_batch* B, double* x, double* y,
         if (y) BCK(hipMemcpy(y, yo, sizeof(double) * B->sum_m, hipMemcpyDeviceToHost));
         if (res)
             BCK(hipMemcpy(res, B->d_res, sizeof(LPResult) * B->count, hipMemcpyDeviceToHost));
+        B->solved = true;
     } catch (const std::exception& e) {
         return bfail(e.what());
     }
@@ -373,6 +380,102 @@ int igaos_batch_run(igaos_batch* B, const igaos_options* opt, double* x,
 }
 
 double igaos_batch_kernel_ms(const igaos_batch* B) { return B ? B->last_ms : 0.0; }
+
+int igaos_batch_update(igaos_batch* B, const double* c, const double* cl,
+                       const double* cu, const double* rmin,
+                       const double* rmax, const double* ax,
+                       const double* acx) {
+    if (!B) return bfail("null batch");
+    if (B->running) return bfail("batch is running");
+    try {
+        auto put = [&](const double* src, const double* dst, long k) {
+            if (src && k)
+                BCK(hipMemcpy(const_cast<double*>(dst), src, sizeof(double) * k,
+                              hipMemcpyHostToDevice));
+        };
+        auto go = [&](const auto& A) {
+            put(c, A.c, B->sum_n);
+            put(cl, A.cl, B->sum_n);
+            put(cu, A.cu, B->sum_n);
+            put(rmin, A.rmin, B->sum_m);
+            put(rmax, A.rmax, B->sum_m);
+            put(ax, A.ax, B->sum_z);
+            put(acx, A.acx, B->sum_z);
+        };
+        if (B->precision == 32) go(B->a32);
+        else go(B->a64);
+        B->solved = false;
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
+
+int igaos_batch_adjoint(igaos_batch* B, const double* gx, const double* gy,
+                        const double* gobj, double tolerance,
+                        int max_iterations, double* gc, double* grmin,
+                        double* grmax, double* gcl, double* gcu, double* gax,
+                        igaos_adjoint_info* info) {
+    if (!B) return bfail("null batch");
+    if (B->running) return bfail("batch is running");
+    if (!B->solved) return bfail("adjoint needs a finished run on the current data");
+    if (!(tolerance >= 0.0) || max_iterations < 0)
+        return bfail("adjoint tolerance must be >= 0 and max_iterations >= 0");
+    static_assert(sizeof(igaos_adjoint_info) == sizeof(igaos::batch::AdjInfo),
+                  "adjoint info layout");
+    try {
+        const long m = B->sum_m, n = B->sum_n, z = B->sum_z;
+        igaos::batch::AdjArrays& J = B->adj;
+        if (!B->d_adj) {
+            Mem& M = B->amem;
+            J.mc = M.alloc<unsigned char>(n);
+            J.mr = M.alloc<unsigned char>(m);
+            for (double** p : {&J.z1, &J.s1, &J.p1, &J.r2, &J.t2}) *p = M.alloc<double>(n);
+            for (double** p : {&J.z2, &J.s2, &J.p2, &J.r1, &J.t1}) *p = M.alloc<double>(m);
+            J.gx = M.alloc<double>(n);
+            J.gy = M.alloc<double>(m);
+            J.gobj = M.alloc<double>(B->count);
+            for (double** p : {&J.gc, &J.gcl, &J.gcu}) *p = M.alloc<double>(n);
+            J.grmin = M.alloc<double>(m);
+            J.grmax = M.alloc<double>(m);
+            J.gax = M.alloc<double>(z);
+            J.info = M.alloc<igaos::batch::AdjInfo>(B->count);
+            B->d_adj = M.alloc<igaos::batch::AdjArrays>(1);
+        }
+        auto in = [&](const double* src, const double* dst, long k) {
+            double* d = const_cast<double*>(dst);
+            if (src) BCK(hipMemcpyAsync(d, src, sizeof(double) * k, hipMemcpyHostToDevice, B->s));
+            else BCK(hipMemsetAsync(d, 0, sizeof(double) * k, B->s));
+        };
+        in(gx, J.gx, n);
+        in(gy, J.gy, m);
+        in(gobj, J.gobj, B->count);
+        J.tol = tolerance;
+        J.maxit = max_iterations;
+        BCK(hipMemcpyAsync(B->d_adj, &J, sizeof(J), hipMemcpyHostToDevice, B->s));
+        if (B->precision == 32)
+            igaos::batch::launch_adjoint(B->d_a32, B->d_desc, B->count, B->d_adj, B->s);
+        else
+            igaos::batch::launch_adjoint(B->d_a64, B->d_desc, B->count, B->d_adj, B->s);
+        BCK(hipGetLastError());
+        BCK(hipStreamSynchronize(B->s));
+        auto out = [&](double* dst, const double* src, long k) {
+            if (dst) BCK(hipMemcpy(dst, src, sizeof(double) * k, hipMemcpyDeviceToHost));
+        };
+        out(gc, J.gc, n);
+        out(gcl, J.gcl, n);
+        out(gcu, J.gcu, n);
+        out(grmin, J.grmin, m);
+        out(grmax, J.grmax, m);
+        out(gax, J.gax, z);
+        if (info)
+            BCK(hipMemcpy(info, J.info, sizeof(igaos::batch::AdjInfo) * B->count,
+                          hipMemcpyDeviceToHost));
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
 
 void igaos_batch_destroy(igaos_batch* B) { delete B; }
 

@@ -80,6 +80,35 @@ class LP:
             acx=np.ascontiguousarray(vals[c_order]),
             obj_const=float(obj_const), name=name)
 
+    def csc_perm(self):
+        """perm with acx == ax[perm]: the CSR position of every CSC entry
+        (the pattern holds each (row, column) pair once)."""
+        rows = np.repeat(np.arange(self.m, dtype=np.int64), np.diff(self.ap))
+        cols = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(self.cp))
+        key_r = rows * self.n + self.ai
+        key_c = self.ci.astype(np.int64) * self.n + cols
+        o_r = np.argsort(key_r, kind="stable")
+        o_c = np.argsort(key_c, kind="stable")
+        perm = np.empty(self.nnz, dtype=np.int64)
+        perm[o_c] = o_r
+        return perm
+
+    def with_values(self, ax=None, c=None, rmin=None, rmax=None):
+        """Copy with new numeric data on the same pattern (ax in CSR
+        order; the CSC values follow)."""
+        import copy
+        q = copy.copy(self)
+        if ax is not None:
+            q.ax = np.ascontiguousarray(ax, dtype=np.float64)
+            q.acx = np.ascontiguousarray(q.ax[self.csc_perm()])
+        if c is not None:
+            q.c = np.ascontiguousarray(c, dtype=np.float64)
+        if rmin is not None:
+            q.rmin = np.ascontiguousarray(rmin, dtype=np.float64)
+        if rmax is not None:
+            q.rmax = np.ascontiguousarray(rmax, dtype=np.float64)
+        return q
+
     def matvec(self, x):
         """Row-sequential A @ x in FP64 (host helper for generators)."""
         x = np.asarray(x, dtype=np.float64)

@@ -189,6 +189,42 @@ int igaos_batch_get_schedule(const igaos_batch* batch, int* tpr, int* grid,
                              int* waves);
 /* Device time of the last run's persistent kernel (HIP events), ms. */
 double igaos_batch_kernel_ms(const igaos_batch* batch);
+/* Replaces the batch's numeric data in place (same sparsity pattern and
+ * sizes; host arrays in the layout of igaos_batch_create, NULL keeps an
+ * array).  ax and acx must hold the same values in CSR and CSC order.
+ * The next run re-scales from the new data. */
+int igaos_batch_update(igaos_batch* batch, const double* c, const double* cl,
+                       const double* cu, const double* rmin,
+                       const double* rmax, const double* ax,
+                       const double* acx);
+
+/* ---- adjoint of the batched solve (no reference counterpart) -----------
+ * Gradients of a scalar loss L(x*, y*, obj) through the solution map of
+ * every instance, at the solution of the last finished run.  With the
+ * active sets B (columns strictly inside their bounds) and R (equality
+ * rows and inequality rows on a bound) read from complementarity, it
+ * solves  A_RB p = gy_R  and  A_RB' q = (gx + gobj c)_B  per instance by
+ * CGLS in FP64 (the transposed fixed-point system of the PDHG map
+ * linearised at the solution) and returns, in original units:
+ *   gc    (sum n): dL/dc = gobj x - [B] p
+ *   grmin, grmax (sum m): dL/d(bound row i sits on) = q_i, on the side the
+ *         sign of y_i names (y_i >= 0: rmax), 0 on the other side
+ *   gcl, gcu (sum n): dL/d(bound column j sits on) for j not in B
+ *   gax   (sum nnz, CSR order): dL/dA_ij = -[R](p_j y_i + q_i x_j)
+ * gx (sum n), gy (sum m) and gobj (count) are the upstream gradients (NULL
+ * = zero); every output may be NULL.  CGLS stops when the normal-equation
+ * residual falls below `tolerance` relative to its start, or after
+ * max_iterations iterations. */
+typedef struct {
+    int basic_cols, active_rows;  /* |B|, |R| */
+    int iters_p, iters_q;         /* CGLS iterations of the two systems */
+    double res_p, res_q;          /* final relative residuals */
+} igaos_adjoint_info;
+int igaos_batch_adjoint(igaos_batch* batch, const double* gx, const double* gy,
+                        const double* gobj, double tolerance,
+                        int max_iterations, double* gc, double* grmin,
+                        double* grmax, double* gcl, double* gcu, double* gax,
+                        igaos_adjoint_info* info);
 const char* igaos_batch_last_error(void);
 void igaos_batch_destroy(igaos_batch* batch);
 
