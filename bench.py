@@ -74,15 +74,19 @@ def bench_batch(args):
     ns = np.array([lp.n for lp in lps])
     log(f"[bench] {count} LPs generated in {time.perf_counter() - t:.1f} s: "
         f"sum m={ms.sum()} n={ns.sum()} nnz={nnz.sum()}")
-    from igaos_amd import autotune
-    t = time.perf_counter()
-    sched = autotune.tune(lps, log=log)
-    log(f"[bench] autotuned in {time.perf_counter() - t:.1f} s: tpr by class "
-        f"{sched.tpr} (nnz edges {sched.edges}), waves/SIMD {sched.waves}")
-    b = BatchSolver(lps, precision=32)
-    autotune.apply(b, sched)
+    sched = None
+    if args.mode == "persistent":
+        from igaos_amd import autotune
+        t = time.perf_counter()
+        sched = autotune.tune(lps, log=log)
+        log(f"[bench] autotuned in {time.perf_counter() - t:.1f} s: tpr by class "
+            f"{sched.tpr} (nnz edges {sched.edges}), waves/SIMD {sched.waves}")
+    b = BatchSolver(lps, precision=32, mode=args.mode)
+    if sched is not None:
+        autotune.apply(b, sched)
     tpr, grid, waves = b.schedule()
-    log(f"[bench] batch resident on the GPU, persistent grid {grid}")
+    log(f"[bench] batch resident on the GPU, mode {args.mode}"
+        + (f", persistent grid {grid}" if args.mode == "persistent" else ""))
     t0 = [0.0]
 
     def prog(done, total):
@@ -161,10 +165,12 @@ def bench_batch(args):
                    "sum_n": int(ns.sum()), "sum_nnz": int(nnz.sum()),
                    "tolerance": args.tolerance,
                    "time_limit_per_instance_s": args.time_limit,
-                   "persistent_grid": int(grid),
-                   "schedule": {"tpr_by_class": sched.tpr,
-                                "nnz_class_edges": sched.edges,
-                                "waves_per_simd": sched.waves}},
+                   "mode": args.mode,
+                   "schedule": None if sched is None else {
+                       "persistent_grid": int(grid),
+                       "tpr_by_class": sched.tpr,
+                       "nnz_class_edges": sched.edges,
+                       "waves_per_simd": sched.waves}},
         "solved": solved, "instances": count * args.steps,
         "iterations_median": float(np.median(its)),
         "iterations_p99": float(np.percentile(its, 99)),
@@ -173,10 +179,15 @@ def bench_batch(args):
             "value": fp64_rate, "unit": "LPs/s", "sample": sample,
             "solved": solved64, "solve_seconds": round(t64, 3)},
         "batched_vs_plain_fp64": (value / fp64_rate) if fp64_rate > 0 else None,
-        "roofline": {"bound": "hbm", "kernel": "k_batch<float>",
+        "roofline": {"bound": "hbm",
+                     "kernel": ("k_batch<float> (one persistent launch)"
+                                if args.mode == "persistent" else
+                                "lockstep run: k_lk_primal/k_lk_dual/k_lk_decide "
+                                "rounds + k_lk_check, HIP events around the run"),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": load_traffic("k_batch_batch8192"),
+                     "traffic": load_traffic("k_batch_batch8192" if args.mode == "persistent"
+                                             else "lockstep_batch8192"),
                      "bytes_per_launch": alg,
                      "ms_per_launch": round(launch_ms, 3)},
         "cpu_baseline": cpu,
@@ -267,6 +278,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="batch", choices=["batch", "zipf"])
     ap.add_argument("--count", type=int, default=8192)
+    ap.add_argument("--mode", default="lockstep", choices=["lockstep", "persistent"],
+                    help="batched execution mode (igaos_batch_set_mode)")
     ap.add_argument("--tolerance", type=float, default=1e-6)
     ap.add_argument("--time-limit", type=float, default=60.0)
     ap.add_argument("--attempts", type=int, default=20)

@@ -37,6 +37,8 @@ SIGNATURES = {
     "igaos_batch_kernel_ms": (C.c_double, [_vp]),
     "igaos_batch_last_error": (C.c_char_p, []),
     "igaos_batch_destroy": (None, [_vp]),
+    "igaos_batch_set_mode": (C.c_int, [_vp, C.c_int]),
+    "igaos_batch_get_mode": (C.c_int, [_vp]),
     "igaos_batch_update": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "igaos_batch_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_int,
                                       _dp, _dp, _dp, _dp, _dp, _dp,

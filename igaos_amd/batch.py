@@ -45,7 +45,7 @@ def _cat(arrs, dtype):
 class BatchSolver:
     """Uploads a list of ``lpgen.LP`` once; ``run`` solves all of them."""
 
-    def __init__(self, lps, precision=32):
+    def __init__(self, lps, precision=32, mode="persistent"):
         L = _core.lib()
         self.count = len(lps)
         self.precision = precision
@@ -74,6 +74,19 @@ class BatchSolver:
         self._coff = np.concatenate([[0], np.cumsum(self.n)])
         self._zoff = np.concatenate([[0], np.cumsum(self.nnz.astype(np.int64))])
         self._lps = list(lps)
+        self.set_mode(mode)
+
+    MODES = ("persistent", "lockstep")
+
+    def set_mode(self, mode):
+        """'persistent': one launch, a workgroup per instance at a time;
+        'lockstep': all active instances advance one attempt per round."""
+        if mode not in self.MODES:
+            raise ValueError(f"mode must be one of {self.MODES}, got {mode!r}")
+        L = _core.lib()
+        if L.igaos_batch_set_mode(self._h, self.MODES.index(mode)) != 0:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+        self.mode = mode
 
     def schedule(self):
         """(lanes per row of each instance, persistent grid, waves/SIMD)."""
@@ -202,9 +215,9 @@ class BatchSolver:
         self.close()
 
 
-def solve_batch(lps, precision=32, **kw) -> BatchSolution:
+def solve_batch(lps, precision=32, mode="persistent", **kw) -> BatchSolution:
     """One-shot batched solve of a list of LPs."""
-    b = BatchSolver(lps, precision=precision)
+    b = BatchSolver(lps, precision=precision, mode=mode)
     try:
         return b.run(**kw)
     finally:

@@ -31,7 +31,7 @@ struct Ctl {
     double eta0, omega0, inc_op, cmax, lam2;
     long long last_rollback, restarts;
     int lp, have_inc, have_pe, last_is_avg, stop, time_limit;
-    int inc_src, rollback, restart, anchor_avg, accepted;
+    int inc_src, rollback, restart, anchor_avg, accepted, done;
     unsigned long long t0;
 };
 
@@ -267,35 +267,247 @@ __device__ void setup(const LP<V>& L, Ctl& C, Lds& S) {
     refresh_refs(L, S);
 }
 
+// Control state of a fresh solve (thread 0).  no_freeze = 1 lets the
+// persistent kernel run the check itself every 64 accepted iterations; the
+// lockstep mode freezes the state there instead (pdhg_ctrl.hpp).
+__device__ inline void init_ctl(Ctl& C, const Params& P, int no_freeze) {
+    const double INF = __builtin_huge_val();
+    LsState& s = C.st;
+    s = LsState{};
+    s.eta = C.eta0;
+    s.omega = C.omega0;
+    s.k = 1;
+    s.maxit = P.maxit;
+    s.fresh = 1;
+    s.no_freeze = no_freeze;
+    C.rs = pdhg::RestartState{-1.0, INF, 0};
+    C.last = Metrics{0, 0, 0, 0, 0, 0};
+    C.inc = Metrics{0, 0, 0, 0, 0, 0};
+    C.last_is_avg = 0;
+    C.have_inc = 0;
+    C.have_pe = 0;
+    C.inc_op = 0.0;
+    C.last_rollback = 0;
+    C.restarts = 0;
+    C.stop = P.maxit < 1;
+    C.time_limit = 0;
+    C.done = 0;
+    C.t0 = wall_clock64();
+}
+
+// Averaging and A x commit of the last accepted step (pdhg.cu:795-799).
 template <typename V>
-__device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
-                          Ctl& C, Lds& S, LPResult& R) {
+__device__ inline void apply_mix(const LP<V>& L, const Ctl& C) {
+    const int cur = C.st.cur;
+    const V w = static_cast<V>(C.st.mix_w);
+    for (int j = threadIdx.x; j < L.n; j += NT) L.xavg()[j] += w * (L.x(cur)[j] - L.xavg()[j]);
+    for (int i = threadIdx.x; i < L.m; i += NT) {
+        L.yavg()[i] += w * (L.y(cur)[i] - L.yavg()[i]);
+        L.axcur()[i] += L.adx()[i];
+    }
+}
+
+// The check after `kdone` accepted iterations, a multiple of 64
+// (src/pdhg/pdhg.cu:801-1084): metrics of both iterates, termination,
+// incumbent, rollback, restart (FP32: refinement of the reference point),
+// primal weight, time limit.  Returns true when the solve stops.
+template <typename V>
+__device__ bool check_lp(const LP<V>& L, const Params& P, Ctl& C, Lds& S,
+                         long long kdone) {
     const int m = L.m, n = L.n, tid = threadIdx.x;
     constexpr bool F32 = sizeof(V) == 4;
     const double INF = __builtin_huge_val();
     const double TOL = P.tol;
-    setup(L, C, S);
+    const int cur = C.st.cur;
+    const Metrics mc = metrics(L, L.x(cur), L.y(cur), C.cmax, S);
+    const Metrics ma = metrics(L, L.xavg(), L.yavg(), C.cmax, S);
     if (tid == 0) {
-        LsState& s = C.st;
-        s = LsState{};
-        s.eta = C.eta0;
-        s.omega = C.omega0;
-        s.k = 1;
-        s.maxit = P.maxit;
-        s.no_freeze = 1;
-        C.rs = pdhg::RestartState{-1.0, INF, 0};
-        C.last = Metrics{0, 0, 0, 0, 0, 0};
-        C.inc = Metrics{0, 0, 0, 0, 0, 0};
-        C.last_is_avg = 0;
-        C.have_inc = 0;
-        C.have_pe = 0;
-        C.inc_op = 0.0;
-        C.last_rollback = 0;
-        C.restarts = 0;
-        C.stop = P.maxit < 1;
-        C.time_limit = 0;
-        C.t0 = wall_clock64();
+        const double ec = mc.finite_res() ? mc.err() : INF;
+        const double ea = ma.finite_res() ? ma.err() : INF;
+        C.last = (ec <= ea) ? mc : ma;
+        C.last_is_avg = (ea < ec);
+        C.inc_src = -1;
+        C.rollback = 0;
+        C.restart = 0;
+        if (pdhg::passes(mc, TOL)) {
+            C.last = mc;
+            C.last_is_avg = 0;
+            C.stop = 1;
+        } else if (pdhg::passes(ma, TOL)) {
+            C.last = ma;
+            C.last_is_avg = 1;
+            C.stop = 1;
+        } else {
+            for (int cc = 0; cc < 2; ++cc) {
+                const Metrics& t = cc ? ma : mc;
+                if (t.pinf <= 1e-4 && __builtin_isfinite(t.op) &&
+                    (!C.have_inc || t.op < C.inc_op)) {
+                    C.have_inc = 1;
+                    C.inc_op = t.op;
+                    C.inc = t;
+                    C.inc_src = cc;
+                }
+            }
+            if (mc.pinf >= 0.5 && C.have_inc && kdone - C.last_rollback > 2000) {
+                C.rollback = 1;
+                C.st.eta = fmin(C.st.eta, 0.5 * C.eta0);
+                C.st.hn = 0;
+                C.rs.epoch_start = kdone;
+                C.rs.epoch_kkt = -1.0;
+                C.rs.prev_cand = INF;
+                C.last_rollback = kdone;
+            }
+            const double cand = fmin(ec, ea);
+            if (pdhg::restart_decide(C.rs, cand, kdone)) {
+                C.restart = 1;
+                C.anchor_avg = ea < ec;
+                C.restarts += 1;
+                C.st.hn = 0;
+                C.rs.epoch_start = kdone;
+                C.rs.epoch_kkt = cand;
+                C.rs.prev_cand = INF;
+            }
+            const double el = static_cast<double>(wall_clock64() - C.t0) / P.ticks_per_s;
+            if (el > P.time_limit_s) C.time_limit = 1;
+        }
     }
+    __syncthreads();
+    if (C.stop) return true;  // a pair passed the termination test
+    if (C.inc_src >= 0) {
+        const V* sx = C.inc_src ? L.xavg() : L.x(cur);
+        const V* sy = C.inc_src ? L.yavg() : L.y(cur);
+        for (int j = tid; j < n; j += NT) L.xb()[j] = full(L.xref(), sx, j);
+        for (int i = tid; i < m; i += NT) L.yb()[i] = full(L.yref(), sy, i);
+    }
+    __syncthreads();
+    if (C.rollback) {
+        if constexpr (F32) {
+            for (int j = tid; j < n; j += NT) {
+                L.xref()[j] = L.xb()[j];
+                L.x(0)[j] = L.x(1)[j] = L.xavg()[j] = V(0);
+            }
+            for (int i = tid; i < m; i += NT) {
+                L.yref()[i] = L.yb()[i];
+                L.y(0)[i] = L.y(1)[i] = L.yavg()[i] = L.axcur()[i] = V(0);
+            }
+            __syncthreads();
+            refresh_refs(L, S);
+        } else {
+            for (int j = tid; j < n; j += NT) L.x(cur)[j] = L.xavg()[j] = static_cast<V>(L.xb()[j]);
+            for (int i = tid; i < m; i += NT) L.y(cur)[i] = L.yavg()[i] = static_cast<V>(L.yb()[i]);
+            __syncthreads();
+            spmv_A<V>(L, L.fax(), dev::GatherPlain<V>{L.x(cur)}, S,
+                          [&](int i, V s) { L.axcur()[i] = s; });
+        }
+        __syncthreads();
+    }
+    if (C.restart) {
+        if constexpr (F32) {
+            const V* ax = C.anchor_avg ? L.xavg() : L.x(cur);
+            const V* ay = C.anchor_avg ? L.yavg() : L.y(cur);
+            for (int j = tid; j < n; j += NT) L.xref()[j] += static_cast<double>(ax[j]);
+            for (int i = tid; i < m; i += NT) L.yref()[i] += static_cast<double>(ay[i]);
+            __syncthreads();
+            for (int j = tid; j < n; j += NT) L.x(0)[j] = L.x(1)[j] = L.xavg()[j] = V(0);
+            for (int i = tid; i < m; i += NT) L.y(0)[i] = L.y(1)[i] = L.yavg()[i] = L.axcur()[i] = V(0);
+            __syncthreads();
+            refresh_refs(L, S);
+        } else {
+            if (C.anchor_avg) {
+                for (int j = tid; j < n; j += NT) L.x(cur)[j] = L.xavg()[j];
+                for (int i = tid; i < m; i += NT) L.y(cur)[i] = L.yavg()[i];
+            } else {
+                for (int j = tid; j < n; j += NT) L.xavg()[j] = L.x(cur)[j];
+                for (int i = tid; i < m; i += NT) L.yavg()[i] = L.y(cur)[i];
+            }
+            __syncthreads();
+            spmv_A<V>(L, L.fax(), dev::GatherPlain<V>{L.x(cur)}, S,
+                          [&](int i, V s) { L.axcur()[i] = s; });
+            __syncthreads();
+        }
+        // primal weight on the epoch-start points (scaled units)
+        double dxn = 0.0, dyn = 0.0;
+        for (int j = tid; j < n; j += NT) {
+            const double v = F32 ? L.xref()[j] : static_cast<double>(L.x(cur)[j]);
+            const double d = v - L.pesx()[j];
+            dxn += d * d;
+            L.pesx()[j] = v;
+        }
+        for (int i = tid; i < m; i += NT) {
+            const double v = F32 ? L.yref()[i] : static_cast<double>(L.y(cur)[i]);
+            const double d = v - L.pesy()[i];
+            dyn += d * d;
+            L.pesy()[i] = v;
+        }
+        dxn = bsum(dxn, S.red);
+        dyn = bsum(dyn, S.red);
+        if (tid == 0) {
+            if (C.have_pe) C.st.omega = pdhg::primal_weight(C.st.omega, dxn, dyn);
+            C.have_pe = 1;
+        }
+        __syncthreads();
+    }
+    if (C.time_limit) return true;  // checked last, as in pdhg.cu:1077-1083
+    return kdone >= P.maxit;
+}
+
+// Exit (src/pdhg/pdhg.cu:1090-1141): status, incumbent fallback, result
+// record and the solution in original units.
+template <typename V>
+__device__ void finish_lp(const LP<V>& L, const Params& P, const LPDesc& D,
+                          Ctl& C, Lds& S, LPResult& R, long long kdone) {
+    const int m = L.m, n = L.n, tid = threadIdx.x;
+    const double TOL = P.tol;
+    __syncthreads();
+    if (tid == 0) {
+        Metrics last = C.last;
+        int st;
+        if (C.time_limit) st = 6;
+        else st = pdhg::certified(last, TOL) ? 1 : 5;
+        int src = C.last_is_avg ? 1 : 0;
+        if (!pdhg::certified(last, TOL) && C.have_inc && C.inc.pinf <= TOL &&
+            __builtin_isfinite(C.inc.op) && (C.inc.op < last.op || last.pinf > TOL)) {
+            last = C.inc;
+            src = 2;
+        }
+        R.status = st;
+        R.last_is_avg = src == 1;
+        R.objective = last.op + D.obj_const;
+        R.pinf = last.pinf;
+        R.dinf = last.dinf;
+        R.gap = last.gap;
+        R.prow = last.prow;
+        R.iterations = kdone;
+        R.attempts = C.st.attempts;
+        R.restarts = C.restarts;
+        R.solve_ms = static_cast<double>(wall_clock64() - C.t0) / P.ticks_per_s * 1e3;
+        R.eta = C.st.eta;
+        R.omega = C.st.omega;
+        R.lam2 = C.lam2;
+        C.inc_src = src;
+    }
+    __syncthreads();
+    const int src = C.inc_src;
+    const int cur = C.st.cur;
+    for (int j = tid; j < n; j += NT) {
+        const double sx = src == 2 ? L.xb()[j]
+                          : full(L.xref(), src == 1 ? L.xavg() : L.x(cur), j);
+        L.xout()[j] = sx * L.dc()[j];
+    }
+    for (int i = tid; i < m; i += NT) {
+        const double sy = src == 2 ? L.yb()[i]
+                          : full(L.yref(), src == 1 ? L.yavg() : L.y(cur), i);
+        L.yout()[i] = 1.0 * sy / L.dr()[i];
+    }
+    __syncthreads();
+}
+
+template <typename V>
+__device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
+                          Ctl& C, Lds& S, LPResult& R) {
+    const int m = L.m, n = L.n, tid = threadIdx.x;
+    setup(L, C, S);
+    if (tid == 0) init_ctl(C, P, 1);
     __syncthreads();
     long long kdone = 0;
     while (!C.stop) {
@@ -352,199 +564,16 @@ __device__ void solve_one(const LP<V>& L, const Params& P, const LPDesc& D,
             if (C.accepted) break;
             fresh = false;
         }
-        // averaging + A x commit of the accepted step (pdhg.cu:795-799)
-        {
-            const int cur = C.st.cur;
-            const V w = static_cast<V>(C.st.mix_w);
-            for (int j = tid; j < n; j += NT) L.xavg()[j] += w * (L.x(cur)[j] - L.xavg()[j]);
-            for (int i = tid; i < m; i += NT) {
-                L.yavg()[i] += w * (L.y(cur)[i] - L.yavg()[i]);
-                L.axcur()[i] += L.adx()[i];
-            }
-            kdone = C.st.k - 1;
-        }
+        apply_mix(L, C);
+        kdone = C.st.k - 1;
         __syncthreads();
         if (kdone % pdhg::CHECK_EVERY != 0) {
             if (kdone >= P.maxit) break;
             continue;
         }
-        // ---- check (pdhg.cu:801-1084)
-        const int cur = C.st.cur;
-        const Metrics mc = metrics(L, L.x(cur), L.y(cur), C.cmax, S);
-        const Metrics ma = metrics(L, L.xavg(), L.yavg(), C.cmax, S);
-        if (tid == 0) {
-            const double ec = mc.finite_res() ? mc.err() : INF;
-            const double ea = ma.finite_res() ? ma.err() : INF;
-            C.last = (ec <= ea) ? mc : ma;
-            C.last_is_avg = (ea < ec);
-            C.inc_src = -1;
-            C.rollback = 0;
-            C.restart = 0;
-            if (pdhg::passes(mc, TOL)) {
-                C.last = mc;
-                C.last_is_avg = 0;
-                C.stop = 1;
-            } else if (pdhg::passes(ma, TOL)) {
-                C.last = ma;
-                C.last_is_avg = 1;
-                C.stop = 1;
-            } else {
-                for (int cc = 0; cc < 2; ++cc) {
-                    const Metrics& t = cc ? ma : mc;
-                    if (t.pinf <= 1e-4 && __builtin_isfinite(t.op) &&
-                        (!C.have_inc || t.op < C.inc_op)) {
-                        C.have_inc = 1;
-                        C.inc_op = t.op;
-                        C.inc = t;
-                        C.inc_src = cc;
-                    }
-                }
-                if (mc.pinf >= 0.5 && C.have_inc && kdone - C.last_rollback > 2000) {
-                    C.rollback = 1;
-                    C.st.eta = fmin(C.st.eta, 0.5 * C.eta0);
-                    C.st.hn = 0;
-                    C.rs.epoch_start = kdone;
-                    C.rs.epoch_kkt = -1.0;
-                    C.rs.prev_cand = INF;
-                    C.last_rollback = kdone;
-                }
-                const double cand = fmin(ec, ea);
-                if (pdhg::restart_decide(C.rs, cand, kdone)) {
-                    C.restart = 1;
-                    C.anchor_avg = ea < ec;
-                    C.restarts += 1;
-                    C.st.hn = 0;
-                    C.rs.epoch_start = kdone;
-                    C.rs.epoch_kkt = cand;
-                    C.rs.prev_cand = INF;
-                }
-                const double el = static_cast<double>(wall_clock64() - C.t0) / P.ticks_per_s;
-                if (el > P.time_limit_s) C.time_limit = 1;
-            }
-        }
-        __syncthreads();
-        if (C.stop) break;  // a pair passed the termination test
-        if (C.inc_src >= 0) {
-            const V* sx = C.inc_src ? L.xavg() : L.x(cur);
-            const V* sy = C.inc_src ? L.yavg() : L.y(cur);
-            for (int j = tid; j < n; j += NT) L.xb()[j] = full(L.xref(), sx, j);
-            for (int i = tid; i < m; i += NT) L.yb()[i] = full(L.yref(), sy, i);
-        }
-        __syncthreads();
-        if (C.rollback) {
-            if constexpr (F32) {
-                for (int j = tid; j < n; j += NT) {
-                    L.xref()[j] = L.xb()[j];
-                    L.x(0)[j] = L.x(1)[j] = L.xavg()[j] = V(0);
-                }
-                for (int i = tid; i < m; i += NT) {
-                    L.yref()[i] = L.yb()[i];
-                    L.y(0)[i] = L.y(1)[i] = L.yavg()[i] = L.axcur()[i] = V(0);
-                }
-                __syncthreads();
-                refresh_refs(L, S);
-            } else {
-                for (int j = tid; j < n; j += NT) L.x(cur)[j] = L.xavg()[j] = static_cast<V>(L.xb()[j]);
-                for (int i = tid; i < m; i += NT) L.y(cur)[i] = L.yavg()[i] = static_cast<V>(L.yb()[i]);
-                __syncthreads();
-                spmv_A<V>(L, L.fax(), dev::GatherPlain<V>{L.x(cur)}, S,
-                              [&](int i, V s) { L.axcur()[i] = s; });
-            }
-            __syncthreads();
-        }
-        if (C.restart) {
-            if constexpr (F32) {
-                const V* ax = C.anchor_avg ? L.xavg() : L.x(cur);
-                const V* ay = C.anchor_avg ? L.yavg() : L.y(cur);
-                for (int j = tid; j < n; j += NT) L.xref()[j] += static_cast<double>(ax[j]);
-                for (int i = tid; i < m; i += NT) L.yref()[i] += static_cast<double>(ay[i]);
-                __syncthreads();
-                for (int j = tid; j < n; j += NT) L.x(0)[j] = L.x(1)[j] = L.xavg()[j] = V(0);
-                for (int i = tid; i < m; i += NT) L.y(0)[i] = L.y(1)[i] = L.yavg()[i] = L.axcur()[i] = V(0);
-                __syncthreads();
-                refresh_refs(L, S);
-            } else {
-                if (C.anchor_avg) {
-                    for (int j = tid; j < n; j += NT) L.x(cur)[j] = L.xavg()[j];
-                    for (int i = tid; i < m; i += NT) L.y(cur)[i] = L.yavg()[i];
-                } else {
-                    for (int j = tid; j < n; j += NT) L.xavg()[j] = L.x(cur)[j];
-                    for (int i = tid; i < m; i += NT) L.yavg()[i] = L.y(cur)[i];
-                }
-                __syncthreads();
-                spmv_A<V>(L, L.fax(), dev::GatherPlain<V>{L.x(cur)}, S,
-                              [&](int i, V s) { L.axcur()[i] = s; });
-                __syncthreads();
-            }
-            // primal weight on the epoch-start points (scaled units)
-            double dxn = 0.0, dyn = 0.0;
-            for (int j = tid; j < n; j += NT) {
-                const double v = F32 ? L.xref()[j] : static_cast<double>(L.x(cur)[j]);
-                const double d = v - L.pesx()[j];
-                dxn += d * d;
-                L.pesx()[j] = v;
-            }
-            for (int i = tid; i < m; i += NT) {
-                const double v = F32 ? L.yref()[i] : static_cast<double>(L.y(cur)[i]);
-                const double d = v - L.pesy()[i];
-                dyn += d * d;
-                L.pesy()[i] = v;
-            }
-            dxn = bsum(dxn, S.red);
-            dyn = bsum(dyn, S.red);
-            if (tid == 0) {
-                if (C.have_pe) C.st.omega = pdhg::primal_weight(C.st.omega, dxn, dyn);
-                C.have_pe = 1;
-            }
-            __syncthreads();
-        }
-        if (C.time_limit) break;  // checked last, as in pdhg.cu:1077-1083
-        if (kdone >= P.maxit) break;
+        if (check_lp(L, P, C, S, kdone)) break;
     }
-    // ---- exit (pdhg.cu:1090-1141)
-    __syncthreads();
-    int src;
-    if (tid == 0) {
-        Metrics last = C.last;
-        int st;
-        if (C.time_limit) st = 6;
-        else st = pdhg::certified(last, TOL) ? 1 : 5;
-        src = C.last_is_avg ? 1 : 0;
-        if (!pdhg::certified(last, TOL) && C.have_inc && C.inc.pinf <= TOL &&
-            __builtin_isfinite(C.inc.op) && (C.inc.op < last.op || last.pinf > TOL)) {
-            last = C.inc;
-            src = 2;
-        }
-        R.status = st;
-        R.last_is_avg = src == 1;
-        R.objective = last.op + D.obj_const;
-        R.pinf = last.pinf;
-        R.dinf = last.dinf;
-        R.gap = last.gap;
-        R.prow = last.prow;
-        R.iterations = kdone;
-        R.attempts = C.st.attempts;
-        R.restarts = C.restarts;
-        R.solve_ms = static_cast<double>(wall_clock64() - C.t0) / P.ticks_per_s * 1e3;
-        R.eta = C.st.eta;
-        R.omega = C.st.omega;
-        R.lam2 = C.lam2;
-        C.inc_src = src;
-    }
-    __syncthreads();
-    src = C.inc_src;
-    const int cur = C.st.cur;
-    for (int j = tid; j < n; j += NT) {
-        const double sx = src == 2 ? L.xb()[j]
-                          : full(L.xref(), src == 1 ? L.xavg() : L.x(cur), j);
-        L.xout()[j] = sx * L.dc()[j];
-    }
-    for (int i = tid; i < m; i += NT) {
-        const double sy = src == 2 ? L.yb()[i]
-                          : full(L.yref(), src == 1 ? L.yavg() : L.y(cur), i);
-        L.yout()[i] = 1.0 * sy / L.dr()[i];
-    }
-    __syncthreads();
+    finish_lp(L, P, D, C, S, R, kdone);
 }
 
 // W = waves per SIMD the register allocation must allow: 2 leaves the
@@ -595,6 +624,261 @@ int batch_occupancy(bool f32, int waves) {
         else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<double, 2>, NT, 0);
     }
     return nb > 0 ? nb : 1;
+}
+
+// ---------------------------------------------------------------------------
+// Lockstep mode.  Every active instance advances by one line-search attempt
+// per round; a round is three launches over the row blocks of all active
+// instances (tables of (instance, local block) pairs), so thousands of
+// small SpMVs run as a few wide, low-register kernels that keep HBM busy,
+// instead of one workgroup walking its instance's blocks one after another.
+// The attempt kernels follow the single-LP engine's k_primal / k_dual
+// (pending averaging folded into the next fresh attempt) and each
+// instance's LsState freezes on its 64-iteration check boundary; once every
+// active instance is frozen, one workgroup per instance runs the check with
+// the persistent kernel's own code (check_lp / finish_lp).  Per-instance
+// reductions run in a fixed order over the instance's own blocks, so every
+// result is independent of the rest of the batch.
+
+template <typename V>
+__global__ __launch_bounds__(NT) void k_lk_setup(const Arrays<V>* A, Params P,
+                                                 Ctl* ctl, int* done) {
+    __shared__ Ctl C;
+    __shared__ Lds S;
+    const int id = blockIdx.x;
+    const LPDesc D = P.lps[id];
+    const LP<V> L = view(A, D);
+    setup(L, C, S);
+    if (threadIdx.x == 0) init_ctl(C, P, 0);
+    __syncthreads();
+    if (C.stop) finish_lp(L, P, D, C, S, P.res[id], 0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        C.done = C.stop;
+        C.lp = id;
+        ctl[id] = C;
+        done[id] = C.done;
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(NT) void k_lk_primal(const Arrays<V>* A,
+                                                  const LPDesc* lps,
+                                                  const int2* tbl, Ctl* ctl,
+                                                  double* part) {
+    __shared__ Lds S;
+    const int2 e = tbl[blockIdx.x];
+    const LsState st = ctl[e.x].st;
+    if (st.frozen) {
+        if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        return;
+    }
+    const LP<V> L = view(A, lps[e.x]);
+    const int cur = st.cur;
+    const V tau = static_cast<V>(st.eta / st.omega);
+    const V w = static_cast<V>(st.mix_w);
+    const bool fresh = st.fresh != 0;
+    const bool mix = fresh && st.mix_pending != 0;
+    const V* __restrict__ x = L.x(cur);
+    V* __restrict__ xn = L.x(cur ^ 1);
+    const V* __restrict__ y = L.y(cur);
+    const int* rb = L.rbT() + 2 * e.y;
+    const int r0 = rb[0], p0 = rb[1], r1 = rb[2], p1 = rb[3];
+    V* rows = reinterpret_cast<V*>(S.rows);
+    if (fresh)
+        dev::rowblock_sums<V>(L.cp(), L.ci(), L.facx(), dev::GatherPlain<V>{y}, r0, r1,
+                              p0, p1, rows, reinterpret_cast<V*>(S.prod),
+                              reinterpret_cast<V*>(S.red));
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < r1 - r0; t += NT) {
+        const int j = r0 + t;
+        V a;
+        if (fresh) {
+            a = rows[t];
+            L.ata()[j] = a;
+        } else {
+            a = L.ata()[j];
+        }
+        const V xo = x[j];
+        if (mix) {
+            const V xa = L.xavg()[j];
+            L.xavg()[j] = xa + w * (xo - xa);
+        }
+        const V g = L.gref()[j] + a;
+        const V v = fmin(fmax(xo - tau * g, L.lo()[j]), L.hi()[j]);
+        xn[j] = v;
+        L.xbar()[j] = V(2) * v - xo;
+        const V d = v - xo;
+        acc += static_cast<double>(d) * static_cast<double>(d);
+    }
+    acc = bsum(acc, S.red);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+template <typename V>
+__global__ __launch_bounds__(NT) void k_lk_dual(const Arrays<V>* A,
+                                                const LPDesc* lps,
+                                                const int2* tbl, Ctl* ctl,
+                                                double* part) {
+    __shared__ Lds S;
+    const int2 e = tbl[blockIdx.x];
+    const LsState st = ctl[e.x].st;
+    if (st.frozen) {
+        if (threadIdx.x == 0) part[2 * blockIdx.x] = part[2 * blockIdx.x + 1] = 0.0;
+        return;
+    }
+    const LP<V> L = view(A, lps[e.x]);
+    const int cur = st.cur;
+    const V sig = static_cast<V>(st.eta * st.omega);
+    const V w = static_cast<V>(st.mix_w);
+    const bool mix = st.fresh != 0 && st.mix_pending != 0;
+    const V* __restrict__ y = L.y(cur);
+    V* __restrict__ yn = L.y(cur ^ 1);
+    const int* rb = L.rbA() + 2 * e.y;
+    const int r0 = rb[0], p0 = rb[1], r1 = rb[2], p1 = rb[3];
+    V* rows = reinterpret_cast<V*>(S.rows);
+    dev::rowblock_sums<V>(L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.xbar()}, r0,
+                          r1, p0, p1, rows, reinterpret_cast<V*>(S.prod),
+                          reinterpret_cast<V*>(S.red));
+    const V* yoff = L.yoff();
+    double a1 = 0.0, a2 = 0.0;
+    for (int t = threadIdx.x; t < r1 - r0; t += NT) {
+        const int i = r0 + t;
+        const V tm = rows[t];
+        const V yo = y[i];
+        V axc = L.axcur()[i];
+        if (mix) {
+            const V ya = L.yavg()[i];
+            L.yavg()[i] = ya + w * (yo - ya);
+            axc = axc + L.adx()[i];
+            L.axcur()[i] = axc;
+        }
+        const V tt = yo + sig * tm;
+        V u = yoff ? tt + yoff[i] : tt;
+        u = u / sig;
+        const V yv = tt - sig * fmin(fmax(u, L.rlo()[i]), L.rhi()[i]);
+        yn[i] = yv;
+        const V dy = yv - yo;
+        const V ad = V(0.5) * (tm - axc);
+        L.adx()[i] = ad;
+        a1 += static_cast<double>(dy) * static_cast<double>(dy);
+        a2 += static_cast<double>(dy) * static_cast<double>(ad);
+    }
+    a1 = bsum(a1, S.red);
+    a2 = bsum(a2, S.red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a1;
+        part[2 * blockIdx.x + 1] = a2;
+    }
+}
+
+// One wavefront per active instance: fixed-order sums of its block
+// partials, then the line-search rule on its state.
+__global__ __launch_bounds__(NT) void k_lk_decide(const LsRange* rng, int nact,
+                                                  Ctl* ctl, const double* partT,
+                                                  const double* partA,
+                                                  int* nfrozen) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int a = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE;
+    if (a >= nact) return;
+    const LsRange r = rng[a];
+    LsState* sp = &ctl[r.lp].st;
+    if (sp->frozen) return;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int b = lane; b < r.nT; b += WAVE) d0 += partT[r.bT0 + b];
+    for (int b = lane; b < r.nA; b += WAVE) {
+        d1 += partA[2 * (r.bA0 + b)];
+        d2 += partA[2 * (r.bA0 + b) + 1];
+    }
+    d0 = dev::wave_sum(d0);
+    d1 = dev::wave_sum(d1);
+    d2 = dev::wave_sum(d2);
+    if (lane == 0) {
+        LsState s = *sp;
+        pdhg::ls_decide(s, d0, d1, d2);
+        *sp = s;
+        if (s.frozen) atomicAdd(nfrozen, 1);
+    }
+}
+
+// The check of every active instance (all frozen): one workgroup each.
+template <typename V>
+__global__ __launch_bounds__(NT) void k_lk_check(const Arrays<V>* A, Params P,
+                                                 const LsRange* rng, Ctl* ctl,
+                                                 int* done) {
+    __shared__ Ctl C;
+    __shared__ Lds S;
+    const int id = rng[blockIdx.x].lp;
+    if (threadIdx.x == 0) C = ctl[id];
+    __syncthreads();
+    const LPDesc D = P.lps[id];
+    const LP<V> L = view(A, D);
+    if (C.st.mix_pending) {
+        apply_mix(L, C);
+        __syncthreads();
+    }
+    const long long kdone = C.st.k - 1;
+    bool stop;
+    if (kdone % pdhg::CHECK_EVERY != 0) stop = kdone >= P.maxit;  // MAXIT off a boundary
+    else stop = check_lp(L, P, C, S, kdone);
+    if (stop) finish_lp(L, P, D, C, S, P.res[id], kdone);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        C.st.mix_pending = 0;
+        if (stop) C.done = 1;
+        else C.st.frozen = 0;
+        ctl[id] = C;
+        done[blockIdx.x] = C.done;
+    }
+}
+
+size_t lockstep_ctl_bytes() { return sizeof(Ctl); }
+
+template <typename V>
+static void lk_setup_impl(const Arrays<V>* a, const Params& p, void* ctl,
+                          int* done, hipStream_t s) {
+    k_lk_setup<V><<<p.count, NT, 0, s>>>(a, p, static_cast<Ctl*>(ctl), done);
+}
+template <typename V>
+static void lk_round_impl(const Arrays<V>* a, const LPDesc* lps, const int2* tblT,
+                          int nbT, const int2* tblA, int nbA, const LsRange* rng,
+                          int nact, void* ctl, double* partT, double* partA,
+                          int* nfrozen, hipStream_t s) {
+    Ctl* c = static_cast<Ctl*>(ctl);
+    if (nbT > 0) k_lk_primal<V><<<nbT, NT, 0, s>>>(a, lps, tblT, c, partT);
+    if (nbA > 0) k_lk_dual<V><<<nbA, NT, 0, s>>>(a, lps, tblA, c, partA);
+    const int per = NT / WAVE;
+    k_lk_decide<<<(nact + per - 1) / per, NT, 0, s>>>(rng, nact, c, partT, partA, nfrozen);
+}
+template <typename V>
+static void lk_check_impl(const Arrays<V>* a, const Params& p, const LsRange* rng,
+                          int nact, void* ctl, int* done, hipStream_t s) {
+    k_lk_check<V><<<nact, NT, 0, s>>>(a, p, rng, static_cast<Ctl*>(ctl), done);
+}
+
+void lockstep_setup(const Arrays<float>* a, const Params& p, void* ctl, int* done,
+                    hipStream_t s) { lk_setup_impl(a, p, ctl, done, s); }
+void lockstep_setup(const Arrays<double>* a, const Params& p, void* ctl, int* done,
+                    hipStream_t s) { lk_setup_impl(a, p, ctl, done, s); }
+void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const int2* tblT,
+                    int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
+                    void* ctl, double* partT, double* partA, int* nfrozen,
+                    hipStream_t s) {
+    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, s);
+}
+void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const int2* tblT,
+                    int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
+                    void* ctl, double* partT, double* partA, int* nfrozen,
+                    hipStream_t s) {
+    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, s);
+}
+void lockstep_check(const Arrays<float>* a, const Params& p, const LsRange* rng,
+                    int nact, void* ctl, int* done, hipStream_t s) {
+    lk_check_impl(a, p, rng, nact, ctl, done, s);
+}
+void lockstep_check(const Arrays<double>* a, const Params& p, const LsRange* rng,
+                    int nact, void* ctl, int* done, hipStream_t s) {
+    lk_check_impl(a, p, rng, nact, ctl, done, s);
 }
 
 }  // namespace igaos::batch

@@ -78,4 +78,31 @@ void launch_batch(const Arrays<double>* a, const Params& p, int grid,
                   int waves, hipStream_t s);
 int batch_occupancy(bool f32, int waves);
 
+// Lockstep mode (batch.hip): per-instance control blocks live in device
+// memory (lockstep_ctl_bytes() each); the host keeps tables of the active
+// instances' row blocks as (instance, local block) pairs and one LsRange
+// per active instance.
+struct LsRange {
+    int lp;        // instance
+    int bT0, nT;   // its blocks in the A' table
+    int bA0, nA;   // its blocks in the A table
+};
+size_t lockstep_ctl_bytes();
+void lockstep_setup(const Arrays<float>* a, const Params& p, void* ctl, int* done,
+                    hipStream_t s);
+void lockstep_setup(const Arrays<double>* a, const Params& p, void* ctl, int* done,
+                    hipStream_t s);
+void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const int2* tblT,
+                    int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
+                    void* ctl, double* partT, double* partA, int* nfrozen,
+                    hipStream_t s);
+void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const int2* tblT,
+                    int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
+                    void* ctl, double* partT, double* partA, int* nfrozen,
+                    hipStream_t s);
+void lockstep_check(const Arrays<float>* a, const Params& p, const LsRange* rng,
+                    int nact, void* ctl, int* done, hipStream_t s);
+void lockstep_check(const Arrays<double>* a, const Params& p, const LsRange* rng,
+                    int nact, void* ctl, int* done, hipStream_t s);
+
 }  // namespace igaos::batch

@@ -5,11 +5,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "adjoint.hpp"
@@ -78,10 +80,25 @@ struct igaos_batch {
     int waves = 2;
     int cus = 1;
     bool solved = false;  // a run finished and the data are unchanged since
+    int device = 0;
+    int mode = 0;         // 0: persistent kernel, 1: lockstep rounds
+    std::thread worker;   // lockstep: drives the rounds of one run
+    std::string worker_err;
+    std::atomic<int> lk_done{0};
+    Mem lkmem;            // lockstep buffers, allocated on first use
+    void* d_ctl = nullptr;
+    int2* d_tblT = nullptr;
+    int2* d_tblA = nullptr;
+    igaos::batch::LsRange* d_rng = nullptr;
+    double* d_partT = nullptr;
+    double* d_partA = nullptr;
+    int* d_done = nullptr;
+    int* d_nfrozen = nullptr;
     Mem amem;             // adjoint workspace, allocated on first use
     igaos::batch::AdjArrays adj{};
     igaos::batch::AdjArrays* d_adj = nullptr;
     ~igaos_batch() {
+        if (worker.joinable()) worker.join();
         if (running && e1) (void)hipEventSynchronize(e1);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -164,11 +181,108 @@ void fill_arrays(igaos_batch& B, igaos::batch::Arrays<V>& A, const int* ap,
     A.yout = M.alloc<double>(m);
 }
 
+template <typename V>
+void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
+                   const igaos::batch::Params& P) {
+    using igaos::batch::LsRange;
+    hipStream_t s = B->s;
+    const int count = B->count;
+    BCK(hipEventRecord(B->e0, s));
+    igaos::batch::lockstep_setup(A, P, B->d_ctl, B->d_done, s);
+    BCK(hipGetLastError());
+    std::vector<int> flags(count);
+    BCK(hipMemcpyAsync(flags.data(), B->d_done, sizeof(int) * count,
+                       hipMemcpyDeviceToHost, s));
+    BCK(hipStreamSynchronize(s));
+    std::vector<int> active;
+    for (int i = 0; i < count; ++i)
+        if (!flags[i]) active.push_back(i);
+    B->lk_done = count - static_cast<int>(active.size());
+    std::vector<int2> tT, tA;
+    std::vector<LsRange> rng;
+    while (!active.empty()) {
+        // tables of the active instances' row blocks
+        tT.clear();
+        tA.clear();
+        rng.clear();
+        for (int lp : active) {
+            const LPDesc& d = B->desc[lp];
+            LsRange r{lp, static_cast<int>(tT.size()), d.nblkT,
+                      static_cast<int>(tA.size()), d.nblkA};
+            for (int b = 0; b < d.nblkT; ++b) tT.push_back(make_int2(lp, b));
+            for (int b = 0; b < d.nblkA; ++b) tA.push_back(make_int2(lp, b));
+            rng.push_back(r);
+        }
+        const int nact = static_cast<int>(active.size());
+        const int nbT = static_cast<int>(tT.size()), nbA = static_cast<int>(tA.size());
+        BCK(hipMemcpyAsync(B->d_tblT, tT.data(), sizeof(int2) * nbT, hipMemcpyHostToDevice, s));
+        BCK(hipMemcpyAsync(B->d_tblA, tA.data(), sizeof(int2) * nbA, hipMemcpyHostToDevice, s));
+        BCK(hipMemcpyAsync(B->d_rng, rng.data(), sizeof(LsRange) * nact,
+                           hipMemcpyHostToDevice, s));
+        BCK(hipMemsetAsync(B->d_nfrozen, 0, sizeof(int), s));
+        // attempt rounds until every active instance froze on its check
+        // boundary (64 accepted iterations, plus retries)
+        int chunk = igaos::pdhg::CHECK_EVERY;
+        for (;;) {
+            for (int r = 0; r < chunk; ++r)
+                igaos::batch::lockstep_round(A, B->d_desc, B->d_tblT, nbT, B->d_tblA, nbA,
+                                             B->d_rng, nact, B->d_ctl, B->d_partT,
+                                             B->d_partA, B->d_nfrozen, s);
+            BCK(hipGetLastError());
+            int nf = 0;
+            BCK(hipMemcpyAsync(&nf, B->d_nfrozen, sizeof(int), hipMemcpyDeviceToHost, s));
+            BCK(hipStreamSynchronize(s));
+            if (nf >= nact) break;
+            chunk = 4;
+        }
+        igaos::batch::lockstep_check(A, P, B->d_rng, nact, B->d_ctl, B->d_done, s);
+        BCK(hipGetLastError());
+        BCK(hipMemcpyAsync(flags.data(), B->d_done, sizeof(int) * nact,
+                           hipMemcpyDeviceToHost, s));
+        BCK(hipStreamSynchronize(s));
+        std::vector<int> next;
+        next.reserve(active.size());
+        for (int a = 0; a < nact; ++a)
+            if (!flags[a]) next.push_back(active[a]);
+        B->lk_done += nact - static_cast<int>(next.size());
+        active.swap(next);
+    }
+    BCK(hipEventRecord(B->e1, s));
+}
+
+void lockstep_alloc(igaos_batch* B) {
+    if (B->d_ctl) return;
+    Mem& M = B->lkmem;
+    long bt = 0, ba = 0;
+    for (const LPDesc& d : B->desc) {
+        bt += d.nblkT;
+        ba += d.nblkA;
+    }
+    B->d_ctl = M.alloc<unsigned char>(igaos::batch::lockstep_ctl_bytes() * B->count);
+    B->d_tblT = M.alloc<int2>(bt);
+    B->d_tblA = M.alloc<int2>(ba);
+    B->d_rng = M.alloc<igaos::batch::LsRange>(B->count);
+    B->d_partT = M.alloc<double>(bt);
+    B->d_partA = M.alloc<double>(2 * ba);
+    B->d_done = M.alloc<int>(B->count);
+    B->d_nfrozen = M.alloc<int>(1);
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* igaos_batch_last_error(void) { return g_berr.c_str(); }
+
+int igaos_batch_set_mode(igaos_batch* B, int mode) {
+    if (!B) return bfail("null batch");
+    if (B->running) return bfail("batch is running");
+    if (mode != 0 && mode != 1) return bfail("mode must be 0 (persistent) or 1 (lockstep)");
+    B->mode = mode;
+    return 0;
+}
+
+int igaos_batch_get_mode(const igaos_batch* B) { return B ? B->mode : -1; }
 
 igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
                                 const int* nnz, const double* c,
@@ -252,6 +366,7 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
         hipDeviceProp_t prop;
         BCK(hipGetDeviceProperties(&prop, 0));
         B->cus = prop.multiProcessorCount;
+        BCK(hipGetDevice(&B->device));
         const int occ = igaos::batch::batch_occupancy(precision == 32, B->waves);
         B->grid = std::min(count, B->cus * occ);
         BCK(hipEventCreate(&B->e0));
@@ -325,6 +440,23 @@ int igaos_batch_launch(igaos_batch* B, const igaos_options* opt) {
         BCK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
         P.ticks_per_s = static_cast<double>(khz) * 1e3;
         P.progress = B->d_prog;
+        if (B->mode == 1) {
+            lockstep_alloc(B);
+            P.progress = nullptr;
+            B->lk_done = 0;
+            B->worker_err.clear();
+            B->running = true;
+            B->worker = std::thread([B, P]() {
+                try {
+                    BCK(hipSetDevice(B->device));
+                    if (B->precision == 32) lockstep_loop(B, B->d_a32, P);
+                    else lockstep_loop(B, B->d_a64, P);
+                } catch (const std::exception& e) {
+                    B->worker_err = e.what();
+                }
+            });
+            return 0;
+        }
         std::memset(B->h_prog, 0, sizeof(unsigned int) * B->count);
         BCK(hipMemsetAsync(B->d_queue, 0, sizeof(int), B->s));
         BCK(hipEventRecord(B->e0, B->s));
@@ -341,6 +473,7 @@ int igaos_batch_launch(igaos_batch* B, const igaos_options* opt) {
 
 int igaos_batch_progress(const igaos_batch* B) {
     if (!B) return bfail("null batch");
+    if (B->mode == 1) return B->lk_done.load();
     long done = 0;
     const volatile unsigned int* p = B->h_prog;
     for (int g = 0; g < B->grid; ++g) done += p[g];
@@ -355,6 +488,10 @@ int igaos_batch_wait(igaos_batch* B, double* x, double* y,
     try {
         B->running = false;
         B->solved = false;
+        if (B->worker.joinable()) {
+            B->worker.join();
+            if (!B->worker_err.empty()) throw std::runtime_error(B->worker_err);
+        }
         BCK(hipEventSynchronize(B->e1));
         float ms = 0.0f;
         BCK(hipEventElapsedTime(&ms, B->e0, B->e1));

@@ -187,7 +187,18 @@ int igaos_batch_set_schedule(igaos_batch* batch, const int* tpr, int grid,
                              int waves);
 int igaos_batch_get_schedule(const igaos_batch* batch, int* tpr, int* grid,
                              int* waves);
-/* Device time of the last run's persistent kernel (HIP events), ms. */
+/* Execution mode of later runs: 0 = one persistent launch, each workgroup
+ * solving whole instances taken from a queue (the default); 1 = lockstep
+ * rounds: every active instance advances one line-search attempt per
+ * round, each round being three launches over the row blocks of all active
+ * instances, with a per-instance check (one workgroup each) every 64
+ * accepted iterations and finished instances dropped from the next
+ * rounds.  Both modes run the same per-instance algorithm and controller;
+ * their SpMV summation orders differ.  In mode 1 launch starts a host
+ * thread that drives the rounds; progress and wait work as in mode 0. */
+int igaos_batch_set_mode(igaos_batch* batch, int mode);
+int igaos_batch_get_mode(const igaos_batch* batch);
+/* Device time of the last run (HIP events around all its launches), ms. */
 double igaos_batch_kernel_ms(const igaos_batch* batch);
 /* Replaces the batch's numeric data in place (same sparsity pattern and
  * sizes; host arrays in the layout of igaos_batch_create, NULL keeps an

@@ -27,10 +27,15 @@ def _c4_key(i):
     return f"gen_batch_instance(seed=0,index={i})"
 
 
+MODES = ["persistent", "lockstep"]
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("precision", [32, 64])
-def test_batch_solves_match_fixtures(precision, objectives):
+def test_batch_solves_match_fixtures(precision, mode, objectives):
     lps = [lpgen.gen_batch_instance(0, i) for i in range(16)]
-    sol = solve_batch(lps, precision=precision, tolerance=1e-6, time_limit=120)
+    sol = solve_batch(lps, precision=precision, mode=mode, tolerance=1e-6,
+                      time_limit=120)
     for i, lp in enumerate(lps):
         ref = _fixture_obj(objectives, _c4_key(i))
         assert sol.status[i] == "near-optimal", (i, sol.status[i])
@@ -43,9 +48,10 @@ def test_batch_solves_match_fixtures(precision, objectives):
         assert mt.pinf <= 1e-6 and mt.gap <= 1e-6
 
 
-def test_batch_fp64_trajectory_matches_oracle():
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_fp64_trajectory_matches_oracle(mode):
     lps = [lpgen.gen_batch_instance(0, i, m_hi=600) for i in range(6)]
-    sol = solve_batch(lps, precision=64, tolerance=0.0, time_limit=1e7,
+    sol = solve_batch(lps, precision=64, mode=mode, tolerance=0.0, time_limit=1e7,
                       max_iterations=64)
     for i, lp in enumerate(lps):
         r = oracle.solve(lp, tolerance=0.0, time_limit=1e7, max_iterations=64)
@@ -64,20 +70,23 @@ def test_batch_fp64_matches_single_engine():
         assert abs(sol.objective[i] - s.objective) <= 1e-6 * (1 + abs(s.objective))
 
 
-def test_instance_independent_of_batch():
+@pytest.mark.parametrize("mode", MODES)
+def test_instance_independent_of_batch(mode):
     lps = [lpgen.gen_batch_instance(1, i, m_hi=400) for i in range(10)]
-    whole = solve_batch(lps, precision=32, tolerance=1e-6, time_limit=60)
-    alone = solve_batch([lps[4]], precision=32, tolerance=1e-6, time_limit=60)
+    whole = solve_batch(lps, precision=32, mode=mode, tolerance=1e-6, time_limit=60)
+    alone = solve_batch([lps[4]], precision=32, mode=mode, tolerance=1e-6,
+                        time_limit=60)
     assert whole.iterations[4] == alone.iterations[0]
     assert np.array_equal(whole.x[4], alone.x[0])
     assert np.array_equal(whole.y[4], alone.y[0])
 
 
-def test_batch_general_rows_and_bounds(objectives):
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_general_rows_and_bounds(mode, objectives):
     # Netlib instances: inequality rows, free and fixed columns, RANGES
     names = ["afiro", "sc50a", "adlittle"]
     lps = [igaos_amd.read_mps(netlib_path(n)).to_lp() for n in names]
-    sol = solve_batch(lps, precision=32, tolerance=1e-6, time_limit=120)
+    sol = solve_batch(lps, precision=32, mode=mode, tolerance=1e-6, time_limit=120)
     for i, n in enumerate(names):
         ref = objectives[f"netlib/{n}"]["highs_ref"]
         assert sol.status[i] == "near-optimal", (n, sol.status[i])
@@ -116,3 +125,18 @@ def test_autotuner_picks_a_valid_schedule(tmp_path):
     s = b.run(tolerance=1e-6, time_limit=60)
     b.close()
     assert s.solved == len(lps)
+
+
+def test_lockstep_iteration_cap_and_modes_agree():
+    lps = [lpgen.gen_batch_instance(2, i, m_hi=300) for i in range(5)]
+    b = BatchSolver(lps, precision=32, mode="lockstep")
+    s1 = b.run(tolerance=0.0, time_limit=1e7, max_iterations=100)
+    assert all(s == "iteration-limit" for s in s1.status)
+    assert list(s1.iterations) == [100] * 5
+    s2 = b.run(tolerance=1e-6, time_limit=60)
+    b.set_mode("persistent")
+    s3 = b.run(tolerance=1e-6, time_limit=60)
+    b.close()
+    for i in range(5):
+        assert s2.status[i] == s3.status[i] == "near-optimal"
+        assert abs(s2.objective[i] - s3.objective[i]) <= 1e-6 * (1 + abs(s3.objective[i]))
