@@ -20,6 +20,7 @@ class BatchResult(C.Structure):
 class AdjointInfo(C.Structure):
     _fields_ = [("basic_cols", C.c_int), ("active_rows", C.c_int),
                 ("iters_p", C.c_int), ("iters_q", C.c_int),
+                ("accepted", C.c_int), ("reserved", C.c_int),
                 ("res_p", C.c_double), ("res_q", C.c_double)]
 
 
@@ -40,6 +41,8 @@ SIGNATURES = {
     "igaos_batch_set_mode": (C.c_int, [_vp, C.c_int]),
     "igaos_batch_get_mode": (C.c_int, [_vp]),
     "igaos_batch_update": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "igaos_batch_polish": (C.c_int, [_vp, C.c_double, C.c_int, _dp, _dp, _dp,
+                                     C.POINTER(AdjointInfo)]),
     "igaos_batch_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_int,
                                       _dp, _dp, _dp, _dp, _dp, _dp,
                                       C.POINTER(AdjointInfo)]),

@@ -6,7 +6,10 @@ and ``a_values`` (sum nnz, CSR order of each instance) -- concatenated over
 the instances, any of them ``requires_grad`` -- solves every instance on the
 GPU with the batched persistent PDHG engine and returns ``x*`` (sum n),
 ``y*`` (sum m, dual sign convention d = c + A'y) and the objectives
-(count).  ``backward`` runs the batched adjoint kernel
+(count).  With ``polish`` (default) each PDHG solution is first polished on
+its active set (igaos_batch_polish: the exact vertex of the identified
+basis, kept only if feasible to 1e-9), so x*, y* and the gradients carry
+FP64 accuracy rather than the solve tolerance.  ``backward`` runs the batched adjoint kernel
 (igaos_batch_adjoint: masked CGLS on the transposed fixed-point system of
 the PDHG map linearised at the solution) and returns the gradients of c,
 rmin, rmax and A's values.  For equality rows pass the same tensor ``b`` as
@@ -64,10 +67,12 @@ class DiffLPBatch:
     ``igaos_amd.lpgen.LP``); numeric data per call."""
 
     def __init__(self, lps, precision=32, tolerance=1e-6, time_limit=60.0,
-                 max_iterations=500000, adjoint_tolerance=1e-10,
-                 adjoint_max_iterations=None):
+                 max_iterations=500000, adjoint_tolerance=1e-12,
+                 adjoint_max_iterations=None, polish=True, mode="persistent"):
         self.lps = list(lps)
-        self.solver = BatchSolver(self.lps, precision=precision)
+        self.solver = BatchSolver(self.lps, precision=precision, mode=mode)
+        self.polish = polish
+        self.last_polish_info = None
         self.tolerance = tolerance
         self.time_limit = time_limit
         self.max_iterations = max_iterations
@@ -86,9 +91,15 @@ class DiffLPBatch:
                               max_iterations=self.max_iterations)
         self._generation += 1
         self.last_solution = sol
-        self._x = np.concatenate(sol.x) if sol.x else np.zeros(0)
-        self._y = np.concatenate(sol.y) if sol.y else np.zeros(0)
-        self._obj = np.asarray(sol.objective, dtype=np.float64)
+        xs, ys, obj = sol.x, sol.y, sol.objective
+        if self.polish:
+            xs, ys, obj, info = self.solver.polish(
+                tolerance=self.adjoint_tolerance,
+                max_iterations=self.adjoint_max_iterations)
+            self.last_polish_info = info
+        self._x = np.concatenate(xs) if xs else np.zeros(0)
+        self._y = np.concatenate(ys) if ys else np.zeros(0)
+        self._obj = np.asarray(obj, dtype=np.float64)
 
     def __call__(self, c, rmin, rmax, a_values):
         return _SolveFn.apply(self, c, rmin, rmax, a_values)

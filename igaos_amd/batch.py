@@ -174,6 +174,31 @@ class BatchSolver:
         if rc != 0:
             raise RuntimeError(L.igaos_batch_last_error().decode())
 
+    def _info(self, info):
+        f = lambda k: np.array([getattr(r, k) for r in info])  # noqa
+        return {k: f(k) for k in ("basic_cols", "active_rows", "iters_p",
+                                  "iters_q", "accepted", "res_p", "res_q")}
+
+    def polish(self, tolerance=1e-12, max_iterations=None):
+        """Active-set polishing of the last run's solutions
+        (igaos_batch_polish): returns (x list, y list, objectives, info);
+        info["accepted"] marks the instances whose polished pair was kept."""
+        from ._batch_sigs import AdjointInfo
+        L = _core.lib()
+        n, m = int(self._coff[-1]), int(self._roff[-1])
+        if max_iterations is None:
+            max_iterations = int(min(200000, 20 * (self.m.max() + self.n.max())))
+        x, y, obj = np.zeros(n), np.zeros(m), np.zeros(self.count)
+        info = (AdjointInfo * self.count)()
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+        rc = L.igaos_batch_polish(self._h, float(tolerance), int(max_iterations),
+                                  dp(x), dp(y), dp(obj), info)
+        if rc != 0:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+        xs = [x[self._coff[i]:self._coff[i + 1]] for i in range(self.count)]
+        ys = [y[self._roff[i]:self._roff[i + 1]] for i in range(self.count)]
+        return xs, ys, obj, self._info(info)
+
     def adjoint(self, gx=None, gy=None, gobj=None, tolerance=1e-10,
                 max_iterations=None) -> dict:
         """Gradients through the last run's solutions (include/igaos_pdhg.h,
@@ -199,9 +224,7 @@ class BatchSolver:
             *[dp(out[k]) for k in ("gc", "grmin", "grmax", "gcl", "gcu", "gax")], info)
         if rc != 0:
             raise RuntimeError(L.igaos_batch_last_error().decode())
-        f = lambda k: np.array([getattr(r, k) for r in info])  # noqa
-        out["info"] = {k: f(k) for k in ("basic_cols", "active_rows", "iters_p",
-                                         "iters_q", "res_p", "res_q")}
+        out["info"] = self._info(info)
         return out
 
     def close(self):

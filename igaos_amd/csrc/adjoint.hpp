@@ -18,6 +18,9 @@
 //   dL/db_i    = [i in R] q_i          (b_i = the bound row i sits on)
 //   dL/dA_ij   = -[i in R] ([j in B] p_j y_i + q_i x_j)
 //   dL/dbnd_j  = [j not in B] (w_j - (A_R' q)_j)   (the bound x_j sits on).
+// Polishing uses the same masks: it solves the active-set equations for
+// corrections of x_B and y_R (x_N on its bounds, y off R zero), and keeps
+// the result only if it is primal and dual feasible to 1e-9.
 // Both systems are solved per instance by CGLS in FP64 on the Ruiz-scaled
 // matrix (the scaled systems are  A~_RB p~ = D_r^-1 gy,  A~_RB' q~ = D_c w
 // with p = D_c p~, q = q~ / D_r), reusing the batched solver's instance
@@ -32,6 +35,8 @@ namespace igaos::batch {
 
 struct AdjInfo {
     int basic_cols, active_rows, iters_p, iters_q;
+    int accepted;         // polish: the polished pair passed the checks
+    int pad;
     double res_p, res_q;  // final relative residuals of the two systems
 };
 
@@ -41,6 +46,8 @@ struct AdjArrays {
     double *z2, *s2, *p2, *r2, *t2;         // A_RB' q = w   (z2, s2, p2: m; r2, t2: n)
     const double *gx, *gy, *gobj;           // upstream gradients
     double *gc, *gcl, *gcu, *grmin, *grmax, *gax;  // outputs (gax in CSR order)
+    double *xs, *ys;                        // polish: scaled candidate pair
+    double* obj;                            // polish: objective of accepted pairs
     AdjInfo* info;
     double tol;
     int maxit;
@@ -48,6 +55,10 @@ struct AdjArrays {
 
 void launch_adjoint(const Arrays<float>* a, const LPDesc* lps, int count,
                     const AdjArrays* adj, hipStream_t s);
+void launch_polish(const Arrays<float>* a, const LPDesc* lps, int count,
+                   const AdjArrays* adj, hipStream_t s);
+void launch_polish(const Arrays<double>* a, const LPDesc* lps, int count,
+                   const AdjArrays* adj, hipStream_t s);
 void launch_adjoint(const Arrays<double>* a, const LPDesc* lps, int count,
                     const AdjArrays* adj, hipStream_t s);
 

@@ -548,37 +548,90 @@ int igaos_batch_update(igaos_batch* B, const double* c, const double* cl,
     return 0;
 }
 
+namespace {
+
+void adjoint_alloc(igaos_batch* B) {
+    if (B->d_adj) return;
+    const long m = B->sum_m, n = B->sum_n, z = B->sum_z;
+    igaos::batch::AdjArrays& J = B->adj;
+    Mem& M = B->amem;
+    J.mc = M.alloc<unsigned char>(n);
+    J.mr = M.alloc<unsigned char>(m);
+    for (double** p : {&J.z1, &J.s1, &J.p1, &J.r2, &J.t2, &J.xs}) *p = M.alloc<double>(n);
+    for (double** p : {&J.z2, &J.s2, &J.p2, &J.r1, &J.t1, &J.ys}) *p = M.alloc<double>(m);
+    J.gx = M.alloc<double>(n);
+    J.gy = M.alloc<double>(m);
+    J.gobj = M.alloc<double>(B->count);
+    J.obj = M.alloc<double>(B->count);
+    for (double** p : {&J.gc, &J.gcl, &J.gcu}) *p = M.alloc<double>(n);
+    J.grmin = M.alloc<double>(m);
+    J.grmax = M.alloc<double>(m);
+    J.gax = M.alloc<double>(z);
+    J.info = M.alloc<igaos::batch::AdjInfo>(B->count);
+    B->d_adj = M.alloc<igaos::batch::AdjArrays>(1);
+}
+
+int check_active_set_call(igaos_batch* B, double tolerance, int max_iterations) {
+    if (!B) return bfail("null batch");
+    if (B->running) return bfail("batch is running");
+    if (!B->solved) return bfail("needs a finished run on the current data");
+    if (!(tolerance >= 0.0) || max_iterations < 0)
+        return bfail("tolerance must be >= 0 and max_iterations >= 0");
+    return 0;
+}
+
+}  // namespace
+
+int igaos_batch_polish(igaos_batch* B, double tolerance, int max_iterations,
+                       double* x, double* y, double* objective,
+                       igaos_adjoint_info* info) {
+    if (check_active_set_call(B, tolerance, max_iterations) != 0) return -1;
+    try {
+        adjoint_alloc(B);
+        igaos::batch::AdjArrays& J = B->adj;
+        J.tol = tolerance;
+        J.maxit = max_iterations;
+        BCK(hipMemcpyAsync(B->d_adj, &J, sizeof(J), hipMemcpyHostToDevice, B->s));
+        // objectives of rejected pairs stay as the run reported them
+        std::vector<LPResult> res(B->count);
+        BCK(hipMemcpy(res.data(), B->d_res, sizeof(LPResult) * B->count,
+                      hipMemcpyDeviceToHost));
+        std::vector<double> obj(B->count);
+        for (int i = 0; i < B->count; ++i) obj[i] = res[i].objective;
+        BCK(hipMemcpy(J.obj, obj.data(), sizeof(double) * B->count, hipMemcpyHostToDevice));
+        if (B->precision == 32)
+            igaos::batch::launch_polish(B->d_a32, B->d_desc, B->count, B->d_adj, B->s);
+        else
+            igaos::batch::launch_polish(B->d_a64, B->d_desc, B->count, B->d_adj, B->s);
+        BCK(hipGetLastError());
+        BCK(hipStreamSynchronize(B->s));
+        const double* xo = B->precision == 32 ? B->a32.xout : B->a64.xout;
+        const double* yo = B->precision == 32 ? B->a32.yout : B->a64.yout;
+        if (x) BCK(hipMemcpy(x, xo, sizeof(double) * B->sum_n, hipMemcpyDeviceToHost));
+        if (y) BCK(hipMemcpy(y, yo, sizeof(double) * B->sum_m, hipMemcpyDeviceToHost));
+        if (objective)
+            BCK(hipMemcpy(objective, J.obj, sizeof(double) * B->count, hipMemcpyDeviceToHost));
+        if (info)
+            BCK(hipMemcpy(info, J.info, sizeof(igaos::batch::AdjInfo) * B->count,
+                          hipMemcpyDeviceToHost));
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
+
 int igaos_batch_adjoint(igaos_batch* B, const double* gx, const double* gy,
                         const double* gobj, double tolerance,
                         int max_iterations, double* gc, double* grmin,
                         double* grmax, double* gcl, double* gcu, double* gax,
                         igaos_adjoint_info* info) {
-    if (!B) return bfail("null batch");
-    if (B->running) return bfail("batch is running");
-    if (!B->solved) return bfail("adjoint needs a finished run on the current data");
-    if (!(tolerance >= 0.0) || max_iterations < 0)
-        return bfail("adjoint tolerance must be >= 0 and max_iterations >= 0");
+    if (check_active_set_call(B, tolerance, max_iterations) != 0) return -1;
     static_assert(sizeof(igaos_adjoint_info) == sizeof(igaos::batch::AdjInfo),
                   "adjoint info layout");
     try {
         const long m = B->sum_m, n = B->sum_n, z = B->sum_z;
+        adjoint_alloc(B);
         igaos::batch::AdjArrays& J = B->adj;
-        if (!B->d_adj) {
-            Mem& M = B->amem;
-            J.mc = M.alloc<unsigned char>(n);
-            J.mr = M.alloc<unsigned char>(m);
-            for (double** p : {&J.z1, &J.s1, &J.p1, &J.r2, &J.t2}) *p = M.alloc<double>(n);
-            for (double** p : {&J.z2, &J.s2, &J.p2, &J.r1, &J.t1}) *p = M.alloc<double>(m);
-            J.gx = M.alloc<double>(n);
-            J.gy = M.alloc<double>(m);
-            J.gobj = M.alloc<double>(B->count);
-            for (double** p : {&J.gc, &J.gcl, &J.gcu}) *p = M.alloc<double>(n);
-            J.grmin = M.alloc<double>(m);
-            J.grmax = M.alloc<double>(m);
-            J.gax = M.alloc<double>(z);
-            J.info = M.alloc<igaos::batch::AdjInfo>(B->count);
-            B->d_adj = M.alloc<igaos::batch::AdjArrays>(1);
-        }
         auto in = [&](const double* src, const double* dst, long k) {
             double* d = const_cast<double*>(dst);
             if (src) BCK(hipMemcpyAsync(d, src, sizeof(double) * k, hipMemcpyHostToDevice, B->s));
@@ -613,7 +666,5 @@ int igaos_batch_adjoint(igaos_batch* B, const double* gx, const double* gy,
     }
     return 0;
 }
-
-void igaos_batch_destroy(igaos_batch* B) { delete B; }
 
 }  // extern "C"

@@ -229,8 +229,21 @@ int igaos_batch_update(igaos_batch* batch, const double* c, const double* cl,
 typedef struct {
     int basic_cols, active_rows;  /* |B|, |R| */
     int iters_p, iters_q;         /* CGLS iterations of the two systems */
+    int accepted;                 /* polish: the polished pair was kept */
+    int reserved;
     double res_p, res_q;          /* final relative residuals */
 } igaos_adjoint_info;
+/* Active-set polishing of the last run's solutions (no reference
+ * counterpart): with the same masks, x_N is put on its bounds, y off R to
+ * zero, and the corrections of x_B and y_R solving A_RB x_B = b_R - A_RN x_N
+ * and A_RB' y_R = -c_B are found by CGLS (FP64); the polished pair replaces
+ * the instance's solution (also for a later igaos_batch_adjoint) only if
+ * it is primal feasible, has d_B = 0 and correct reduced-cost and dual
+ * signs, all to 1e-9 relative (info.accepted).  x, y, objective (count)
+ * receive the resulting solutions; any may be NULL. */
+int igaos_batch_polish(igaos_batch* batch, double tolerance, int max_iterations,
+                       double* x, double* y, double* objective,
+                       igaos_adjoint_info* info);
 int igaos_batch_adjoint(igaos_batch* batch, const double* gx, const double* gy,
                         const double* gobj, double tolerance,
                         int max_iterations, double* gc, double* grmin,

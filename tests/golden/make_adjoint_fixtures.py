@@ -4,7 +4,8 @@ gradients of the differentiable batched solve.
 Workload: the 512-LP differentiable batch (igaos_amd.lpgen.gen_diff_batch,
 seed 0: m log-uniform in [256, 1024], n = 2m, 6 nonzeros per column, box
 [0, 10]), each instance made nondegenerate by
-lpgen.perturb_nondegenerate(lp, seed=index, scale=1e-6).  The first 32
+lpgen.perturb_nondegenerate(lp, seed=index, scale=1e-6), with c, b and A's
+values then rounded to FP32.  The first 32
 instances whose exact solution (scipy HiGHS, tests/highs_util.py) is
 nondegenerate -- exactly m columns strictly inside their bounds by more
 than 1e-7 and every other reduced cost above 1e-7 in magnitude -- are kept.
@@ -43,10 +44,16 @@ GROUPS = ("c", "b", "A")
 
 
 def instance(index):
-    """Instance `index` of gen_diff_batch(seed=0), made nondegenerate."""
+    """Instance `index` of gen_diff_batch(seed=0), made nondegenerate, with
+    c, b and A's values rounded to FP32 (the differentiable layer takes
+    FP32 tensors; the finite differences are taken in FP64 around exactly
+    those values)."""
     lp = lpgen.gen_batch_instance(0, index, m_lo=256, m_hi=1024, ratio_lo=2.0,
                                   ratio_hi=2.0, k_lo=6, k_hi=6)
-    return lpgen.perturb_nondegenerate(lp, seed=index, scale=1e-6)
+    lp = lpgen.perturb_nondegenerate(lp, seed=index, scale=1e-6)
+    f = lambda v: np.asarray(v, np.float32).astype(np.float64)  # noqa
+    b = f(lp.rmin)
+    return lp.with_values(ax=f(lp.ax), c=f(lp.c), rmin=b, rmax=b.copy())
 
 
 def weights(index, n):

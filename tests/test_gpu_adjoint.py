@@ -11,7 +11,9 @@ Bars:
   * the kernel's gradients equal the dense adjoint oracle's at the same
     (x*, y*) to 1e-8 relative;
   * the objective's closed forms: dobj/dc = x*, dobj/db = -y*,
-    dobj/dA = y* x*' (to the solve's 1e-6 tolerance).
+    dobj/dA = y* x*';
+  * active-set polishing recovers the exact vertex (x*, y*) of every
+    instance to 1e-9 relative (the FP64 dense refinement of the HiGHS basis).
 """
 import json
 import os
@@ -22,6 +24,7 @@ import torch
 
 from oracle.adjoint import adjoint as oracle_adjoint
 from tests.golden.make_adjoint_fixtures import direction, instance, weights
+from tests.highs_util import highs_xy
 
 pytestmark = pytest.mark.gpu
 
@@ -56,6 +59,7 @@ def test_gradients_match_finite_differences(fixtures, layer):
     x, y, obj = layer.solve_eq(c, b, ax)
     sol = layer.last_solution
     assert sol.solved == len(layer.lps), sol.status
+    assert layer.last_polish_info["accepted"].all(), layer.last_polish_info
     w = torch.from_numpy(np.concatenate(
         [weights(e["index"], e["n"]) for e in fixtures["instances"]])).float()
     loss = (w * x).sum() + 0.5 * (y * y).sum()
@@ -104,13 +108,13 @@ def test_objective_closed_forms(layer):
     xs = x.detach().numpy()
     ys = y.detach().numpy()
     assert np.array_equal(c.grad.numpy(), xs)
-    assert _rel(rmin.grad.numpy(), -ys) <= 1e-4
+    assert _rel(rmin.grad.numpy(), -ys) <= 1e-8
     for i, lp in enumerate(layer.lps[:8]):
         zs = slice(int(layer.zoff[i]), int(layer.zoff[i + 1]))
         xi = layer.split(xs, "n")[i]
         yi = layer.split(ys, "m")[i]
         rows = np.repeat(np.arange(lp.m), np.diff(lp.ap))
-        assert _rel(ax.grad.numpy()[zs], yi[rows] * xi[lp.ai]) <= 1e-4, i
+        assert _rel(ax.grad.numpy()[zs], yi[rows] * xi[lp.ai]) <= 1e-8, i
 
 
 def test_stale_forward_is_solved_again(layer):
@@ -122,3 +126,18 @@ def test_stale_forward_is_solved_again(layer):
     x1.sum().backward()            # must differentiate the first problem
     ref = layer.solver.adjoint(gx=np.ones(int(layer.coff[-1])))
     assert np.allclose(c1.grad.numpy(), ref["gc"])
+
+
+def test_polish_recovers_exact_vertex(layer):
+    c, rmin, rmax, ax = (t.double().numpy() for t in layer.data(torch.float64))
+    layer.solver.update(c=c, rmin=rmin, rmax=rmax, ax=ax)
+    sol = layer.solver.run(tolerance=1e-6, time_limit=120)
+    xs, ys, obj, info = layer.solver.polish(tolerance=1e-12)
+    assert info["accepted"].all()
+    for i, lp in enumerate(layer.lps[:6]):
+        xr, yr, objr = highs_xy(lp)
+        assert _rel(xs[i], xr) <= 1e-9, i
+        assert _rel(ys[i], yr) <= 1e-9, i
+        assert abs(obj[i] - objr) <= 1e-12 * (1 + abs(objr)), i
+        # the unpolished PDHG pair is only 1e-6 accurate
+        assert abs(sol.objective[i] - objr) <= 1e-6 * (1 + abs(objr))
