@@ -20,8 +20,9 @@ class BatchResult(C.Structure):
 class AdjointInfo(C.Structure):
     _fields_ = [("basic_cols", C.c_int), ("active_rows", C.c_int),
                 ("iters_p", C.c_int), ("iters_q", C.c_int),
-                ("accepted", C.c_int), ("reserved", C.c_int),
-                ("res_p", C.c_double), ("res_q", C.c_double)]
+                ("accepted", C.c_int), ("flags", C.c_int),
+                ("res_p", C.c_double), ("res_q", C.c_double),
+                ("viol", C.c_double)]
 
 
 SIGNATURES = {

@@ -177,7 +177,8 @@ class BatchSolver:
     def _info(self, info):
         f = lambda k: np.array([getattr(r, k) for r in info])  # noqa
         return {k: f(k) for k in ("basic_cols", "active_rows", "iters_p",
-                                  "iters_q", "accepted", "res_p", "res_q")}
+                                  "iters_q", "accepted", "flags", "res_p",
+                                  "res_q", "viol")}
 
     def polish(self, tolerance=1e-12, max_iterations=None):
         """Active-set polishing of the last run's solutions

@@ -46,12 +46,19 @@ __device__ inline Work work_of(const AdjArrays& J, long co, long ro) {
 }
 
 // Active sets of the pair (x, y) in original units; returns |B| and |R|.
+// A column's score is its distance to the nearer bound over |d~_j| (both
+// scaled); B holds the columns scoring above 1.  At a nondegenerate vertex
+// |B| = |R|; when an approximate solution leaves them unequal, the
+// lowest-scoring columns of B are moved out (or the highest-scoring ones
+// outside moved in), at most 64, one workgroup argmin/argmax at a time.
+// `score` is n doubles of scratch.
 template <typename V>
 __device__ void masks(const LP<V>& L, const double* x, const double* y,
-                      unsigned char* mc, unsigned char* mr, Lds& S, double& nb,
-                      double& nr) {
+                      unsigned char* mc, unsigned char* mr, double* score,
+                      Lds& S, double& nb, double& nr) {
     const double* dr = L.dr();
     const double* dc = L.dc();
+    const double INF = __builtin_huge_val();
     nr = 0.0;
     nb = 0.0;
     spmv_A<double>(L, L.sax(), GatherDiv{x, dc}, S, [&](int i, double a) {
@@ -65,15 +72,34 @@ __device__ void masks(const LP<V>& L, const double* x, const double* y,
     });
     __syncthreads();
     spmv_T<double>(L, L.sacx(), GatherMul{y, dr}, S, [&](int j, double a) {
-        const double d = L.sc()[j] + a;
+        const double d = fabs(L.sc()[j] + a);
         const double xt = x[j] / dc[j];
         const double sl = fmin(xt - L.scl()[j], L.scu()[j] - xt);
-        const bool in = sl > fabs(d);
+        const double sc = d > 0.0 ? sl / d : (sl > 0.0 ? INF : 0.0);
+        score[j] = sc;
+        const bool in = sc > 1.0;
         mc[j] = in;
         nb += in;
     });
     nr = bsum(nr, S.red);
     nb = bsum(nb, S.red);
+    for (int k = 0; k < 64 && nb != nr; ++k) {
+        const bool drop = nb > nr;  // uniform
+        // best candidate: lowest score in B (drop) or highest outside B
+        double best = -INF;
+        for (int j = threadIdx.x; j < L.n; j += NT)
+            if ((mc[j] != 0) == drop) best = fmax(best, drop ? -score[j] : score[j]);
+        best = bmax(best, S.red);
+        if (!(best > -INF)) break;
+        double idx = -INF;  // smallest index attaining it (as -index)
+        for (int j = threadIdx.x; j < L.n; j += NT)
+            if ((mc[j] != 0) == drop && (drop ? -score[j] : score[j]) == best)
+                idx = fmax(idx, -static_cast<double>(j));
+        idx = bmax(idx, S.red);
+        if (threadIdx.x == 0) mc[static_cast<int>(-idx)] = drop ? 0 : 1;
+        nb += drop ? -1.0 : 1.0;
+        __syncthreads();
+    }
 }
 
 // CGLS on both masked systems; w.r1 (m) and w.r2 (n) hold the right-hand
@@ -191,7 +217,6 @@ __global__ __launch_bounds__(NT) void k_polish(const Arrays<V>* A,
                                                const LPDesc* lps,
                                                const AdjArrays* Jp) {
     __shared__ Lds S;
-    __shared__ int bad;
     const AdjArrays& J = *Jp;
     const LPDesc D = lps[blockIdx.x];
     const LP<V> L = view(A, D);
@@ -207,7 +232,8 @@ __global__ __launch_bounds__(NT) void k_polish(const Arrays<V>* A,
     double* xs = J.xs + co;  // scaled candidate x~
     double* ys = J.ys + ro;  // scaled candidate y~
     double nb, nr;
-    masks(L, x, y, mc, mr, S, nb, nr);
+    masks(L, x, y, mc, mr, xs, S, nb, nr);
+    __syncthreads();
     // candidate start: nonbasic columns on their nearer bound, inactive
     // rows' duals zero
     for (int j = tid; j < n; j += NT) {
@@ -217,7 +243,6 @@ __global__ __launch_bounds__(NT) void k_polish(const Arrays<V>* A,
         xs[j] = v;
     }
     for (int i = tid; i < m; i += NT) ys[i] = mr[i] ? y[i] * dr[i] : 0.0;
-    if (tid == 0) bad = 0;
     __syncthreads();
     // right-hand sides: r1 = b~_R - A~ x~ on R, r2 = -(c~ + A~'y~) on B
     spmv_A<double>(L, L.sax(), dev::GatherPlain<double>{xs}, S, [&](int i, double a) {
@@ -236,39 +261,48 @@ __global__ __launch_bounds__(NT) void k_polish(const Arrays<V>* A,
     for (int i = tid; i < m; i += NT)
         if (mr[i]) ys[i] += w.z2[i];
     __syncthreads();
-    // feasibility of the polished pair (original-unit tolerances 1e-9)
+    // feasibility of the polished pair: largest relative violation of each
+    // condition (column bounds, rows, active-row dual signs, d_B = 0,
+    // reduced-cost signs on N)
     double cmax = 0.0;
     for (int j = tid; j < n; j += NT) cmax = fmax(cmax, fabs(L.c()[j]));
     cmax = bmax(cmax, S.red);
-    int nbad = 0;
+    double vb = 0.0, vr = 0.0, vs = 0.0, vd = 0.0, vn = 0.0;
     for (int j = tid; j < n; j += NT) {
         const double xv = xs[j] * dc[j];
         const double lo = L.cl()[j], hi = L.cu()[j];
-        if (xv < lo - 1e-9 * (1.0 + fabs(lo)) || xv > hi + 1e-9 * (1.0 + fabs(hi))) ++nbad;
+        vb = fmax(vb, fmax(lo - xv, 0.0) / (1.0 + fabs(lo)));
+        vb = fmax(vb, fmax(xv - hi, 0.0) / (1.0 + fabs(hi)));
     }
     spmv_A<double>(L, L.sax(), dev::GatherPlain<double>{xs}, S, [&](int i, double a) {
         const double ax = a * dr[i];
         const double lo = L.rmin()[i], hi = L.rmax()[i];
-        if (ax < lo - 1e-9 * (1.0 + fabs(lo)) || ax > hi + 1e-9 * (1.0 + fabs(hi))) ++nbad;
+        vr = fmax(vr, fmax(lo - ax, 0.0) / (1.0 + fabs(lo)));
+        vr = fmax(vr, fmax(ax - hi, 0.0) / (1.0 + fabs(hi)));
         if (mr[i] && lo != hi) {  // an active inequality row's dual sign
             const double yv = ys[i] / dr[i];
-            const bool upper = y[i] >= 0.0;
-            if ((upper && yv < -1e-9) || (!upper && yv > 1e-9)) ++nbad;
+            vs = fmax(vs, y[i] >= 0.0 ? fmax(-yv, 0.0) : fmax(yv, 0.0));
         }
     });
     spmv_T<double>(L, L.sacx(), dev::GatherPlain<double>{ys}, S, [&](int j, double a) {
-        const double d = L.c()[j] + a / dc[j];
-        const double tol = 1e-9 * (1.0 + cmax);
+        const double d = (L.c()[j] + a / dc[j]) / (1.0 + cmax);
         if (mc[j]) {
-            if (fabs(d) > tol) ++nbad;  // an interior column has d_j = 0
+            vd = fmax(vd, fabs(d));  // an interior column has d_j = 0
         } else {
             const bool lower = xs[j] == L.scl()[j];
-            if ((lower && d < -tol) || (!lower && d > tol)) ++nbad;
+            vn = fmax(vn, lower ? fmax(-d, 0.0) : fmax(d, 0.0));
         }
     });
-    if (nbad) atomicAdd(&bad, nbad);
-    __syncthreads();
-    const bool ok = bad == 0 && e1 <= 1e-8 && e2 <= 1e-8;
+    vb = bmax(vb, S.red);
+    vr = bmax(vr, S.red);
+    vs = bmax(vs, S.red);
+    vd = bmax(vd, S.red);
+    vn = bmax(vn, S.red);
+    const double TOLP = 1e-9;
+    const int flags = (vb > TOLP) | (vr > TOLP) << 1 | (vs > TOLP) << 2 |
+                      (vd > TOLP) << 3 | (vn > TOLP) << 4;
+    const double viol = fmax(fmax(vb, vr), fmax(vs, fmax(vd, vn)));
+    const bool ok = flags == 0;
     if (ok) {
         double op = 0.0;
         for (int j = tid; j < n; j += NT) {
@@ -286,8 +320,10 @@ __global__ __launch_bounds__(NT) void k_polish(const Arrays<V>* A,
         inf.iters_p = it1;
         inf.iters_q = it2;
         inf.accepted = ok;
+        inf.flags = flags;
         inf.res_p = e1;
         inf.res_q = e2;
+        inf.viol = viol;
         J.info[blockIdx.x] = inf;
     }
 }
@@ -315,7 +351,7 @@ __global__ __launch_bounds__(NT) void k_adjoint(const Arrays<V>* A,
     const Work w = work_of(J, co, ro);
 
     double nb, nr;
-    masks(L, x, y, mc, mr, S, nb, nr);
+    masks(L, x, y, mc, mr, J.xs + co, S, nb, nr);
     for (int i = tid; i < m; i += NT) w.r1[i] = mr[i] ? gy[i] / dr[i] : 0.0;
     for (int j = tid; j < n; j += NT)
         w.r2[j] = mc[j] ? dc[j] * (gx[j] + gobj * c[j]) : 0.0;

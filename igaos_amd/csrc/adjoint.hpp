@@ -20,7 +20,7 @@
 //   dL/dbnd_j  = [j not in B] (w_j - (A_R' q)_j)   (the bound x_j sits on).
 // Polishing uses the same masks: it solves the active-set equations for
 // corrections of x_B and y_R (x_N on its bounds, y off R zero), and keeps
-// the result only if it is primal and dual feasible to 1e-9.
+// the result only if it is primal and dual feasible to 1e-9 (relative).
 // Both systems are solved per instance by CGLS in FP64 on the Ruiz-scaled
 // matrix (the scaled systems are  A~_RB p~ = D_r^-1 gy,  A~_RB' q~ = D_c w
 // with p = D_c p~, q = q~ / D_r), reusing the batched solver's instance
@@ -36,8 +36,10 @@ namespace igaos::batch {
 struct AdjInfo {
     int basic_cols, active_rows, iters_p, iters_q;
     int accepted;         // polish: the polished pair passed the checks
-    int pad;
+    int flags;            // polish: failed checks (bits: column bounds, rows,
+                          // active-row dual signs, d_B = 0, reduced-cost signs)
     double res_p, res_q;  // final relative residuals of the two systems
+    double viol;          // polish: largest relative violation
 };
 
 struct AdjArrays {

@@ -667,4 +667,6 @@ int igaos_batch_adjoint(igaos_batch* B, const double* gx, const double* gy,
     return 0;
 }
 
+void igaos_batch_destroy(igaos_batch* B) { delete B; }
+
 }  // extern "C"

@@ -230,8 +230,11 @@ typedef struct {
     int basic_cols, active_rows;  /* |B|, |R| */
     int iters_p, iters_q;         /* CGLS iterations of the two systems */
     int accepted;                 /* polish: the polished pair was kept */
-    int reserved;
+    int flags;                    /* polish: failed checks, bit 0 column
+                                     bounds, 1 rows, 2 active-row dual
+                                     signs, 3 d_B = 0, 4 reduced-cost signs */
     double res_p, res_q;          /* final relative residuals */
+    double viol;                  /* polish: largest relative violation */
 } igaos_adjoint_info;
 /* Active-set polishing of the last run's solutions (no reference
  * counterpart): with the same masks, x_N is put on its bounds, y off R to

@@ -55,10 +55,26 @@ def masks(lp, x, y, dr=None, dc=None):
     with np.errstate(invalid="ignore"):
         slr = np.minimum(at - lp.rmin / dr, lp.rmax / dr - at)
     R = eq | (np.abs(yt) > slr)
-    dt = lp.c * dc + dc * (A.T @ (yt / dr))
-    with np.errstate(invalid="ignore"):
+    dt = np.abs(lp.c * dc + dc * (A.T @ (yt / dr)))
+    with np.errstate(invalid="ignore", divide="ignore"):
         slc = np.minimum(xt - lp.cl / dc, lp.cu / dc - xt)
-    B = slc > np.abs(dt)
+        score = np.where(dt > 0, slc / np.where(dt > 0, dt, 1.0),
+                         np.where(slc > 0, np.inf, 0.0))
+    B = score > 1.0
+    # a nondegenerate vertex has |B| = |R|: move the lowest-scoring columns
+    # out of B (or the highest-scoring ones in), smallest index on ties
+    for _ in range(64):
+        nb, nr = int(B.sum()), int(R.sum())
+        if nb == nr:
+            break
+        if nb > nr:
+            cand = np.flatnonzero(B)
+            j = cand[np.argmin(score[cand])]
+            B[j] = False
+        else:
+            cand = np.flatnonzero(~B)
+            j = cand[np.argmax(score[cand])]
+            B[j] = True
     return B, R
 
 

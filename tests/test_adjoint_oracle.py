@@ -157,3 +157,22 @@ def test_committed_fixtures_first_instances():
             u = direction(ent["index"], name, rec["attempt"], grads[name].size)
             an = grads[name] @ u
             assert abs(an - rec["value"]) <= 1e-5 * max(1.0, abs(rec["value"])), name
+
+
+def test_masks_balance_a_misclassified_column():
+    # an approximate solution whose nonbasic column sits off its bound by
+    # more than its reduced cost: the raw rule puts it in B, the count
+    # balance (|B| = |R| at a nondegenerate vertex) takes it out again
+    from oracle import ruiz
+    lp = _small(1)
+    x, y, _ = highs_xy(lp)
+    B0 = basis(lp, x)
+    dr, dc = ruiz(lp)
+    A = lp.to_scipy()
+    dt = np.abs(lp.c + A.T @ y) * dc          # |d~|
+    j = np.flatnonzero(~B0)[np.argmin(dt[~B0])]
+    xs = x.copy()
+    step = 1.5 * dt[j] * dc[j]                # scaled distance 1.5 |d~_j|
+    xs[j] = lp.cl[j] + step if x[j] <= lp.cl[j] + 1e-12 else lp.cu[j] - step
+    B, R = masks(lp, xs, y)
+    assert (B == B0).all() and B.sum() == R.sum()
