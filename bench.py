@@ -111,13 +111,19 @@ def bench_batch(args):
         solved += sol.solved
         log(f"[bench] step {k}: {sol.solved}/{count} solved, launch "
             f"{sol.kernel_ms:.0f} ms")
+    lk = b.lockstep_timing() if args.mode == "lockstep" else None
     b.close()
     value = solved / elapsed
     its = np.array(sol.iterations)
     att = np.array(sol.attempts)
     alg = float((att * bytes_per_iteration(ms, ns, nnz, 4, 4)).sum())
     launch_ms = kernel_ms / args.steps
-    achieved = alg / (launch_ms * 1e-3) / 1e9
+    achieved_run = alg / (launch_ms * 1e-3) / 1e9
+    if lk is not None and lk["samples"] > 0:
+        # dominant kernel: the dual attempt kernel of the sampled rounds
+        achieved = lk["dual_bytes"] / (lk["dual_ms"] * 1e-3) / 1e9
+    else:
+        achieved = achieved_run
 
     # plain FP64 single-LP engine, instance by instance, on a fixed sample
     sample = list(range(0, count, max(1, count // args.fp64_sample)))
@@ -182,14 +188,16 @@ def bench_batch(args):
         "roofline": {"bound": "hbm",
                      "kernel": ("k_batch<float> (one persistent launch)"
                                 if args.mode == "persistent" else
-                                "lockstep run: k_lk_primal/k_lk_dual/k_lk_decide "
-                                "rounds + k_lk_check, HIP events around the run"),
+                                "k_lk_dual<float> (HIP events around it in sampled "
+                                "lockstep rounds)"),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": load_traffic("k_batch_batch8192" if args.mode == "persistent"
                                              else "lockstep_batch8192"),
-                     "bytes_per_launch": alg,
-                     "ms_per_launch": round(launch_ms, 3)},
+                     "bytes_per_launch": alg if lk is None else lk["dual_bytes"],
+                     "ms_per_launch": round(launch_ms if lk is None else lk["dual_ms"], 4),
+                     "lockstep_samples": None if lk is None else lk,
+                     "whole_run_gbs": round(achieved_run, 1)},
         "cpu_baseline": cpu,
     }
 

@@ -41,7 +41,10 @@ SIGNATURES = {
     "igaos_batch_destroy": (None, [_vp]),
     "igaos_batch_set_mode": (C.c_int, [_vp, C.c_int]),
     "igaos_batch_get_mode": (C.c_int, [_vp]),
+    "igaos_batch_lockstep_timing": (C.c_int, [_vp, _dp, _dp, _dp, _dp,
+                                              C.POINTER(C.c_long)]),
     "igaos_batch_update": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "igaos_batch_set_solution": (C.c_int, [_vp, _dp, _dp]),
     "igaos_batch_polish": (C.c_int, [_vp, C.c_double, C.c_int, _dp, _dp, _dp,
                                      C.POINTER(AdjointInfo)]),
     "igaos_batch_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_int,

@@ -66,7 +66,7 @@ class DiffLPBatch:
     """Batched LP layer: patterns and column bounds from ``lps`` (a list of
     ``igaos_amd.lpgen.LP``); numeric data per call."""
 
-    def __init__(self, lps, precision=32, tolerance=1e-6, time_limit=60.0,
+    def __init__(self, lps, precision=32, tolerance=1e-8, time_limit=60.0,
                  max_iterations=500000, adjoint_tolerance=1e-12,
                  adjoint_max_iterations=None, polish=True, mode="persistent"):
         self.lps = list(lps)
@@ -96,6 +96,33 @@ class DiffLPBatch:
             xs, ys, obj, info = self.solver.polish(
                 tolerance=self.adjoint_tolerance,
                 max_iterations=self.adjoint_max_iterations)
+            rej = np.flatnonzero(info["accepted"] == 0)
+            if rej.size:
+                # the active set was misread at this tolerance: solve those
+                # instances again 100x tighter and polish once more
+                cat = lambda v, off, i: v[int(off[i]):int(off[i + 1])]  # noqa
+                sub_lps = [self.lps[i].with_values(
+                    c=cat(c, self.coff, i), rmin=cat(rmin, self.roff, i),
+                    rmax=cat(rmax, self.roff, i), ax=cat(ax, self.zoff, i))
+                    for i in rej]
+                sub = BatchSolver(sub_lps, precision=self.solver.precision,
+                                  mode=self.solver.mode)
+                try:
+                    sub.run(tolerance=max(self.tolerance * 1e-2, 1e-11),
+                            time_limit=self.time_limit,
+                            max_iterations=self.max_iterations, want_xy=False)
+                    sx, sy, sobj, sinfo = sub.polish(
+                        tolerance=self.adjoint_tolerance,
+                        max_iterations=self.adjoint_max_iterations)
+                finally:
+                    sub.close()
+                xs, ys, obj = list(xs), list(ys), np.array(obj)
+                for k, i in enumerate(rej):
+                    if sinfo["accepted"][k]:
+                        xs[i], ys[i], obj[i] = sx[k], sy[k], sobj[k]
+                        for key in info:
+                            info[key][i] = sinfo[key][k]
+                self.solver.set_solution(xs, ys)
             self.last_polish_info = info
         self._x = np.concatenate(xs) if xs else np.zeros(0)
         self._y = np.concatenate(ys) if ys else np.zeros(0)

@@ -174,6 +174,29 @@ class BatchSolver:
         if rc != 0:
             raise RuntimeError(L.igaos_batch_last_error().decode())
 
+    def set_solution(self, x=None, y=None):
+        """Replace the last run's solutions (concatenated) that polish and
+        adjoint start from (igaos_batch_set_solution)."""
+        L = _core.lib()
+        conv = lambda v: None if v is None else np.ascontiguousarray(np.concatenate(v) if isinstance(v, list) else v, np.float64)  # noqa
+        xa, ya = conv(x), conv(y)
+        dp = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+        if L.igaos_batch_set_solution(self._h, dp(xa), dp(ya)) != 0:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+
+    def lockstep_timing(self) -> dict:
+        """HIP-event timing of sampled lockstep rounds of the last run
+        (igaos_batch_lockstep_timing): mean ms and algorithmic bytes per
+        launch of the primal and dual attempt kernels."""
+        L = _core.lib()
+        v = [C.c_double() for _ in range(4)]
+        n = C.c_long()
+        L.igaos_batch_lockstep_timing(self._h, *[C.byref(t) for t in v], C.byref(n))
+        k = max(1, n.value)
+        return {"samples": n.value,
+                "primal_ms": v[0].value / k, "primal_bytes": v[1].value / k,
+                "dual_ms": v[2].value / k, "dual_bytes": v[3].value / k}
+
     def _info(self, info):
         f = lambda k: np.array([getattr(r, k) for r in info])  # noqa
         return {k: f(k) for k in ("basic_cols", "active_rows", "iters_p",

@@ -843,10 +843,13 @@ template <typename V>
 static void lk_round_impl(const Arrays<V>* a, const LPDesc* lps, const int2* tblT,
                           int nbT, const int2* tblA, int nbA, const LsRange* rng,
                           int nact, void* ctl, double* partT, double* partA,
-                          int* nfrozen, hipStream_t s) {
+                          int* nfrozen, hipEvent_t* ev, hipStream_t s) {
     Ctl* c = static_cast<Ctl*>(ctl);
+    if (ev) (void)hipEventRecord(ev[0], s);
     if (nbT > 0) k_lk_primal<V><<<nbT, NT, 0, s>>>(a, lps, tblT, c, partT);
+    if (ev) (void)hipEventRecord(ev[1], s);
     if (nbA > 0) k_lk_dual<V><<<nbA, NT, 0, s>>>(a, lps, tblA, c, partA);
+    if (ev) (void)hipEventRecord(ev[2], s);
     const int per = NT / WAVE;
     k_lk_decide<<<(nact + per - 1) / per, NT, 0, s>>>(rng, nact, c, partT, partA, nfrozen);
 }
@@ -863,14 +866,14 @@ void lockstep_setup(const Arrays<double>* a, const Params& p, void* ctl, int* do
 void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const int2* tblT,
                     int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
-                    hipStream_t s) {
-    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, s);
+                    hipEvent_t* ev, hipStream_t s) {
+    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, ev, s);
 }
 void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const int2* tblT,
                     int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
-                    hipStream_t s) {
-    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, s);
+                    hipEvent_t* ev, hipStream_t s) {
+    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, ev, s);
 }
 void lockstep_check(const Arrays<float>* a, const Params& p, const LsRange* rng,
                     int nact, void* ctl, int* done, hipStream_t s) {

@@ -81,7 +81,9 @@ int batch_occupancy(bool f32, int waves);
 // Lockstep mode (batch.hip): per-instance control blocks live in device
 // memory (lockstep_ctl_bytes() each); the host keeps tables of the active
 // instances' row blocks as (instance, local block) pairs and one LsRange
-// per active instance.
+// per active instance.  lockstep_round records ev[0..2] (if not null)
+// before the primal kernel, between the two attempt kernels and after the
+// dual kernel.
 struct LsRange {
     int lp;        // instance
     int bT0, nT;   // its blocks in the A' table
@@ -95,11 +97,11 @@ void lockstep_setup(const Arrays<double>* a, const Params& p, void* ctl, int* do
 void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const int2* tblT,
                     int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
-                    hipStream_t s);
+                    hipEvent_t* ev, hipStream_t s);
 void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const int2* tblT,
                     int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
-                    hipStream_t s);
+                    hipEvent_t* ev, hipStream_t s);
 void lockstep_check(const Arrays<float>* a, const Params& p, const LsRange* rng,
                     int nact, void* ctl, int* done, hipStream_t s);
 void lockstep_check(const Arrays<double>* a, const Params& p, const LsRange* rng,

@@ -94,11 +94,21 @@ struct igaos_batch {
     double* d_partA = nullptr;
     int* d_done = nullptr;
     int* d_nfrozen = nullptr;
+    // lockstep timing samples: rounds 0, 16, 32, 48 of each check period
+    // (every active instance works in those), HIP events around the two
+    // attempt kernels, with their algorithmic bytes
+    hipEvent_t lk_ev[4][3] = {};
+    long lk_samples = 0;
+    double lk_primal_ms = 0.0, lk_dual_ms = 0.0;
+    double lk_primal_bytes = 0.0, lk_dual_bytes = 0.0;
     Mem amem;             // adjoint workspace, allocated on first use
     igaos::batch::AdjArrays adj{};
     igaos::batch::AdjArrays* d_adj = nullptr;
     ~igaos_batch() {
         if (worker.joinable()) worker.join();
+        for (auto& t : lk_ev)
+            for (hipEvent_t e : t)
+                if (e) (void)hipEventDestroy(e);
         if (running && e1) (void)hipEventSynchronize(e1);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -215,6 +225,14 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
         }
         const int nact = static_cast<int>(active.size());
         const int nbT = static_cast<int>(tT.size()), nbA = static_cast<int>(tA.size());
+        // algorithmic bytes of one round's attempt kernels over the active set
+        const double wa = sizeof(V), wv = sizeof(V);
+        double bp = 0.0, bd = 0.0;
+        for (int lp : active) {
+            const LPDesc& d = B->desc[lp];
+            bp += d.nnz * (wa + 4) + 4.0 * (d.n + 1) + 7.0 * d.n * wv;
+            bd += d.nnz * (wa + 4) + 4.0 * (d.m + 1) + 8.0 * d.m * wv;
+        }
         BCK(hipMemcpyAsync(B->d_tblT, tT.data(), sizeof(int2) * nbT, hipMemcpyHostToDevice, s));
         BCK(hipMemcpyAsync(B->d_tblA, tA.data(), sizeof(int2) * nbA, hipMemcpyHostToDevice, s));
         BCK(hipMemcpyAsync(B->d_rng, rng.data(), sizeof(LsRange) * nact,
@@ -223,15 +241,30 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
         // attempt rounds until every active instance froze on its check
         // boundary (64 accepted iterations, plus retries)
         int chunk = igaos::pdhg::CHECK_EVERY;
+        bool first = true;
         for (;;) {
-            for (int r = 0; r < chunk; ++r)
+            int ns = 0;
+            for (int r = 0; r < chunk; ++r) {
+                hipEvent_t* ev = (first && r % 16 == 0 && ns < 4) ? B->lk_ev[ns++] : nullptr;
                 igaos::batch::lockstep_round(A, B->d_desc, B->d_tblT, nbT, B->d_tblA, nbA,
                                              B->d_rng, nact, B->d_ctl, B->d_partT,
-                                             B->d_partA, B->d_nfrozen, s);
+                                             B->d_partA, B->d_nfrozen, ev, s);
+            }
             BCK(hipGetLastError());
             int nf = 0;
             BCK(hipMemcpyAsync(&nf, B->d_nfrozen, sizeof(int), hipMemcpyDeviceToHost, s));
             BCK(hipStreamSynchronize(s));
+            for (int k = 0; k < ns; ++k) {
+                float a = 0.0f, b = 0.0f;
+                BCK(hipEventElapsedTime(&a, B->lk_ev[k][0], B->lk_ev[k][1]));
+                BCK(hipEventElapsedTime(&b, B->lk_ev[k][1], B->lk_ev[k][2]));
+                B->lk_primal_ms += a;
+                B->lk_dual_ms += b;
+                B->lk_primal_bytes += bp;
+                B->lk_dual_bytes += bd;
+                B->lk_samples += 1;
+            }
+            first = false;
             if (nf >= nact) break;
             chunk = 4;
         }
@@ -252,6 +285,8 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
 
 void lockstep_alloc(igaos_batch* B) {
     if (B->d_ctl) return;
+    for (auto& t : B->lk_ev)
+        for (hipEvent_t& e : t) BCK(hipEventCreate(&e));
     Mem& M = B->lkmem;
     long bt = 0, ba = 0;
     for (const LPDesc& d : B->desc) {
@@ -283,6 +318,18 @@ int igaos_batch_set_mode(igaos_batch* B, int mode) {
 }
 
 int igaos_batch_get_mode(const igaos_batch* B) { return B ? B->mode : -1; }
+
+int igaos_batch_lockstep_timing(const igaos_batch* B, double* primal_ms,
+                                double* primal_bytes, double* dual_ms,
+                                double* dual_bytes, long* samples) {
+    if (!B) return bfail("null batch");
+    if (primal_ms) *primal_ms = B->lk_primal_ms;
+    if (primal_bytes) *primal_bytes = B->lk_primal_bytes;
+    if (dual_ms) *dual_ms = B->lk_dual_ms;
+    if (dual_bytes) *dual_bytes = B->lk_dual_bytes;
+    if (samples) *samples = B->lk_samples;
+    return 0;
+}
 
 igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
                                 const int* nnz, const double* c,
@@ -444,6 +491,9 @@ int igaos_batch_launch(igaos_batch* B, const igaos_options* opt) {
             lockstep_alloc(B);
             P.progress = nullptr;
             B->lk_done = 0;
+            B->lk_samples = 0;
+            B->lk_primal_ms = B->lk_dual_ms = 0.0;
+            B->lk_primal_bytes = B->lk_dual_bytes = 0.0;
             B->worker_err.clear();
             B->running = true;
             B->worker = std::thread([B, P]() {
@@ -581,6 +631,21 @@ int check_active_set_call(igaos_batch* B, double tolerance, int max_iterations) 
 }
 
 }  // namespace
+
+int igaos_batch_set_solution(igaos_batch* B, const double* x, const double* y) {
+    if (!B) return bfail("null batch");
+    if (B->running) return bfail("batch is running");
+    if (!B->solved) return bfail("needs a finished run on the current data");
+    try {
+        double* xo = B->precision == 32 ? B->a32.xout : B->a64.xout;
+        double* yo = B->precision == 32 ? B->a32.yout : B->a64.yout;
+        if (x) BCK(hipMemcpy(xo, x, sizeof(double) * B->sum_n, hipMemcpyHostToDevice));
+        if (y) BCK(hipMemcpy(yo, y, sizeof(double) * B->sum_m, hipMemcpyHostToDevice));
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
 
 int igaos_batch_polish(igaos_batch* B, double tolerance, int max_iterations,
                        double* x, double* y, double* objective,

@@ -198,6 +198,15 @@ int igaos_batch_get_schedule(const igaos_batch* batch, int* tpr, int* grid,
  * thread that drives the rounds; progress and wait work as in mode 0. */
 int igaos_batch_set_mode(igaos_batch* batch, int mode);
 int igaos_batch_get_mode(const igaos_batch* batch);
+/* Lockstep timing of the last run: rounds 0, 16, 32 and 48 of every check
+ * period (all active instances work in them) are timed with HIP events
+ * around the primal and the dual attempt kernel; returns the summed
+ * milliseconds and algorithmic bytes (k_primal: nnz (w_a + 4) + 4 (n + 1)
+ * + 7 n w, k_dual: nnz (w_a + 4) + 4 (m + 1) + 8 m w, summed over the
+ * active instances) and the number of sampled rounds. */
+int igaos_batch_lockstep_timing(const igaos_batch* batch, double* primal_ms,
+                                double* primal_bytes, double* dual_ms,
+                                double* dual_bytes, long* samples);
 /* Device time of the last run (HIP events around all its launches), ms. */
 double igaos_batch_kernel_ms(const igaos_batch* batch);
 /* Replaces the batch's numeric data in place (same sparsity pattern and
@@ -244,6 +253,10 @@ typedef struct {
  * it is primal feasible, has d_B = 0 and correct reduced-cost and dual
  * signs, all to 1e-9 relative (info.accepted).  x, y, objective (count)
  * receive the resulting solutions; any may be NULL. */
+/* Replaces the solutions of the last run (host x: sum n, y: sum m; NULL
+ * keeps one) that igaos_batch_polish and igaos_batch_adjoint start from,
+ * e.g. with solutions of the same data obtained elsewhere. */
+int igaos_batch_set_solution(igaos_batch* batch, const double* x, const double* y);
 int igaos_batch_polish(igaos_batch* batch, double tolerance, int max_iterations,
                        double* x, double* y, double* objective,
                        igaos_adjoint_info* info);

@@ -41,7 +41,7 @@ def fixtures():
 def layer(fixtures):
     from igaos_amd.autograd import DiffLPBatch
     lps = [instance(e["index"]) for e in fixtures["instances"]]
-    prob = DiffLPBatch(lps, precision=32, tolerance=1e-6, time_limit=120)
+    prob = DiffLPBatch(lps, precision=32, tolerance=1e-8, time_limit=120)
     yield prob
     prob.close()
 
@@ -131,13 +131,27 @@ def test_stale_forward_is_solved_again(layer):
 def test_polish_recovers_exact_vertex(layer):
     c, rmin, rmax, ax = (t.double().numpy() for t in layer.data(torch.float64))
     layer.solver.update(c=c, rmin=rmin, rmax=rmax, ax=ax)
-    sol = layer.solver.run(tolerance=1e-6, time_limit=120)
+    sol = layer.solver.run(tolerance=1e-8, time_limit=120)
     xs, ys, obj, info = layer.solver.polish(tolerance=1e-12)
-    assert info["accepted"].all()
+    assert info["accepted"].all(), info
+    assert info["viol"].max() <= 1e-12
     for i, lp in enumerate(layer.lps[:6]):
         xr, yr, objr = highs_xy(lp)
         assert _rel(xs[i], xr) <= 1e-9, i
         assert _rel(ys[i], yr) <= 1e-9, i
         assert abs(obj[i] - objr) <= 1e-12 * (1 + abs(objr)), i
-        # the unpolished PDHG pair is only 1e-6 accurate
-        assert abs(sol.objective[i] - objr) <= 1e-6 * (1 + abs(objr))
+        # the unpolished PDHG pair meets the solve tolerance only
+        assert abs(sol.objective[i] - objr) <= 1e-8 * (1 + abs(objr))
+
+
+def test_rejected_polish_is_solved_again(layer):
+    # at 1e-6 some instance's active set is misread (a swapped pair); the
+    # forward solves it again 100x tighter before polishing
+    c, rmin, rmax, ax = layer.data(torch.float64)
+    tol = layer.tolerance
+    layer.tolerance = 1e-6
+    try:
+        x, y, obj = layer.solve_eq(c, rmin, ax)
+    finally:
+        layer.tolerance = tol
+    assert layer.last_polish_info["accepted"].all(), layer.last_polish_info
