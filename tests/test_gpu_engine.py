@@ -237,3 +237,25 @@ def test_cli_pdhg_json():
     assert js["status"] == "near-optimal" and js["engine"] == "pdhg"
     assert abs(js["objective"] + 464.7531429) < 1e-3
     assert len(js["x"]) == 32 and len(js["y"]) == 27
+
+
+def test_column_locality_order_keeps_parity():
+    # n >= 65536, m > 4096 and Zipf column lengths: the engine renumbers
+    # the columns by length internally; the trajectory and the returned x
+    # (in the model's own column order) still follow the oracle
+    lp = lpgen.gen_zipf_badly_scaled(seed=3, m=40_000, n=100_000)
+    lens = np.diff(lp.cp)
+    top = np.sort(lens)[::-1][: lp.n // 50].sum()
+    assert top * 4 >= lp.nnz  # the engine's criterion holds for this LP
+    K = 64
+    r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
+    s = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
+    assert s.iterations == r.iterations == K
+    assert s.info["attempts"] == r.attempts
+    assert _rel(s.x, r.x) <= 1e-9 and _rel(s.y, r.y) <= 1e-9
+    s32 = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K,
+                             precision=32)
+    # the FP32 engine's x comes back in the model's order too: the
+    # column-specific upper bounds (U[1, 100]) hold
+    assert s32.iterations == K
+    assert np.all(s32.x >= lp.cl - 1e-6) and np.all(s32.x <= lp.cu * (1 + 1e-6) + 1e-6)
