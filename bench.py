@@ -203,6 +203,63 @@ def bench_batch(args):
 
 
 # ----------------------------------------------------------------- zipf --
+# ------------------------------------------------------------------ diff --
+def bench_diff(args):
+    """Differentiable batch: 512 LPs (m log-uniform [256, 1024], n = 2m,
+    6 nnz/col), FP32 c, b, A values as requires_grad tensors, loss
+    sum(w.x*) + 0.5 |y*|^2; a step is forward (batched solve to the layer's
+    tolerance + active-set polish) and backward (batched adjoint)."""
+    import numpy as np
+    import torch
+    from igaos_amd import lpgen
+    from igaos_amd.autograd import DiffLPBatch
+
+    count = 512
+    t = time.perf_counter()
+    lps = []
+    for i in range(count):
+        lp = lpgen.gen_batch_instance(0, i, m_lo=256, m_hi=1024, ratio_lo=2.0,
+                                      ratio_hi=2.0, k_lo=6, k_hi=6)
+        lps.append(lpgen.perturb_nondegenerate(lp, seed=i, scale=1e-6))
+    log(f"[bench] {count} LPs generated in {time.perf_counter() - t:.1f} s")
+    layer = DiffLPBatch(lps, precision=32, mode=args.mode, time_limit=args.time_limit)
+    c, rmin, rmax, ax = layer.data(torch.float32)
+    w = torch.from_numpy(np.random.default_rng(7).standard_normal(c.numel())).float()
+    fwd = bwd = 0.0
+    for k in range(args.warmup + args.steps):
+        cc, bb, aa = (v.clone().requires_grad_(True) for v in (c, rmin, ax))
+        t0 = time.perf_counter()
+        x, y, obj = layer.solve_eq(cc, bb, aa)
+        t1 = time.perf_counter()
+        loss = (w * x).sum() + 0.5 * (y * y).sum()
+        loss.backward()
+        t2 = time.perf_counter()
+        if k >= args.warmup:
+            fwd += t1 - t0
+            bwd += t2 - t1
+        log(f"[bench] step {k}: forward {t1 - t0:.2f} s backward {t2 - t1:.2f} s")
+    sol = layer.last_solution
+    pinfo = layer.last_polish_info
+    ainfo = layer.last_adjoint_info
+    layer.close()
+    el = fwd + bwd
+    return {
+        "metric": "differentiable batch: LPs/s (forward solve + polish, backward adjoint)",
+        "value": count * args.steps / el, "unit": "LPs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el * 1000.0 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64refine",
+        "data": "synthetic",
+        "config": {"workload": "512 LPs, m log-uniform [256,1024], n=2m, 6 nnz/col, "
+                               "nondegenerate perturbation 1e-6, seed 0",
+                   "tolerance": layer.tolerance, "mode": args.mode},
+        "forward_s": fwd / args.steps, "backward_s": bwd / args.steps,
+        "certified": sol.solved, "polished": int(np.sum(pinfo["accepted"])),
+        "adjoint_cgls_iterations_max": int(max(ainfo["iters_p"].max(), ainfo["iters_q"].max())),
+        "adjoint_residual_max": float(max(ainfo["res_p"].max(), ainfo["res_q"].max())),
+    }
+
+
 def bench_zipf(args):
     import numpy as np
     import igaos_amd
@@ -284,7 +341,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--workload", default="batch", choices=["batch", "zipf"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "zipf", "diff"])
     ap.add_argument("--count", type=int, default=8192)
     ap.add_argument("--mode", default="lockstep", choices=["lockstep", "persistent"],
                     help="batched execution mode (igaos_batch_set_mode)")
@@ -314,7 +371,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    line = bench_batch(args) if args.workload == "batch" else bench_zipf(args)
+    line = {"batch": bench_batch, "zipf": bench_zipf, "diff": bench_diff}[args.workload](args)
     print(json.dumps(line), flush=True)
 
 

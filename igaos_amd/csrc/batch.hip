@@ -640,6 +640,33 @@ int batch_occupancy(bool f32, int waves) {
 // reductions run in a fixed order over the instance's own blocks, so every
 // result is independent of the rest of the batch.
 
+// XCD-aware work mapping.  Workgroups are dispatched to the 8 XCDs round
+// robin (workgroup b runs on XCD b % 8), and each XCD has its own L2.  The
+// tables list an instance's blocks consecutively, so the identity mapping
+// would spread every instance over all XCDs and each XCD would fetch the
+// instance's gathered vector on its own.  This bijection of [0, n) gives
+// XCD x the contiguous range of table entries [x q + min(x, r), ...) of
+// length q + (x < r), with q = n / 8, r = n % 8.
+constexpr int XCDS = 8;
+__device__ __forceinline__ int xcd_entry(int b, int n) {
+    const int q = n / XCDS, r = n % XCDS;
+    const int x = b % XCDS, slot = b / XCDS;
+    return x * q + min(x, r) + slot;
+}
+
+// the column indices of the attempt kernels: int or the 16-bit copy
+template <typename I>
+__device__ __forceinline__ const I* idx_of(const int* i32, const unsigned short* i16);
+template <>
+__device__ __forceinline__ const int* idx_of<int>(const int* i32, const unsigned short*) {
+    return i32;
+}
+template <>
+__device__ __forceinline__ const unsigned short* idx_of<unsigned short>(
+    const int*, const unsigned short* i16) {
+    return i16;
+}
+
 template <typename V>
 __global__ __launch_bounds__(NT) void k_lk_setup(const Arrays<V>* A, Params P,
                                                  Ctl* ctl, int* done) {
@@ -661,16 +688,17 @@ __global__ __launch_bounds__(NT) void k_lk_setup(const Arrays<V>* A, Params P,
     }
 }
 
-template <typename V>
+template <typename V, typename I>
 __global__ __launch_bounds__(NT) void k_lk_primal(const Arrays<V>* A,
                                                   const LPDesc* lps,
                                                   const int2* tbl, Ctl* ctl,
                                                   double* part) {
     __shared__ Lds S;
-    const int2 e = tbl[blockIdx.x];
+    const int te = xcd_entry(blockIdx.x, gridDim.x);
+    const int2 e = tbl[te];
     const LsState st = ctl[e.x].st;
     if (st.frozen) {
-        if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        if (threadIdx.x == 0) part[te] = 0.0;
         return;
     }
     const LP<V> L = view(A, lps[e.x]);
@@ -686,7 +714,8 @@ __global__ __launch_bounds__(NT) void k_lk_primal(const Arrays<V>* A,
     const int r0 = rb[0], p0 = rb[1], r1 = rb[2], p1 = rb[3];
     V* rows = reinterpret_cast<V*>(S.rows);
     if (fresh)
-        dev::rowblock_sums<V>(L.cp(), L.ci(), L.facx(), dev::GatherPlain<V>{y}, r0, r1,
+        dev::rowblock_sums<V>(L.cp(), idx_of<I>(L.ci(), L.ci16()), L.facx(),
+                              dev::GatherPlain<V>{y}, r0, r1,
                               p0, p1, rows, reinterpret_cast<V*>(S.prod),
                               reinterpret_cast<V*>(S.red));
     double acc = 0.0;
@@ -712,19 +741,20 @@ __global__ __launch_bounds__(NT) void k_lk_primal(const Arrays<V>* A,
         acc += static_cast<double>(d) * static_cast<double>(d);
     }
     acc = bsum(acc, S.red);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[te] = acc;
 }
 
-template <typename V>
+template <typename V, typename I>
 __global__ __launch_bounds__(NT) void k_lk_dual(const Arrays<V>* A,
                                                 const LPDesc* lps,
                                                 const int2* tbl, Ctl* ctl,
                                                 double* part) {
     __shared__ Lds S;
-    const int2 e = tbl[blockIdx.x];
+    const int te = xcd_entry(blockIdx.x, gridDim.x);
+    const int2 e = tbl[te];
     const LsState st = ctl[e.x].st;
     if (st.frozen) {
-        if (threadIdx.x == 0) part[2 * blockIdx.x] = part[2 * blockIdx.x + 1] = 0.0;
+        if (threadIdx.x == 0) part[2 * te] = part[2 * te + 1] = 0.0;
         return;
     }
     const LP<V> L = view(A, lps[e.x]);
@@ -737,7 +767,8 @@ __global__ __launch_bounds__(NT) void k_lk_dual(const Arrays<V>* A,
     const int* rb = L.rbA() + 2 * e.y;
     const int r0 = rb[0], p0 = rb[1], r1 = rb[2], p1 = rb[3];
     V* rows = reinterpret_cast<V*>(S.rows);
-    dev::rowblock_sums<V>(L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.xbar()}, r0,
+    dev::rowblock_sums<V>(L.ap(), idx_of<I>(L.ai(), L.ai16()), L.fax(),
+                          dev::GatherPlain<V>{L.xbar()}, r0,
                           r1, p0, p1, rows, reinterpret_cast<V*>(S.prod),
                           reinterpret_cast<V*>(S.red));
     const V* yoff = L.yoff();
@@ -767,8 +798,8 @@ __global__ __launch_bounds__(NT) void k_lk_dual(const Arrays<V>* A,
     a1 = bsum(a1, S.red);
     a2 = bsum(a2, S.red);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = a1;
-        part[2 * blockIdx.x + 1] = a2;
+        part[2 * te] = a1;
+        part[2 * te + 1] = a2;
     }
 }
 
@@ -843,12 +874,18 @@ template <typename V>
 static void lk_round_impl(const Arrays<V>* a, const LPDesc* lps, const int2* tblT,
                           int nbT, const int2* tblA, int nbA, const LsRange* rng,
                           int nact, void* ctl, double* partT, double* partA,
-                          int* nfrozen, hipEvent_t* ev, hipStream_t s) {
+                          int* nfrozen, hipEvent_t* ev, bool idx16, hipStream_t s) {
     Ctl* c = static_cast<Ctl*>(ctl);
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (nbT > 0) k_lk_primal<V><<<nbT, NT, 0, s>>>(a, lps, tblT, c, partT);
+    if (nbT > 0) {
+        if (idx16) k_lk_primal<V, unsigned short><<<nbT, NT, 0, s>>>(a, lps, tblT, c, partT);
+        else k_lk_primal<V, int><<<nbT, NT, 0, s>>>(a, lps, tblT, c, partT);
+    }
     if (ev) (void)hipEventRecord(ev[1], s);
-    if (nbA > 0) k_lk_dual<V><<<nbA, NT, 0, s>>>(a, lps, tblA, c, partA);
+    if (nbA > 0) {
+        if (idx16) k_lk_dual<V, unsigned short><<<nbA, NT, 0, s>>>(a, lps, tblA, c, partA);
+        else k_lk_dual<V, int><<<nbA, NT, 0, s>>>(a, lps, tblA, c, partA);
+    }
     if (ev) (void)hipEventRecord(ev[2], s);
     const int per = NT / WAVE;
     k_lk_decide<<<(nact + per - 1) / per, NT, 0, s>>>(rng, nact, c, partT, partA, nfrozen);
@@ -866,14 +903,16 @@ void lockstep_setup(const Arrays<double>* a, const Params& p, void* ctl, int* do
 void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const int2* tblT,
                     int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
-                    hipEvent_t* ev, hipStream_t s) {
-    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, ev, s);
+                    hipEvent_t* ev, bool idx16, hipStream_t s) {
+    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, ev,
+                  idx16, s);
 }
 void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const int2* tblT,
                     int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
-                    hipEvent_t* ev, hipStream_t s) {
-    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, ev, s);
+                    hipEvent_t* ev, bool idx16, hipStream_t s) {
+    lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, ev,
+                  idx16, s);
 }
 void lockstep_check(const Arrays<float>* a, const Params& p, const LsRange* rng,
                     int nact, void* ctl, int* done, hipStream_t s) {

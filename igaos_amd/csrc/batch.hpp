@@ -45,6 +45,8 @@ struct LPResult {
 template <typename V>
 struct Arrays {
     const int *ap, *ai, *cp, *ci;     // local indices, per-LP pointer arrays
+    const unsigned short *ai16, *ci16;  // 16-bit copies of ai, ci when every
+                                        // instance has m, n <= 65536 (else null)
     const int *rbA, *rbT;             // per-LP row-block tables (pairs)
     const double *ax, *acx;           // original values
     const double *c, *cl, *cu, *rmin, *rmax;
@@ -94,14 +96,15 @@ void lockstep_setup(const Arrays<float>* a, const Params& p, void* ctl, int* don
                     hipStream_t s);
 void lockstep_setup(const Arrays<double>* a, const Params& p, void* ctl, int* done,
                     hipStream_t s);
+// idx16: read the 16-bit column indices (Arrays::ai16, ci16).
 void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const int2* tblT,
                     int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
-                    hipEvent_t* ev, hipStream_t s);
+                    hipEvent_t* ev, bool idx16, hipStream_t s);
 void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const int2* tblT,
                     int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
-                    hipEvent_t* ev, hipStream_t s);
+                    hipEvent_t* ev, bool idx16, hipStream_t s);
 void lockstep_check(const Arrays<float>* a, const Params& p, const LsRange* rng,
                     int nact, void* ctl, int* done, hipStream_t s);
 void lockstep_check(const Arrays<double>* a, const Params& p, const LsRange* rng,

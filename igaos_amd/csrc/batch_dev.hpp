@@ -62,6 +62,8 @@ struct LP {
     __device__ const int* ai() const { return A->ai + zo; }
     __device__ const int* cp() const { return A->cp + cpo; }
     __device__ const int* ci() const { return A->ci + zo; }
+    __device__ const unsigned short* ai16() const { return A->ai16 + zo; }
+    __device__ const unsigned short* ci16() const { return A->ci16 + zo; }
     __device__ const int* rbA() const { return A->rbA + rbao; }
     __device__ const int* rbT() const { return A->rbT + rbto; }
     __device__ const double* ax() const { return A->ax + zo; }

@@ -82,6 +82,7 @@ struct igaos_batch {
     bool solved = false;  // a run finished and the data are unchanged since
     int device = 0;
     int mode = 0;         // 0: persistent kernel, 1: lockstep rounds
+    bool idx16 = false;   // lockstep kernels read 16-bit column indices
     std::thread worker;   // lockstep: drives the rounds of one run
     std::string worker_err;
     std::atomic<int> lk_done{0};
@@ -142,6 +143,16 @@ void fill_arrays(igaos_batch& B, igaos::batch::Arrays<V>& A, const int* ap,
     A.cp = M.upload(cp, n + B.count);
     A.ci = M.upload(ci, z);
     A.acx = M.upload(acx, z);
+    A.ai16 = A.ci16 = nullptr;
+    bool fits = true;
+    for (const LPDesc& d : B.desc) fits = fits && d.m <= 65536 && d.n <= 65536;
+    if (fits && z > 0) {  // 16-bit column indices for the lockstep attempt kernels
+        std::vector<unsigned short> h(z);
+        for (long p = 0; p < z; ++p) h[p] = static_cast<unsigned short>(ai[p]);
+        A.ai16 = M.upload(h.data(), z);
+        for (long p = 0; p < z; ++p) h[p] = static_cast<unsigned short>(ci[p]);
+        A.ci16 = M.upload(h.data(), z);
+    }
     A.c = M.upload(c, n);
     A.cl = M.upload(cl, n);
     A.cu = M.upload(cu, n);
@@ -248,7 +259,7 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
                 hipEvent_t* ev = (first && r % 16 == 0 && ns < 4) ? B->lk_ev[ns++] : nullptr;
                 igaos::batch::lockstep_round(A, B->d_desc, B->d_tblT, nbT, B->d_tblA, nbA,
                                              B->d_rng, nact, B->d_ctl, B->d_partT,
-                                             B->d_partA, B->d_nfrozen, ev, s);
+                                             B->d_partA, B->d_nfrozen, ev, B->idx16, s);
             }
             BCK(hipGetLastError());
             int nf = 0;
@@ -401,9 +412,11 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
         BCK(hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking));
         if (precision == 32) {
             fill_arrays(*B, B->a32, ap, ai, ax, cp, ci, acx, c, cl, cu, rmin, rmax);
+            B->idx16 = B->a32.ai16 != nullptr;
             B->d_a32 = B->mem.upload(&B->a32, 1);
         } else {
             fill_arrays(*B, B->a64, ap, ai, ax, cp, ci, acx, c, cl, cu, rmin, rmax);
+            B->idx16 = B->a64.ai16 != nullptr;
             B->d_a64 = B->mem.upload(&B->a64, 1);
         }
         B->d_desc = B->mem.upload(B->desc.data(), count);

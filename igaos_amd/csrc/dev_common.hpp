@@ -141,10 +141,11 @@ __device__ __forceinline__ void block_rows(const RowBlocks& rb, int blk,
 // row then reduce: inside a wavefront with xor shuffles, across the 2 or 4
 // wavefronts of a block through LDS when the block holds only 1-2 rows.
 // Single row longer than NNZ_MAX: the workgroup strides over it four
-// loads deep per thread.
-template <typename Acc, typename Val, typename G>
+// loads deep per thread.  Column indices are int, or 16-bit where every
+// index fits (the batched solver's instances).
+template <typename Acc, typename Val, typename G, typename I>
 __device__ __forceinline__ void rowblock_sums(const int* __restrict__ ptr,
-                                              const int* __restrict__ idx,
+                                              const I* __restrict__ idx,
                                               const Val* __restrict__ val,
                                               const G& gx, int r0, int r1,
                                               int p0, int p1, Acc* s_rows,

@@ -176,3 +176,67 @@ def test_masks_balance_a_misclassified_column():
     xs[j] = lp.cl[j] + step if x[j] <= lp.cl[j] + 1e-12 else lp.cu[j] - step
     B, R = masks(lp, xs, y)
     assert (B == B0).all() and B.sum() == R.sum()
+
+
+def _enumerate(lp):
+    """Optimal vertex of a tiny equality LP with box bounds by brute force:
+    every choice of m basic columns and bound for each other column."""
+    import itertools
+    A = lp.to_scipy().toarray()
+    best = None
+    for B in itertools.combinations(range(lp.n), lp.m):
+        AB = A[:, B]
+        if abs(np.linalg.det(AB)) < 1e-9:
+            continue
+        N = [j for j in range(lp.n) if j not in B]
+        for side in itertools.product((0, 1), repeat=len(N)):
+            x = np.zeros(lp.n)
+            for j, sd in zip(N, side):
+                x[j] = lp.cu[j] if sd else lp.cl[j]
+            x[list(B)] = np.linalg.solve(AB, lp.rmin - A[:, N] @ x[N])
+            if np.any(x < lp.cl - 1e-9) or np.any(x > lp.cu + 1e-9):
+                continue
+            obj = lp.c @ x
+            if best is None or obj < best[0] - 1e-12:
+                y = np.linalg.solve(AB.T, -lp.c[list(B)])
+                best = (obj, x, y)
+    return best
+
+
+def test_adjoint_oracle_against_vertex_enumeration():
+    # tiny LPs solved by brute force (no LP solver involved): the oracle's
+    # gradients equal central differences of the enumerated optimum
+    rng = np.random.default_rng(11)
+    checked = 0
+    for trial in range(12):
+        m, n = 2, 5
+        A = rng.standard_normal((m, n))
+        x0 = rng.uniform(0, 10, n)
+        lp = lpgen.LP.from_triplets(m, n, *np.nonzero(A), A[np.nonzero(A)],
+                                    rng.standard_normal(n), np.zeros(n),
+                                    np.full(n, 10.0), A @ x0, A @ x0)
+        base = _enumerate(lp)
+        if base is None:
+            continue
+        _, x, y = base
+        inside = (x > 1e-6) & (x < 10 - 1e-6)
+        if inside.sum() != m:
+            continue  # degenerate vertex
+        w = rng.standard_normal(n)
+        g = adjoint(lp, x, y, gx=w, gy=y)
+        L = lambda q: (lambda r: w @ r[1] + 0.5 * r[2] @ r[2])(_enumerate(q))  # noqa
+        for name, grad, size in (("c", g.gc, n), ("b", g.grmin + g.grmax, m),
+                                 ("A", g.gax, lp.nnz)):
+            u = rng.standard_normal(size)
+            u /= np.linalg.norm(u)
+            if name == "c":
+                qp, qm = lp.with_values(c=lp.c + H * u), lp.with_values(c=lp.c - H * u)
+            elif name == "b":
+                qp = lp.with_values(rmin=lp.rmin + H * u, rmax=lp.rmax + H * u)
+                qm = lp.with_values(rmin=lp.rmin - H * u, rmax=lp.rmax - H * u)
+            else:
+                qp, qm = lp.with_values(ax=lp.ax + H * u), lp.with_values(ax=lp.ax - H * u)
+            fd = (L(qp) - L(qm)) / (2 * H)
+            assert abs(grad @ u - fd) <= 1e-6 * max(1.0, abs(fd)), (trial, name)
+        checked += 1
+    assert checked >= 4
