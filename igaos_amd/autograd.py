@@ -66,7 +66,7 @@ class DiffLPBatch:
     """Batched LP layer: patterns and column bounds from ``lps`` (a list of
     ``igaos_amd.lpgen.LP``); numeric data per call."""
 
-    def __init__(self, lps, precision=32, tolerance=1e-8, time_limit=60.0,
+    def __init__(self, lps, precision=32, tolerance=1e-6, time_limit=60.0,
                  max_iterations=500000, adjoint_tolerance=1e-12,
                  adjoint_max_iterations=None, polish=True, mode="persistent"):
         self.lps = list(lps)
@@ -96,10 +96,13 @@ class DiffLPBatch:
             xs, ys, obj, info = self.solver.polish(
                 tolerance=self.adjoint_tolerance,
                 max_iterations=self.adjoint_max_iterations)
-            rej = np.flatnonzero(info["accepted"] == 0)
+            certified = np.array([st == "near-optimal" for st in sol.status])
+            rej = np.flatnonzero((info["accepted"] == 0) & certified)
             if rej.size:
-                # the active set was misread at this tolerance: solve those
-                # instances again 100x tighter and polish once more
+                # certified, but the active set was misread at this
+                # tolerance: solve those instances again 100x tighter and
+                # polish once more (uncertified instances keep their PDHG
+                # pair; their gradients are not meaningful)
                 cat = lambda v, off, i: v[int(off[i]):int(off[i + 1])]  # noqa
                 sub_lps = [self.lps[i].with_values(
                     c=cat(c, self.coff, i), rmin=cat(rmin, self.roff, i),
