@@ -274,3 +274,20 @@ def test_column_locality_order_keeps_parity():
     # column-specific upper bounds (U[1, 100]) hold
     assert s32.iterations == K
     assert np.all(s32.x >= lp.cl - 1e-6) and np.all(s32.x <= lp.cu * (1 + 1e-6) + 1e-6)
+
+
+def test_reference_recorded_pdhg_battery():
+    # the reference's own PDHG runs at its default tolerance 1e-4 (recorded,
+    # not re-run): same status, and an objective within the tolerance band
+    # of the recorded one (trajectories differ below the tolerance)
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "reference_pdhg_battery.json")) as f:
+        bat = json.load(f)
+    for name, rec in bat["instances"].items():
+        s = igaos_amd.solve(netlib_path(name), tolerance=bat["tolerance"],
+                            time_limit=120, engine="pdhg")
+        assert s.status == rec["status"], (name, s.status)
+        ref = rec["objective"]
+        assert abs(s.objective - ref) <= 3e-4 * (1 + abs(ref)), (name, s.objective, ref)
