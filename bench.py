@@ -277,16 +277,20 @@ def bench_zipf(args):
     for prec in (64, 32):
         e = igaos_amd.Engine(lp, precision=prec)
         tm = e.profile_attempts(args.attempts)
+        sl = e.column_slices()
         w = 8 if prec == 64 else 4
         bp = lp.nnz * (w + 4) + 4 * (lp.n + 1) + 7 * lp.n * w
         bd = lp.nnz * (w + 4) + 4 * (lp.m + 1) + 8 * lp.m * w
         kern[prec] = dict(ms_primal=tm["ms_primal"], ms_dual=tm["ms_dual"],
                           gbs_primal=bp / tm["ms_primal"] / 1e6,
                           gbs_dual=bd / tm["ms_dual"] / 1e6,
-                          bytes_primal=bp, bytes_dual=bd)
-        log(f"[bench] FP{prec}: k_primal {tm['ms_primal']:.3f} ms "
-            f"({kern[prec]['gbs_primal']:.0f} GB/s), k_dual "
-            f"{tm['ms_dual']:.3f} ms ({kern[prec]['gbs_dual']:.0f} GB/s)")
+                          bytes_primal=bp, bytes_dual=bd,
+                          kernels_primal=step_kernels("primal", sl["primal"], prec),
+                          kernels_dual=step_kernels("dual", sl["dual"], prec))
+        log(f"[bench] FP{prec}: primal step {tm['ms_primal']:.3f} ms "
+            f"({kern[prec]['gbs_primal']:.0f} GB/s, {kern[prec]['kernels_primal']}), "
+            f"dual step {tm['ms_dual']:.3f} ms ({kern[prec]['gbs_dual']:.0f} GB/s, "
+            f"{kern[prec]['kernels_dual']})")
         if prec == 32:
             torch_sync()
             t0 = time.perf_counter()
@@ -296,9 +300,9 @@ def bench_zipf(args):
             log(f"[bench] FP32 solve: {s.status} {s.iterations} it {el:.1f} s")
         e.close()
     k64 = kern[64]
-    dom = "k_dual" if k64["ms_dual"] >= k64["ms_primal"] else "k_primal"
-    ms_dom = k64["ms_dual" if dom == "k_dual" else "ms_primal"]
-    b_dom = k64["bytes_dual" if dom == "k_dual" else "bytes_primal"]
+    dom = "dual" if k64["ms_dual"] >= k64["ms_primal"] else "primal"
+    ms_dom = k64[f"ms_{dom}"]
+    b_dom = k64[f"bytes_{dom}"]
     achieved = b_dom / ms_dom / 1e6
     solved = 1 if s.status == "near-optimal" else 0
     cpu = None
@@ -325,14 +329,22 @@ def bench_zipf(args):
                        "iterations_per_s": s.iterations / el,
                        "pinf": s.pinf, "dinf": s.dinf, "rel_gap": s.rel_gap},
         "attempt_kernels": kern,
-        "roofline": {"bound": "hbm", "kernel": f"{dom}<double>",
+        "roofline": {"bound": "hbm", "kernel": f"FP64 {dom} step: {k64[f'kernels_{dom}']}",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": load_traffic(f"{dom}_zipf"),
+                     "traffic": load_traffic(f"{dom}_step_zipf"),
                      "bytes_per_launch": b_dom,
                      "ms_per_launch": round(ms_dom, 4)},
         "cpu_baseline": cpu,
     }
+
+
+def step_kernels(step, slices, prec):
+    """Kernels of one primal / dual step of the single-LP engine."""
+    t = "double" if prec == 64 else "float"
+    if slices:
+        return f"k_{step}_slices<{t}> x{slices} + k_{step}_epi<{t}>"
+    return f"k_{step}<{t}>"
 
 
 def torch_sync():

@@ -28,7 +28,8 @@ class Options(C.Structure):
 
 class EngineConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("graph_slots", C.c_int),
-                ("max_repair_m", C.c_int), ("device", C.c_int)]
+                ("max_repair_m", C.c_int), ("device", C.c_int),
+                ("column_slices", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -72,6 +73,7 @@ SIGNATURES = {
     "igaos_pdhg_run": (C.c_int, [_vp, C.POINTER(Options), _dp, _dp,
                                  C.POINTER(SolveInfo), C.c_char_p, C.c_int]),
     "igaos_pdhg_setup_ms": (C.c_double, [_vp]),
+    "igaos_pdhg_column_slices": (C.c_int, [_vp, _ip, _ip]),
     "igaos_pdhg_destroy": (None, [_vp]),
     "igaos_pdhg_profile_attempts": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip,
                                               _ip]),

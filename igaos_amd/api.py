@@ -203,10 +203,11 @@ class Engine:
     returns the same Solution ``solve_lp`` would.
     """
 
-    def __init__(self, lp, precision=64, graph_slots=None, max_repair_m=None):
+    def __init__(self, lp, precision=64, graph_slots=None, max_repair_m=None,
+                 column_slices=None):
         self.model = lp if isinstance(lp, Model) else model_from_lp(lp)
         self.precision = precision
-        cfg = _config(precision, graph_slots, max_repair_m)
+        cfg = _config(precision, graph_slots, max_repair_m, column_slices)
         h = _core.lib().igaos_pdhg_create(self.model._h, C.byref(cfg))
         if not h:
             raise RuntimeError(_core.last_error())
@@ -230,6 +231,14 @@ class Engine:
             raise RuntimeError(_core.last_error())
         return dict(ms_primal=a.value, ms_dual=b.value,
                     blocks_primal=ga.value, blocks_dual=gb.value)
+
+    def column_slices(self):
+        """Column slices of the primal-step (A'y) and dual-step (A xbar)
+        products; 0 means the fused row-block kernels."""
+        p, d = C.c_int(), C.c_int()
+        if _core.lib().igaos_pdhg_column_slices(self._h, C.byref(p), C.byref(d)) != 0:
+            raise RuntimeError(_core.last_error())
+        return dict(primal=p.value, dual=d.value)
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -266,7 +275,8 @@ def _options(time_limit, tolerance, max_iterations, presolve, seed,
     return o
 
 
-def _config(precision=64, graph_slots=None, max_repair_m=None) -> EngineConfig:
+def _config(precision=64, graph_slots=None, max_repair_m=None,
+            column_slices=None) -> EngineConfig:
     c = EngineConfig()
     _core.lib().igaos_engine_config_default(C.byref(c))
     c.precision = int(precision)
@@ -274,6 +284,8 @@ def _config(precision=64, graph_slots=None, max_repair_m=None) -> EngineConfig:
         c.graph_slots = int(graph_slots)
     if max_repair_m is not None:
         c.max_repair_m = int(max_repair_m)
+    if column_slices is not None:
+        c.column_slices = int(column_slices)
     return c
 
 
@@ -298,7 +310,7 @@ def solve(path, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
 
 def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
              precision=64, verbosity=0, graph_slots=None,
-             max_repair_m=None) -> Solution:
+             max_repair_m=None, column_slices=None) -> Solution:
     """Solve an in-memory LP (``lpgen.LP`` or ``Model``) with the engine.
 
     ``precision=64`` is the plain FP64 engine; ``precision=32`` runs FP32
@@ -307,7 +319,7 @@ def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
     model = lp if isinstance(lp, Model) else model_from_lp(lp)
     return _run(model, _options(time_limit, tolerance, max_iterations, True,
                                 0, verbosity),
-                _config(precision, graph_slots, max_repair_m))
+                _config(precision, graph_slots, max_repair_m, column_slices))
 
 
 has_pdhg = _has_pdhg()

@@ -195,6 +195,7 @@ void igaos_engine_config_default(igaos_engine_config* cfg) {
     cfg->graph_slots = e.graph_slots;
     cfg->max_repair_m = e.max_repair_m;
     cfg->device = e.device;
+    cfg->column_slices = e.column_slices;
 }
 
 namespace {
@@ -218,6 +219,8 @@ bool to_config(const igaos_engine_config* cfg, igaos::pdhg::EngineConfig& ec) {
     ec.graph_slots = std::max(0, cfg->graph_slots);
     ec.max_repair_m = cfg->max_repair_m;
     ec.device = cfg->device;
+    if (cfg->column_slices < -1 || cfg->column_slices > 16) return false;
+    ec.column_slices = cfg->column_slices;
     return true;
 }
 
@@ -329,6 +332,16 @@ double igaos_pdhg_setup_ms(const igaos_engine* engine) {
 }
 
 void igaos_pdhg_destroy(igaos_engine* engine) { delete engine; }
+
+int igaos_pdhg_column_slices(const igaos_engine* engine, int* slices_primal,
+                             int* slices_dual) {
+    if (!engine) return fail("null engine");
+    int p = 0, d = 0;
+    engine->e->column_slices(p, d);
+    if (slices_primal) *slices_primal = p;
+    if (slices_dual) *slices_dual = d;
+    return 0;
+}
 
 int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
                                 double* ms_primal, double* ms_dual,

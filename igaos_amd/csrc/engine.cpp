@@ -173,6 +173,10 @@ class Engine final : public EngineBase {
     Solution run(const Options& opt, SolveInfo* info) override;
     AttemptTiming profile(int attempts) override;
     double setup_ms() const override { return setup_ms_; }
+    void column_slices(int& primal, int& dual) const override {
+        primal = ib_.sT.S;
+        dual = ib_.sA.S;
+    }
 
   private:
     static constexpr bool F32 = std::is_same<V, float>::value;
@@ -202,12 +206,21 @@ class Engine final : public EngineBase {
     DBuf<unsigned int> tickets_;
     DBuf<LsState> st_;
     DBuf<Metrics> met_;
+    // column-sliced copies of A (gathers x) and A' (gathers y), when used
+    struct SliceBufs {
+        DBuf<int> ptr, idx, quad, bslice;
+        DBuf<V> val, part;
+    } slA_, slT_;
     dev::IterBufs<V> ib_;
     dev::KktData kd_;
     dev::RowBlocks rA_, rT_;
     double lam2_ = 1.0, eta0_ = 1.0, omega0_ = 1.0;
 
     void setup();
+    int slices_for(int cols, long nnz) const;
+    void make_slices(dev::Sliced<V>& sl, SliceBufs& B, int rows, int cols,
+                     const std::vector<int>& ptr, const std::vector<int>& idx,
+                     const double* sval, int S);
     void reset();
     double power_iteration();
     double dsum(const double* a, const double* b, long n);
@@ -344,8 +357,15 @@ void Engine<V>::setup() {
     x0_.alloc(n_); x1_.alloc(n_); xbar_.alloc(n_); ata_.alloc(n_); xavg_.alloc(n_);
     y0_.alloc(m_); y1_.alloc(m_); yavg_.alloc(m_); axcur_.alloc(m_); adx_.alloc(m_);
     xb_.alloc(n_); yb_.alloc(m_); pesx_.alloc(n_); pesy_.alloc(m_);
-    partT_.alloc(std::max(rT_.nblk, 1));
-    partA_.alloc(2 * std::max(rA_.nblk, 1));
+    {
+        const int sT = slices_for(m_, nnz), sA = slices_for(n_, nnz);
+        if (sT) make_slices(ib_.sT, slT_, n_, m_, lp_.cp, lp_.ci, sacx_.p, sT);
+        if (sA) make_slices(ib_.sA, slA_, m_, n_, lp_.ap, lp_.ai, sax_.p, sA);
+        ib_.nT_part = sT ? (n_ + dev::BS - 1) / dev::BS : rT_.nblk;
+        ib_.nA_part = sA ? std::max((m_ + dev::BS - 1) / dev::BS, 1) : std::max(rA_.nblk, 1);
+    }
+    partT_.alloc(std::max(ib_.nT_part, 1));
+    partA_.alloc(2 * ib_.nA_part);
     partC_.alloc(8 * std::max(rT_.nblk, 1));
     partR_.alloc(8 * std::max(rA_.nblk, 1));
     partR2_.alloc(2 * 2 * 1024);
@@ -401,6 +421,61 @@ void Engine<V>::setup() {
         HIPCK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
     }
     sync();
+}
+
+// Column slicing of the product that gathers a vector of `cols` entries:
+// used when that vector (cols * sizeof(V) bytes) outgrows what an XCD's
+// 4 MB L2 keeps under the gathers (3 MB and up) and the matrix has enough
+// nonzeros (8M) that every slice still fills the GPU.  Slices hold at least
+// 0.5 MB of the vector each, at most 8 of them: on the 1M-column Zipf LP
+// (m = 400k, 68.5M nonzeros) 8 slices of x took the A xbar step from 450
+// to 403 us (FP64) and 382 to 284 us (FP32), and 4 slices of the FP64 y
+// the A'y step from 467 to 434 us, while 8 slices of y (0.4 MB each) or 16
+// of either were slower than 4 / 8: each slice adds a pass over the
+// partial row sums.
+template <typename V>
+int Engine<V>::slices_for(int cols, long nnz) const {
+    const int k = cfg_.column_slices;
+    if (k >= 0) return std::min(k, dev::MAX_SLICES);
+    const double bytes = static_cast<double>(cols) * sizeof(V);
+    if (bytes < 3.0e6 || nnz < 8000000) return 0;
+    int S = 8;
+    while (S > 2 && bytes / S < 0.5e6) S /= 2;
+    return S;
+}
+
+template <typename V>
+void Engine<V>::make_slices(dev::Sliced<V>& sl, SliceBufs& B, int rows, int cols,
+                            const std::vector<int>& ptr, const std::vector<int>& idx,
+                            const double* sval, int S) {
+    const dev::HostSlices h = dev::build_slices(rows, cols, ptr.data(), idx.data(), S);
+    const long nnz = static_cast<long>(h.idx.size());
+    B.ptr.upload(h.ptr.data(), h.ptr.size(), s_);
+    B.idx.upload(h.idx.data(), h.idx.size(), s_);
+    B.quad.upload(h.quad.data(), h.quad.size(), s_);
+    B.bslice.upload(h.bslice.data(), h.bslice.size(), s_);
+    B.val.alloc(nnz);
+    B.part.alloc(static_cast<size_t>(S) * rows);
+    {
+        DBuf<int> src;
+        src.upload(h.src.data(), h.src.size(), s_);
+        dev::gather_values(sval, src.p, B.val.p, nnz, s_);
+        sync();  // src is freed on scope exit
+    }
+    sl.S = S;
+    sl.rows = rows;
+    sl.nblk = static_cast<int>(h.bslice.size());
+    sl.sb[0] = 0;
+    for (int sc = 0, k = 0; sc < S; ++sc) {
+        while (k < sl.nblk && h.bslice[k] == sc) ++k;
+        sl.sb[sc + 1] = k;
+    }
+    sl.ptr = B.ptr.p;
+    sl.idx = B.idx.p;
+    sl.val = B.val.p;
+    sl.quad = reinterpret_cast<const int4*>(B.quad.p);
+    sl.bslice = B.bslice.p;
+    sl.part = B.part.p;
 }
 
 // Fresh start of a solve on the resident data: x = y = 0 (pdhg.cu:576-580).

@@ -17,6 +17,7 @@ struct EngineConfig {
     int graph_slots = 16;    // attempts captured per hipGraph replay (0 = no graph)
     int max_repair_m = 4096; // host dense dual repair only up to this many rows
     int device = 0;
+    int column_slices = -1;  // column slices of A / A' products: -1 auto, 0 off, 1..16 forced
 };
 
 struct SolveInfo {
@@ -46,6 +47,8 @@ class EngineBase {
     virtual Solution run(const Options& options, SolveInfo* info) = 0;
     virtual AttemptTiming profile(int attempts) = 0;
     virtual double setup_ms() const = 0;
+    // column slices of the primal-step (A'y) and dual-step (A xbar) products
+    virtual void column_slices(int& primal, int& dual) const = 0;
 };
 // Throws std::runtime_error when no HIP device is visible or a HIP call
 // fails, std::invalid_argument for a malformed model.

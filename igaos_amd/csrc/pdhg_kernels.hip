@@ -196,12 +196,12 @@ __global__ __launch_bounds__(BS) void k_dual(IterBufs<V> b) {
 // Line-search decision: fixed-order sums of the attempt's partials, then
 // the accept/reject rule on the device-resident state.
 template <typename V>
-__global__ __launch_bounds__(FIN) void k_decide(IterBufs<V> b, int nA) {
+__global__ __launch_bounds__(FIN) void k_decide(IterBufs<V> b) {
     __shared__ double red[FIN / WAVE];
     if (b.st->frozen) return;
-    const double d0 = fin_sum(b.partT, b.rbT.nblk, 1, red);
-    const double d1 = fin_sum(b.partA, nA, 2, red);
-    const double d2 = fin_sum(b.partA + 1, nA, 2, red);
+    const double d0 = fin_sum(b.partT, b.nT_part, 1, red);
+    const double d1 = fin_sum(b.partA, b.nA_part, 2, red);
+    const double d2 = fin_sum(b.partA + 1, b.nA_part, 2, red);
     if (threadIdx.x == 0) {
         LsState s = *b.st;
         pdhg::ls_decide(s, d0, d1, d2);
@@ -221,30 +221,239 @@ __global__ __launch_bounds__(BS) void k_flush(int n, int m, const V* x,
     }
 }
 
+
+// ------------------------------------------------------- sliced path --
+// Large LPs: the products of A' (gathers y) and A (gathers xbar) run over
+// the column-sliced copies (one launch, slice-major blocks, partial row
+// sums per slice), then an epilogue kernel per column / row adds the
+// partials in slice order and applies the same primal / dual step as
+// k_primal / k_dual, one thread per column / row.
+
+template <typename V>
+__device__ __forceinline__ void slice_rows(const Sliced<V>& sl, int blk, const V* x,
+                                           V* s_prod, V* s_rows, V* red_v) {
+    const int4 q = sl.quad[blk];
+    const int sc = sl.bslice[blk];
+    rowblock_sums<V>(sl.ptr + static_cast<long>(sc) * (sl.rows + 1), sl.idx, sl.val,
+                     GatherPlain<V>{x}, q.x, q.y, q.z, q.w, s_rows, s_prod, red_v);
+    V* out = sl.part + static_cast<long>(sc) * sl.rows;
+    for (int t = threadIdx.x; t < q.y - q.x; t += BS) out[q.x + t] = s_rows[t];
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_primal_slices(IterBufs<V> b, int b0) {
+    __shared__ V s_prod[NNZ_MAX];
+    __shared__ V s_rows[ROWS_MAX];
+    __shared__ V red_v[BS / WAVE];
+    const LsState st = *b.st;
+    if (st.frozen || !st.fresh) return;  // a retry reuses the stored A'y
+    slice_rows(b.sT, b0 + static_cast<int>(blockIdx.x), b.y[st.cur], s_prod, s_rows, red_v);
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_primal_epi(IterBufs<V> b) {
+    __shared__ double red[BS / WAVE];
+    const LsState st = *b.st;
+    if (st.frozen) return;
+    const int cur = st.cur;
+    const V tau = static_cast<V>(st.eta / st.omega);
+    const V w = static_cast<V>(st.mix_w);
+    const bool fresh = st.fresh != 0;
+    const bool mix = fresh && st.mix_pending != 0;
+    const int n = b.n, S = b.sT.S;
+    const int j = blockIdx.x * BS + threadIdx.x;
+    double acc = 0.0;
+    if (j < n) {
+        V a;
+        if (fresh) {
+            a = b.sT.part[j];
+            for (int sc = 1; sc < S; ++sc) a += b.sT.part[static_cast<long>(sc) * n + j];
+            b.ata[j] = a;
+        } else {
+            a = b.ata[j];
+        }
+        const V xo = b.x[cur][j];
+        if (mix) {
+            const V xa = b.xavg[j];
+            b.xavg[j] = xa + w * (xo - xa);
+        }
+        const V g = b.gref[j] + a;
+        const V v = fmin(fmax(xo - tau * g, b.lo[j]), b.hi[j]);
+        b.x[cur ^ 1][j] = v;
+        b.xbar[j] = V(2) * v - xo;
+        const V d = v - xo;
+        acc = static_cast<double>(d) * static_cast<double>(d);
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) b.partT[blockIdx.x] = acc;
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_dual_slices(IterBufs<V> b, int b0) {
+    __shared__ V s_prod[NNZ_MAX];
+    __shared__ V s_rows[ROWS_MAX];
+    __shared__ V red_v[BS / WAVE];
+    if (b.st->frozen) return;
+    slice_rows(b.sA, b0 + static_cast<int>(blockIdx.x), b.xbar, s_prod, s_rows, red_v);
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_dual_epi(IterBufs<V> b) {
+    __shared__ double red[BS / WAVE];
+    const LsState st = *b.st;
+    if (st.frozen) return;
+    const int cur = st.cur;
+    const V sig = static_cast<V>(st.eta * st.omega);
+    const V w = static_cast<V>(st.mix_w);
+    const bool mix = st.fresh != 0 && st.mix_pending != 0;
+    const int m = b.m, S = b.sA.S;
+    const int i = blockIdx.x * BS + threadIdx.x;
+    double a1 = 0.0, a2 = 0.0;
+    if (i < m) {
+        V tm = b.sA.part[i];
+        for (int sc = 1; sc < S; ++sc) tm += b.sA.part[static_cast<long>(sc) * m + i];
+        const V yo = b.y[cur][i];
+        V axc = b.axcur[i];
+        if (mix) {
+            const V ya = b.yavg[i];
+            b.yavg[i] = ya + w * (yo - ya);
+            axc = axc + b.adx[i];
+            b.axcur[i] = axc;
+        }
+        const V tt = yo + sig * tm;
+        V u = b.yoff ? tt + b.yoff[i] : tt;
+        u = u / sig;
+        const V yv = tt - sig * fmin(fmax(u, b.rlo[i]), b.rhi[i]);
+        b.y[cur ^ 1][i] = yv;
+        const V dy = yv - yo;
+        const V ad = V(0.5) * (tm - axc);
+        b.adx[i] = ad;
+        a1 = static_cast<double>(dy) * static_cast<double>(dy);
+        a2 = static_cast<double>(dy) * static_cast<double>(ad);
+    }
+    a1 = block_sum(a1, red);
+    a2 = block_sum(a2, red);
+    if (threadIdx.x == 0) {
+        b.partA[2 * blockIdx.x] = a1;
+        b.partA[2 * blockIdx.x + 1] = a2;
+    }
+}
+
+template <typename T>
+__global__ void k_gather_values(const double* src_val, const int* src, T* dst, long nnz) {
+    const long q = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (q < nnz) dst[q] = static_cast<T>(src_val[src[q]]);
+}
+
+void gather_values(const double* src_val, const int* src, double* dst, long nnz,
+                   hipStream_t s) {
+    if (nnz > 0)
+        k_gather_values<double><<<(nnz + BS - 1) / BS, BS, 0, s>>>(src_val, src, dst, nnz);
+}
+void gather_values(const double* src_val, const int* src, float* dst, long nnz,
+                   hipStream_t s) {
+    if (nnz > 0)
+        k_gather_values<float><<<(nnz + BS - 1) / BS, BS, 0, s>>>(src_val, src, dst, nnz);
+}
+
+HostSlices build_slices(int rows, int cols, const int* ptr, const int* idx, int S) {
+    HostSlices h;
+    h.S = S;
+    const long nnz = ptr[rows];
+    // column ranges with (about) equal nonzero counts
+    std::vector<long> ccount(static_cast<size_t>(cols) + 1, 0);
+    for (long p = 0; p < nnz; ++p) ccount[idx[p] + 1] += 1;
+    for (int j = 0; j < cols; ++j) ccount[j + 1] += ccount[j];
+    std::vector<int> bound(S + 1, cols);
+    bound[0] = 0;
+    for (int sc = 1; sc < S; ++sc) {
+        const long target = nnz * sc / S;
+        int j = bound[sc - 1];
+        while (j < cols && ccount[j] < target) ++j;
+        bound[sc] = j;
+    }
+    std::vector<int> slice_of(cols);
+    for (int sc = 0; sc < S; ++sc)
+        for (int j = bound[sc]; j < bound[sc + 1]; ++j) slice_of[j] = sc;
+    h.ptr.assign(static_cast<size_t>(S) * (rows + 1), 0);
+    std::vector<long> rowcnt(static_cast<size_t>(S) * rows, 0);
+    for (int r = 0; r < rows; ++r)
+        for (int p = ptr[r]; p < ptr[r + 1]; ++p)
+            rowcnt[static_cast<size_t>(slice_of[idx[p]]) * rows + r] += 1;
+    long off = 0;
+    for (int sc = 0; sc < S; ++sc) {
+        int* sp = h.ptr.data() + static_cast<size_t>(sc) * (rows + 1);
+        for (int r = 0; r < rows; ++r) {
+            sp[r] = static_cast<int>(off);
+            off += rowcnt[static_cast<size_t>(sc) * rows + r];
+        }
+        sp[rows] = static_cast<int>(off);
+    }
+    h.idx.resize(nnz);
+    h.src.resize(nnz);
+    std::vector<int> fill(static_cast<size_t>(S) * rows);
+    for (int sc = 0; sc < S; ++sc)
+        for (int r = 0; r < rows; ++r)
+            fill[static_cast<size_t>(sc) * rows + r] = h.ptr[static_cast<size_t>(sc) * (rows + 1) + r];
+    for (int r = 0; r < rows; ++r)
+        for (int p = ptr[r]; p < ptr[r + 1]; ++p) {
+            const int sc = slice_of[idx[p]];
+            const int q = fill[static_cast<size_t>(sc) * rows + r]++;
+            h.idx[q] = idx[p];
+            h.src[q] = p;
+        }
+    for (int sc = 0; sc < S; ++sc) {
+        const std::vector<int> rb =
+            build_row_blocks(h.ptr.data() + static_cast<size_t>(sc) * (rows + 1), rows);
+        const int nb = static_cast<int>(rb.size() / 2) - 1;
+        for (int k = 0; k < nb; ++k) {
+            h.quad.insert(h.quad.end(), {rb[2 * k], rb[2 * k + 2], rb[2 * k + 1], rb[2 * k + 3]});
+            h.bslice.push_back(sc);
+        }
+    }
+    return h;
+}
+
+template <typename V>
+static void primal_impl(const IterBufs<V>& b, hipStream_t s) {
+    if (b.sT.S) {
+        for (int sc = 0; sc < b.sT.S; ++sc) {
+            const int nb = b.sT.sb[sc + 1] - b.sT.sb[sc];
+            if (nb > 0) k_primal_slices<V><<<nb, BS, 0, s>>>(b, b.sT.sb[sc]);
+        }
+        if (b.nT_part > 0) k_primal_epi<V><<<b.nT_part, BS, 0, s>>>(b);
+    } else if (b.rbT.nblk > 0) {
+        k_primal<V><<<b.rbT.nblk, BS, 0, s>>>(b);
+    }
+}
+template <typename V>
+static void dual_impl(const IterBufs<V>& b, hipStream_t s) {
+    if (b.sA.S) {
+        for (int sc = 0; sc < b.sA.S; ++sc) {
+            const int nb = b.sA.sb[sc + 1] - b.sA.sb[sc];
+            if (nb > 0) k_dual_slices<V><<<nb, BS, 0, s>>>(b, b.sA.sb[sc]);
+        }
+        k_dual_epi<V><<<grid_of(b.nA_part), BS, 0, s>>>(b);
+    } else {
+        k_dual<V><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
+    }
+}
 template <typename V>
 static void attempt_impl(const IterBufs<V>& b, hipStream_t s) {
-    if (b.rbT.nblk > 0) k_primal<V><<<b.rbT.nblk, BS, 0, s>>>(b);
-    k_dual<V><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
-    k_decide<V><<<1, FIN, 0, s>>>(b, grid_of(b.rbA.nblk));
+    primal_impl(b, s);
+    dual_impl(b, s);
+    k_decide<V><<<1, FIN, 0, s>>>(b);
 }
 void launch_attempt(const IterBufs<double>& b, hipStream_t s) { attempt_impl(b, s); }
-void launch_primal(const IterBufs<double>& b, hipStream_t s) {
-    if (b.rbT.nblk > 0) k_primal<double><<<b.rbT.nblk, BS, 0, s>>>(b);
-}
-void launch_primal(const IterBufs<float>& b, hipStream_t s) {
-    if (b.rbT.nblk > 0) k_primal<float><<<b.rbT.nblk, BS, 0, s>>>(b);
-}
-void launch_dual(const IterBufs<double>& b, hipStream_t s) {
-    k_dual<double><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
-}
-void launch_dual(const IterBufs<float>& b, hipStream_t s) {
-    k_dual<float><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
-}
+void launch_primal(const IterBufs<double>& b, hipStream_t s) { primal_impl(b, s); }
+void launch_primal(const IterBufs<float>& b, hipStream_t s) { primal_impl(b, s); }
+void launch_dual(const IterBufs<double>& b, hipStream_t s) { dual_impl(b, s); }
+void launch_dual(const IterBufs<float>& b, hipStream_t s) { dual_impl(b, s); }
 void launch_decide(const IterBufs<double>& b, hipStream_t s) {
-    k_decide<double><<<1, FIN, 0, s>>>(b, grid_of(b.rbA.nblk));
+    k_decide<double><<<1, FIN, 0, s>>>(b);
 }
 void launch_decide(const IterBufs<float>& b, hipStream_t s) {
-    k_decide<float><<<1, FIN, 0, s>>>(b, grid_of(b.rbA.nblk));
+    k_decide<float><<<1, FIN, 0, s>>>(b);
 }
 void launch_attempt(const IterBufs<float>& b, hipStream_t s) { attempt_impl(b, s); }
 

@@ -17,6 +17,42 @@ namespace igaos::dev {
 // (dev_common.hpp), nblk + 1 pairs.
 std::vector<int> build_row_blocks(const int* ptr, int nrows);
 
+// Column-sliced copy of a matrix for gather locality on large LPs.  The
+// columns are cut into S contiguous ranges holding equal numbers of
+// nonzeros; each row's nonzeros are split by range (keeping their order)
+// and stored slice-major.  The slices' row blocks run as S launches, one
+// per slice, so that while a slice runs only its range of the gathered
+// vector is in use and stays resident in every XCD's L2.  Each block
+// writes partial row sums part[s * rows + r]; an epilogue kernel adds the
+// S partials of a row in slice order.
+struct HostSlices {
+    int S = 0;
+    std::vector<int> ptr;     // S * (rows + 1) offsets into idx
+    std::vector<int> idx;     // column indices, slice-major
+    std::vector<int> src;     // position of each entry in the original CSR
+    std::vector<int> quad;    // per block: first row, end row, first nz, end nz
+    std::vector<int> bslice;  // per block: its slice
+};
+HostSlices build_slices(int rows, int cols, const int* ptr, const int* idx, int S);
+
+constexpr int MAX_SLICES = 16;
+template <typename V>
+struct Sliced {
+    int S = 0, rows = 0, nblk = 0;
+    int sb[MAX_SLICES + 1] = {};  // blocks of slice s: [sb[s], sb[s + 1])
+    const int* ptr = nullptr;
+    const int* idx = nullptr;
+    const V* val = nullptr;
+    const int4* quad = nullptr;
+    const int* bslice = nullptr;
+    V* part = nullptr;        // S * rows partial row sums
+};
+// dst[q] = V(src_val[src[q]]) for q < nnz (the sliced copy of the values).
+void gather_values(const double* src_val, const int* src, double* dst, long nnz,
+                   hipStream_t s);
+void gather_values(const double* src_val, const int* src, float* dst, long nnz,
+                   hipStream_t s);
+
 // y = A x  (plain SpMV).
 void spmv(const RowBlocks& rb, const int* ptr, const int* idx,
           const double* val, const double* x, double* y, hipStream_t s);
@@ -41,8 +77,12 @@ struct IterBufs {
     V *axcur = nullptr, *adx = nullptr;
     const V *gref = nullptr, *lo = nullptr, *hi = nullptr;   // primal data
     const V *rlo = nullptr, *rhi = nullptr, *yoff = nullptr; // dual data
-    double* partT = nullptr;  // nblkT partials of |dx|^2
-    double* partA = nullptr;  // nblkA pairs (|dy|^2, dy'Adx)
+    double* partT = nullptr;  // nT_part partials of |dx|^2
+    double* partA = nullptr;  // nA_part pairs (|dy|^2, dy'Adx)
+    int nT_part = 0, nA_part = 0;
+    // column-sliced A' (gathers y) and A (gathers xbar); S = 0: unsliced,
+    // the fused row-block kernels run instead
+    Sliced<V> sT, sA;
     pdhg::LsState* st = nullptr;
     unsigned int* ticket = nullptr;
 };

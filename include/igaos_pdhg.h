@@ -87,6 +87,8 @@ typedef struct {
     int graph_slots;  /* attempts per hipGraph replay; 0 = direct launches */
     int max_repair_m; /* dense dual repair only for m <= this */
     int device;
+    int column_slices; /* column slices of the A / A' products (gather
+                          locality on large LPs): -1 auto, 0 off, 1..16 */
 } igaos_engine_config;
 void igaos_engine_config_default(igaos_engine_config* cfg);
 
@@ -136,6 +138,10 @@ void igaos_pdhg_destroy(igaos_engine* engine);
 int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
                                 double* ms_primal, double* ms_dual,
                                 int* blocks_primal, int* blocks_dual);
+/* Column slices the engine uses for the A' y product of the primal step and
+ * the A xbar product of the dual step (0: fused row-block kernels). */
+int igaos_pdhg_column_slices(const igaos_engine* engine, int* slices_primal,
+                             int* slices_dual);
 
 /* ---- batched persistent solve (no reference counterpart) ----------------
  * Thousands of independent LPs solved in one persistent launch: each

@@ -140,3 +140,48 @@ def test_lockstep_iteration_cap_and_modes_agree():
     for i in range(5):
         assert s2.status[i] == s3.status[i] == "near-optimal"
         assert abs(s2.objective[i] - s3.objective[i]) <= 1e-6 * (1 + abs(s3.objective[i]))
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_infeasible_and_unbounded_instances_do_not_disturb_the_batch(mode):
+    # PDHG never certifies infeasibility or unboundedness (the reference has
+    # no detector either): such instances run to the cap and report
+    # iteration-limit, while the rest of the batch solves as usual
+    inf = lpgen.LP.from_triplets(1, 2, [0, 0], [0, 1], [1.0, 1.0], [1.0, 1.0],
+                                 [0.0, 0.0], [10.0, 10.0], [-1.0], [-1.0])
+    unb = lpgen.LP.from_triplets(1, 2, [0, 0], [0, 1], [1.0, -1.0], [-1.0, 0.0],
+                                 [0.0, 0.0], [np.inf, np.inf], [0.0], [0.0])
+    good = [lpgen.gen_batch_instance(6, i, m_hi=300) for i in range(3)]
+    lps = [good[0], inf, good[1], unb, good[2]]
+    sol = solve_batch(lps, precision=32, mode=mode, tolerance=1e-6, time_limit=1e7,
+                      max_iterations=100000)
+    assert sol.status[1] == "iteration-limit" and sol.pinf[1] > 1e-3
+    assert sol.status[3] == "iteration-limit"
+    ref = solve_batch(good, precision=32, mode=mode, tolerance=1e-6, time_limit=1e7,
+                      max_iterations=100000)
+    for k, i in enumerate((0, 2, 4)):
+        assert sol.status[i] == "near-optimal"
+        assert np.array_equal(sol.x[i], ref.x[k])  # independent of the batch
+
+
+@pytest.mark.parametrize("precision,min_certified", [(32, 30), (64, 31)])
+def test_netlib_sweep_as_one_lockstep_batch(precision, min_certified):
+    # the 36 small Netlib LPs of the reference's sweep as one batch (60 s,
+    # tolerance 1e-6): most certify, each certified objective lies within
+    # 2e-6 relative of HiGHS's (the reference's recorded sweep values), and
+    # the rest end at the time limit rather than in a wrong status
+    import csv
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "netlib", "sweep_highs.csv")) as f:
+        rows = list(csv.DictReader(f))
+    lps = [igaos_amd.read_mps(netlib_path(r["instance"])).to_lp() for r in rows]
+    sol = solve_batch(lps, precision=precision, mode="lockstep", tolerance=1e-6,
+                      time_limit=60)
+    assert sol.solved >= min_certified, sol.solved
+    for r, st, ob in zip(rows, sol.status, sol.objective):
+        ref = float(r["highs_objective"])
+        if st == "near-optimal":
+            assert abs(ob - ref) <= 2e-6 * (1 + abs(ref)), (r["instance"], ob, ref)
+        else:
+            assert st == "time-limit", (r["instance"], st)

@@ -276,6 +276,36 @@ def test_column_locality_order_keeps_parity():
     assert np.all(s32.x >= lp.cl - 1e-6) and np.all(s32.x <= lp.cu * (1 + 1e-6) + 1e-6)
 
 
+@pytest.mark.parametrize("slices", [1, 3, 8])
+def test_column_sliced_products_keep_parity(slices):
+    # the sliced A / A' products (per-slice partial row sums added in slice
+    # order) follow the oracle's trajectory like the fused row-block ones;
+    # one slice holds each row's nonzeros in the original blocks and order,
+    # so it sums every row as the fused kernels do (the split primal / dual
+    # step may contract its multiply-adds differently: ulp-level drift)
+    lp = lpgen.gen_zipf_badly_scaled(seed=4, m=6_000, n=20_000, kmax=600)
+    K = 64
+    r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
+    base = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K,
+                              column_slices=0)
+    s = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K,
+                           column_slices=slices)
+    assert s.iterations == r.iterations == K
+    assert s.info["attempts"] == r.attempts
+    assert _rel(s.x, r.x) <= 1e-9 and _rel(s.y, r.y) <= 1e-9
+    if slices == 1:
+        dx, dy = _rel(s.x, base.x), _rel(s.y, base.y)
+        assert dx <= 1e-12 and dy <= 1e-12, (dx, dy)
+    eq = lpgen.gen_random_eq(seed=5, m=3000, n=8000)
+    for prec in (64, 32):
+        got = igaos_amd.solve_lp(eq, time_limit=120, tolerance=1e-6, precision=prec,
+                                 column_slices=slices)
+        ref = igaos_amd.solve_lp(eq, time_limit=120, tolerance=1e-6, precision=prec,
+                                 column_slices=0)
+        assert got.status == ref.status == "near-optimal", (prec, got.message)
+        assert abs(got.objective - ref.objective) <= 1e-6 * (1 + abs(ref.objective))
+
+
 def test_reference_recorded_pdhg_battery():
     # the reference's own PDHG runs at its default tolerance 1e-4 (recorded,
     # not re-run): same status, and an objective within the tolerance band
