@@ -25,7 +25,9 @@ FP32 solve (the LP needs well over 10^4 iterations to reach 1e-6).
 
 ``cpu_baseline`` is the oracle (a single-threaded C host replay of the
 reference engine) on a bounded sample of the same workload, named in
-``sample``.
+``sample``.  Beside it, ``cpu_baseline_reference_simplex`` times the
+reference's own CPU path for these LPs, its revised simplex built from its
+sources (``oracle/_ref``, present when it was built), on the same sample.
 """
 from __future__ import annotations
 
@@ -158,6 +160,24 @@ def bench_batch(args):
                           f"{cs} of the batch, {sc}/{len(cs)} certified in "
                           f"{tc:.1f} s of solve time ({wall:.1f} s wall incl. "
                           f"setup), budget {args.cpu_time_limit} s each")}
+    cpu_ref = None
+    if not args.no_cpu_baseline:
+        import oracle
+        if oracle.have_ref_simplex():
+            # the reference's own CPU path for these LPs: its revised
+            # simplex (built from its sources), same instances, 5 s each
+            cs = sample[: args.cpu_sample]
+            wall = time.perf_counter()
+            sc = 0
+            for i in cs:
+                st = oracle.ref_simplex(lps[i], time_limit=5.0)[0]
+                sc += st == "optimal"
+            wall = time.perf_counter() - wall
+            cpu_ref = {"value": sc / wall, "unit": "LPs/s", "cores": 1,
+                       "kind": "reference",
+                       "sample": (f"reference revised simplex (src/simplex, 1 thread) "
+                                  f"on instances {cs}, {sc}/{len(cs)} optimal in "
+                                  f"{wall:.1f} s wall, budget 5 s each")}
 
     return {
         "metric": METRIC, "value": value, "unit": "LPs/s", "n_gpus": 1,
@@ -199,6 +219,7 @@ def bench_batch(args):
                      "lockstep_samples": None if lk is None else lk,
                      "whole_run_gbs": round(achieved_run, 1)},
         "cpu_baseline": cpu,
+        "cpu_baseline_reference_simplex": cpu_ref,
     }
 
 

@@ -654,17 +654,28 @@ __device__ __forceinline__ int xcd_entry(int b, int n) {
     return x * q + min(x, r) + slot;
 }
 
-// the column indices of the attempt kernels: int or the 16-bit copy
+// Resident waves per SIMD the attempt kernels are compiled for: 8 (the
+// LDS limit of 8 workgroups per CU) in FP32, which caps them at 64 VGPRs;
+// 6 in FP64, whose wider values would spill under that cap.
+template <typename V>
+constexpr int LK_WAVES = sizeof(V) == 4 ? 8 : 6;
+
+// Index type of the attempt kernels: int (the CSR / CSC arrays as they
+// are), or 16-bit on the gather-ordered copies (Arrays::ais16 ...).
 template <typename I>
-__device__ __forceinline__ const I* idx_of(const int* i32, const unsigned short* i16);
-template <>
-__device__ __forceinline__ const int* idx_of<int>(const int* i32, const unsigned short*) {
-    return i32;
-}
-template <>
-__device__ __forceinline__ const unsigned short* idx_of<unsigned short>(
-    const int*, const unsigned short* i16) {
-    return i16;
+constexpr bool GATHER_ORDERED = sizeof(I) == 2;
+
+// Values of the gather-ordered copies: per table entry (one row block),
+// dst[p] = src[p0 + slot[p]] over the block's nonzeros.
+template <typename V>
+__global__ __launch_bounds__(NT) void k_lk_sortvals(const Arrays<V>* A, const LPDesc* lps,
+                                                    const LkEnt* tbl, bool transposed) {
+    const LkEnt e = tbl[blockIdx.x];
+    const LP<V> L = view(A, lps[e.lp]);
+    const V* src = transposed ? L.facx() : L.fax();
+    V* dst = transposed ? L.facxs() : L.faxs();
+    const unsigned short* slot = transposed ? L.cslot() : L.aslot();
+    for (int p = e.p0 + threadIdx.x; p < e.p1; p += NT) dst[p] = src[e.p0 + slot[p]];
 }
 
 template <typename V>
@@ -689,19 +700,21 @@ __global__ __launch_bounds__(NT) void k_lk_setup(const Arrays<V>* A, Params P,
 }
 
 template <typename V, typename I>
-__global__ __launch_bounds__(NT) void k_lk_primal(const Arrays<V>* A,
+__global__ __launch_bounds__(NT, LK_WAVES<V>) void k_lk_primal(const Arrays<V>* A,
                                                   const LPDesc* lps,
-                                                  const int2* tbl, Ctl* ctl,
+                                                  const LkEnt* tbl, Ctl* ctl,
                                                   double* part) {
     __shared__ Lds S;
     const int te = xcd_entry(blockIdx.x, gridDim.x);
-    const int2 e = tbl[te];
-    const LsState st = ctl[e.x].st;
+    const LkEnt e = tbl[te];
+    // both loads depend only on the entry: issue them together
+    const LPDesc D = lps[e.lp];
+    const LsState st = ctl[e.lp].st;
     if (st.frozen) {
         if (threadIdx.x == 0) part[te] = 0.0;
         return;
     }
-    const LP<V> L = view(A, lps[e.x]);
+    const LP<V> L = view(A, D);
     const int cur = st.cur;
     const V tau = static_cast<V>(st.eta / st.omega);
     const V w = static_cast<V>(st.mix_w);
@@ -710,14 +723,18 @@ __global__ __launch_bounds__(NT) void k_lk_primal(const Arrays<V>* A,
     const V* __restrict__ x = L.x(cur);
     V* __restrict__ xn = L.x(cur ^ 1);
     const V* __restrict__ y = L.y(cur);
-    const int* rb = L.rbT() + 2 * e.y;
-    const int r0 = rb[0], p0 = rb[1], r1 = rb[2], p1 = rb[3];
+    const int r0 = e.r0, p0 = e.p0, r1 = e.r1, p1 = e.p1;
     V* rows = reinterpret_cast<V*>(S.rows);
-    if (fresh)
-        dev::rowblock_sums<V>(L.cp(), idx_of<I>(L.ci(), L.ci16()), L.facx(),
-                              dev::GatherPlain<V>{y}, r0, r1,
-                              p0, p1, rows, reinterpret_cast<V*>(S.prod),
-                              reinterpret_cast<V*>(S.red));
+    if (fresh) {
+        if constexpr (GATHER_ORDERED<I>)
+            dev::rowblock_sums<V>(L.cp(), L.cis16(), L.facxs(), dev::GatherPlain<V>{y}, r0,
+                                  r1, p0, p1, rows, reinterpret_cast<V*>(S.prod),
+                                  reinterpret_cast<V*>(S.red), L.cslot());
+        else
+            dev::rowblock_sums<V>(L.cp(), L.ci(), L.facx(), dev::GatherPlain<V>{y}, r0, r1,
+                                  p0, p1, rows, reinterpret_cast<V*>(S.prod),
+                                  reinterpret_cast<V*>(S.red));
+    }
     double acc = 0.0;
     for (int t = threadIdx.x; t < r1 - r0; t += NT) {
         const int j = r0 + t;
@@ -745,32 +762,36 @@ __global__ __launch_bounds__(NT) void k_lk_primal(const Arrays<V>* A,
 }
 
 template <typename V, typename I>
-__global__ __launch_bounds__(NT) void k_lk_dual(const Arrays<V>* A,
+__global__ __launch_bounds__(NT, LK_WAVES<V>) void k_lk_dual(const Arrays<V>* A,
                                                 const LPDesc* lps,
-                                                const int2* tbl, Ctl* ctl,
+                                                const LkEnt* tbl, Ctl* ctl,
                                                 double* part) {
     __shared__ Lds S;
     const int te = xcd_entry(blockIdx.x, gridDim.x);
-    const int2 e = tbl[te];
-    const LsState st = ctl[e.x].st;
+    const LkEnt e = tbl[te];
+    const LPDesc D = lps[e.lp];
+    const LsState st = ctl[e.lp].st;
     if (st.frozen) {
         if (threadIdx.x == 0) part[2 * te] = part[2 * te + 1] = 0.0;
         return;
     }
-    const LP<V> L = view(A, lps[e.x]);
+    const LP<V> L = view(A, D);
     const int cur = st.cur;
     const V sig = static_cast<V>(st.eta * st.omega);
     const V w = static_cast<V>(st.mix_w);
     const bool mix = st.fresh != 0 && st.mix_pending != 0;
     const V* __restrict__ y = L.y(cur);
     V* __restrict__ yn = L.y(cur ^ 1);
-    const int* rb = L.rbA() + 2 * e.y;
-    const int r0 = rb[0], p0 = rb[1], r1 = rb[2], p1 = rb[3];
+    const int r0 = e.r0, p0 = e.p0, r1 = e.r1, p1 = e.p1;
     V* rows = reinterpret_cast<V*>(S.rows);
-    dev::rowblock_sums<V>(L.ap(), idx_of<I>(L.ai(), L.ai16()), L.fax(),
-                          dev::GatherPlain<V>{L.xbar()}, r0,
-                          r1, p0, p1, rows, reinterpret_cast<V*>(S.prod),
-                          reinterpret_cast<V*>(S.red));
+    if constexpr (GATHER_ORDERED<I>)
+        dev::rowblock_sums<V>(L.ap(), L.ais16(), L.faxs(), dev::GatherPlain<V>{L.xbar()},
+                              r0, r1, p0, p1, rows, reinterpret_cast<V*>(S.prod),
+                              reinterpret_cast<V*>(S.red), L.aslot());
+    else
+        dev::rowblock_sums<V>(L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.xbar()}, r0,
+                              r1, p0, p1, rows, reinterpret_cast<V*>(S.prod),
+                              reinterpret_cast<V*>(S.red));
     const V* yoff = L.yoff();
     double a1 = 0.0, a2 = 0.0;
     for (int t = threadIdx.x; t < r1 - r0; t += NT) {
@@ -871,8 +892,8 @@ static void lk_setup_impl(const Arrays<V>* a, const Params& p, void* ctl,
     k_lk_setup<V><<<p.count, NT, 0, s>>>(a, p, static_cast<Ctl*>(ctl), done);
 }
 template <typename V>
-static void lk_round_impl(const Arrays<V>* a, const LPDesc* lps, const int2* tblT,
-                          int nbT, const int2* tblA, int nbA, const LsRange* rng,
+static void lk_round_impl(const Arrays<V>* a, const LPDesc* lps, const LkEnt* tblT,
+                          int nbT, const LkEnt* tblA, int nbA, const LsRange* rng,
                           int nact, void* ctl, double* partT, double* partA,
                           int* nfrozen, hipEvent_t* ev, bool idx16, hipStream_t s) {
     Ctl* c = static_cast<Ctl*>(ctl);
@@ -891,6 +912,21 @@ static void lk_round_impl(const Arrays<V>* a, const LPDesc* lps, const int2* tbl
     k_lk_decide<<<(nact + per - 1) / per, NT, 0, s>>>(rng, nact, c, partT, partA, nfrozen);
 }
 template <typename V>
+static void lk_sort_impl(const Arrays<V>* a, const LPDesc* lps, const LkEnt* tblT, int nbT,
+                         const LkEnt* tblA, int nbA, hipStream_t s) {
+    if (nbT > 0) k_lk_sortvals<V><<<nbT, NT, 0, s>>>(a, lps, tblT, true);
+    if (nbA > 0) k_lk_sortvals<V><<<nbA, NT, 0, s>>>(a, lps, tblA, false);
+}
+void lockstep_sort_values(const Arrays<float>* a, const LPDesc* lps, const LkEnt* tblT,
+                          int nbT, const LkEnt* tblA, int nbA, hipStream_t s) {
+    lk_sort_impl(a, lps, tblT, nbT, tblA, nbA, s);
+}
+void lockstep_sort_values(const Arrays<double>* a, const LPDesc* lps, const LkEnt* tblT,
+                          int nbT, const LkEnt* tblA, int nbA, hipStream_t s) {
+    lk_sort_impl(a, lps, tblT, nbT, tblA, nbA, s);
+}
+
+template <typename V>
 static void lk_check_impl(const Arrays<V>* a, const Params& p, const LsRange* rng,
                           int nact, void* ctl, int* done, hipStream_t s) {
     k_lk_check<V><<<nact, NT, 0, s>>>(a, p, rng, static_cast<Ctl*>(ctl), done);
@@ -900,15 +936,15 @@ void lockstep_setup(const Arrays<float>* a, const Params& p, void* ctl, int* don
                     hipStream_t s) { lk_setup_impl(a, p, ctl, done, s); }
 void lockstep_setup(const Arrays<double>* a, const Params& p, void* ctl, int* done,
                     hipStream_t s) { lk_setup_impl(a, p, ctl, done, s); }
-void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const int2* tblT,
-                    int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
+void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const LkEnt* tblT,
+                    int nbT, const LkEnt* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
                     hipEvent_t* ev, bool idx16, hipStream_t s) {
     lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, ev,
                   idx16, s);
 }
-void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const int2* tblT,
-                    int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
+void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const LkEnt* tblT,
+                    int nbT, const LkEnt* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
                     hipEvent_t* ev, bool idx16, hipStream_t s) {
     lk_round_impl(a, lps, tblT, nbT, tblA, nbA, rng, nact, ctl, partT, partA, nfrozen, ev,

@@ -47,6 +47,13 @@ struct Arrays {
     const int *ap, *ai, *cp, *ci;     // local indices, per-LP pointer arrays
     const unsigned short *ai16, *ci16;  // 16-bit copies of ai, ci when every
                                         // instance has m, n <= 65536 (else null)
+    // Gather-ordered copies for the lockstep attempt kernels (with ai16):
+    // inside every multi-row block the nonzeros sorted by column (A) / row
+    // (A'), so that a wavefront's gathers fall into few cache lines;
+    // aslot / cslot give each one's position in the block's row order, and
+    // faxs / facxs hold the values in the same order (filled per run).
+    const unsigned short *ais16, *cis16, *aslot, *cslot;
+    V *faxs, *facxs;
     const int *rbA, *rbT;             // per-LP row-block tables (pairs)
     const double *ax, *acx;           // original values
     const double *c, *cl, *cu, *rmin, *rmax;
@@ -82,10 +89,17 @@ int batch_occupancy(bool f32, int waves);
 
 // Lockstep mode (batch.hip): per-instance control blocks live in device
 // memory (lockstep_ctl_bytes() each); the host keeps tables of the active
-// instances' row blocks as (instance, local block) pairs and one LsRange
-// per active instance.  lockstep_round records ev[0..2] (if not null)
-// before the primal kernel, between the two attempt kernels and after the
-// dual kernel.
+// instances' row blocks (one LkEnt per block, carrying the block's rows and
+// nonzero range so that a workgroup reads its whole work description in
+// one load) and one LsRange per active instance.  lockstep_round records
+// ev[0..2] (if not null) before the primal kernel, between the two attempt
+// kernels and after the dual kernel.
+struct alignas(32) LkEnt {
+    int lp;        // instance
+    int r0, r1;    // rows [r0, r1) of the instance's matrix (A or A')
+    int p0, p1;    // their nonzeros [p0, p1) (instance-local offsets)
+    int pad[3];
+};
 struct LsRange {
     int lp;        // instance
     int bT0, nT;   // its blocks in the A' table
@@ -96,13 +110,18 @@ void lockstep_setup(const Arrays<float>* a, const Params& p, void* ctl, int* don
                     hipStream_t s);
 void lockstep_setup(const Arrays<double>* a, const Params& p, void* ctl, int* done,
                     hipStream_t s);
-// idx16: read the 16-bit column indices (Arrays::ai16, ci16).
-void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const int2* tblT,
-                    int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
+// idx16: read the 16-bit gather-ordered copies (Arrays::ais16, cis16 with
+// their slots and values; lockstep_sort_values fills the values).
+void lockstep_sort_values(const Arrays<float>* a, const LPDesc* lps, const LkEnt* tblT,
+                          int nbT, const LkEnt* tblA, int nbA, hipStream_t s);
+void lockstep_sort_values(const Arrays<double>* a, const LPDesc* lps, const LkEnt* tblT,
+                          int nbT, const LkEnt* tblA, int nbA, hipStream_t s);
+void lockstep_round(const Arrays<float>* a, const LPDesc* lps, const LkEnt* tblT,
+                    int nbT, const LkEnt* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
                     hipEvent_t* ev, bool idx16, hipStream_t s);
-void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const int2* tblT,
-                    int nbT, const int2* tblA, int nbA, const LsRange* rng, int nact,
+void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const LkEnt* tblT,
+                    int nbT, const LkEnt* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
                     hipEvent_t* ev, bool idx16, hipStream_t s);
 void lockstep_check(const Arrays<float>* a, const Params& p, const LsRange* rng,

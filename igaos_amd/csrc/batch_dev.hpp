@@ -64,6 +64,12 @@ struct LP {
     __device__ const int* ci() const { return A->ci + zo; }
     __device__ const unsigned short* ai16() const { return A->ai16 + zo; }
     __device__ const unsigned short* ci16() const { return A->ci16 + zo; }
+    __device__ const unsigned short* ais16() const { return A->ais16 + zo; }
+    __device__ const unsigned short* cis16() const { return A->cis16 + zo; }
+    __device__ const unsigned short* aslot() const { return A->aslot + zo; }
+    __device__ const unsigned short* cslot() const { return A->cslot + zo; }
+    __device__ V* faxs() const { return A->faxs + zo; }
+    __device__ V* facxs() const { return A->facxs + zo; }
     __device__ const int* rbA() const { return A->rbA + rbao; }
     __device__ const int* rbT() const { return A->rbT + rbto; }
     __device__ const double* ax() const { return A->ax + zo; }

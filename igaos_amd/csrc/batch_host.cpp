@@ -88,8 +88,8 @@ struct igaos_batch {
     std::atomic<int> lk_done{0};
     Mem lkmem;            // lockstep buffers, allocated on first use
     void* d_ctl = nullptr;
-    int2* d_tblT = nullptr;
-    int2* d_tblA = nullptr;
+    igaos::batch::LkEnt* d_tblT = nullptr;
+    igaos::batch::LkEnt* d_tblA = nullptr;
     igaos::batch::LsRange* d_rng = nullptr;
     double* d_partT = nullptr;
     double* d_partA = nullptr;
@@ -127,6 +127,49 @@ int bfail(const std::string& w) {
     return -1;
 }
 
+// Gather order of one matrix (A: rows ptr = ap, indices ai, blocks rbA;
+// or A' alike): inside every row block of more than one row the nonzeros
+// sorted by index (stable), as 16-bit indices plus each one's position in
+// the block's row order.  Instances are spread over host threads.
+void gather_order(const std::vector<LPDesc>& desc, const std::vector<int>& rb, bool trans,
+                  const int* idx, long z, std::vector<unsigned short>& sidx,
+                  std::vector<unsigned short>& slot) {
+    sidx.resize(z);
+    slot.resize(z);
+    const int nt = static_cast<int>(
+        std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        std::vector<int> ord;
+        for (int lp = next++; lp < static_cast<int>(desc.size()); lp = next++) {
+            const LPDesc& d = desc[lp];
+            const int* b = rb.data() + (trans ? d.rbto : d.rbao);
+            const int nb = trans ? d.nblkT : d.nblkA;
+            const int* ix = idx + d.zoff;
+            unsigned short* si = sidx.data() + d.zoff;
+            unsigned short* sl = slot.data() + d.zoff;
+            for (int k = 0; k < nb; ++k) {
+                const int r0 = b[2 * k], p0 = b[2 * k + 1];
+                const int r1 = b[2 * k + 2], p1 = b[2 * k + 3];
+                ord.resize(p1 - p0);
+                std::iota(ord.begin(), ord.end(), 0);
+                if (r1 - r0 > 1)
+                    std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
+                        return ix[p0 + u] < ix[p0 + v];
+                    });
+                for (int q = 0; q < p1 - p0; ++q) {
+                    si[p0 + q] = static_cast<unsigned short>(ix[p0 + ord[q]]);
+                    sl[p0 + q] = static_cast<unsigned short>(ord[q]);
+                }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
 template <typename V>
 void fill_arrays(igaos_batch& B, igaos::batch::Arrays<V>& A, const int* ap,
                  const int* ai, const double* ax, const int* cp,
@@ -143,15 +186,26 @@ void fill_arrays(igaos_batch& B, igaos::batch::Arrays<V>& A, const int* ap,
     A.cp = M.upload(cp, n + B.count);
     A.ci = M.upload(ci, z);
     A.acx = M.upload(acx, z);
-    A.ai16 = A.ci16 = nullptr;
+    A.ai16 = A.ci16 = A.ais16 = A.cis16 = A.aslot = A.cslot = nullptr;
+    A.faxs = A.facxs = nullptr;
     bool fits = true;
     for (const LPDesc& d : B.desc) fits = fits && d.m <= 65536 && d.n <= 65536;
     if (fits && z > 0) {  // 16-bit column indices for the lockstep attempt kernels
-        std::vector<unsigned short> h(z);
+        std::vector<unsigned short> h(z), hs;
         for (long p = 0; p < z; ++p) h[p] = static_cast<unsigned short>(ai[p]);
         A.ai16 = M.upload(h.data(), z);
         for (long p = 0; p < z; ++p) h[p] = static_cast<unsigned short>(ci[p]);
         A.ci16 = M.upload(h.data(), z);
+        // block sizes are at most 2048 nonzeros except single long rows,
+        // which keep their order: every slot fits 16 bits
+        gather_order(B.desc, B.rbA, false, ai, z, h, hs);
+        A.ais16 = M.upload(h.data(), z);
+        A.aslot = M.upload(hs.data(), z);
+        gather_order(B.desc, B.rbT, true, ci, z, h, hs);
+        A.cis16 = M.upload(h.data(), z);
+        A.cslot = M.upload(hs.data(), z);
+        A.faxs = M.alloc<V>(z);
+        A.facxs = M.alloc<V>(z);
     }
     A.c = M.upload(c, n);
     A.cl = M.upload(cl, n);
@@ -218,8 +272,21 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
     std::vector<int> active;
     for (int i = 0; i < count; ++i)
         if (!flags[i]) active.push_back(i);
+    long periods_done = 0;
     B->lk_done = count - static_cast<int>(active.size());
-    std::vector<int2> tT, tA;
+    using igaos::batch::LkEnt;
+    std::vector<LkEnt> tT, tA;
+    // one entry per row block b of an instance's table rb (pairs of first
+    // row, first nonzero)
+    auto entry = [](int lp, const int* rb, int b) {
+        LkEnt e{};
+        e.lp = lp;
+        e.r0 = rb[2 * b];
+        e.p0 = rb[2 * b + 1];
+        e.r1 = rb[2 * b + 2];
+        e.p1 = rb[2 * b + 3];
+        return e;
+    };
     std::vector<LsRange> rng;
     while (!active.empty()) {
         // tables of the active instances' row blocks
@@ -230,8 +297,8 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
             const LPDesc& d = B->desc[lp];
             LsRange r{lp, static_cast<int>(tT.size()), d.nblkT,
                       static_cast<int>(tA.size()), d.nblkA};
-            for (int b = 0; b < d.nblkT; ++b) tT.push_back(make_int2(lp, b));
-            for (int b = 0; b < d.nblkA; ++b) tA.push_back(make_int2(lp, b));
+            for (int b = 0; b < d.nblkT; ++b) tT.push_back(entry(lp, B->rbT.data() + d.rbto, b));
+            for (int b = 0; b < d.nblkA; ++b) tA.push_back(entry(lp, B->rbA.data() + d.rbao, b));
             rng.push_back(r);
         }
         const int nact = static_cast<int>(active.size());
@@ -244,11 +311,14 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
             bp += d.nnz * (wa + 4) + 4.0 * (d.n + 1) + 7.0 * d.n * wv;
             bd += d.nnz * (wa + 4) + 4.0 * (d.m + 1) + 8.0 * d.m * wv;
         }
-        BCK(hipMemcpyAsync(B->d_tblT, tT.data(), sizeof(int2) * nbT, hipMemcpyHostToDevice, s));
-        BCK(hipMemcpyAsync(B->d_tblA, tA.data(), sizeof(int2) * nbA, hipMemcpyHostToDevice, s));
+        BCK(hipMemcpyAsync(B->d_tblT, tT.data(), sizeof(LkEnt) * nbT, hipMemcpyHostToDevice, s));
+        BCK(hipMemcpyAsync(B->d_tblA, tA.data(), sizeof(LkEnt) * nbA, hipMemcpyHostToDevice, s));
         BCK(hipMemcpyAsync(B->d_rng, rng.data(), sizeof(LsRange) * nact,
                            hipMemcpyHostToDevice, s));
         BCK(hipMemsetAsync(B->d_nfrozen, 0, sizeof(int), s));
+        if (B->idx16 && periods_done == 0)  // values in gather order, once per run
+            igaos::batch::lockstep_sort_values(A, B->d_desc, B->d_tblT, nbT, B->d_tblA,
+                                               nbA, s);
         // attempt rounds until every active instance froze on its check
         // boundary (64 accepted iterations, plus retries)
         int chunk = igaos::pdhg::CHECK_EVERY;
@@ -279,6 +349,7 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
             if (nf >= nact) break;
             chunk = 4;
         }
+        ++periods_done;
         igaos::batch::lockstep_check(A, P, B->d_rng, nact, B->d_ctl, B->d_done, s);
         BCK(hipGetLastError());
         BCK(hipMemcpyAsync(flags.data(), B->d_done, sizeof(int) * nact,
@@ -305,8 +376,8 @@ void lockstep_alloc(igaos_batch* B) {
         ba += d.nblkA;
     }
     B->d_ctl = M.alloc<unsigned char>(igaos::batch::lockstep_ctl_bytes() * B->count);
-    B->d_tblT = M.alloc<int2>(bt);
-    B->d_tblA = M.alloc<int2>(ba);
+    B->d_tblT = M.alloc<igaos::batch::LkEnt>(bt);
+    B->d_tblA = M.alloc<igaos::batch::LkEnt>(ba);
     B->d_rng = M.alloc<igaos::batch::LsRange>(B->count);
     B->d_partT = M.alloc<double>(bt);
     B->d_partA = M.alloc<double>(2 * ba);

@@ -143,13 +143,21 @@ __device__ __forceinline__ void block_rows(const RowBlocks& rb, int blk,
 // Single row longer than NNZ_MAX: the workgroup strides over it four
 // loads deep per thread.  Column indices are int, or 16-bit where every
 // index fits (the batched solver's instances).
+//
+// `slot` (optional): the block's nonzeros are stored in another order
+// (idx, val permuted alike) and nonzero q of the block belongs at position
+// slot[q] of the block's row-major order; products are placed there, so
+// every row sums exactly as without the permutation.  A block of a single
+// row keeps its own order (slot[q] = q).
 template <typename Acc, typename Val, typename G, typename I>
 __device__ __forceinline__ void rowblock_sums(const int* __restrict__ ptr,
                                               const I* __restrict__ idx,
                                               const Val* __restrict__ val,
                                               const G& gx, int r0, int r1,
                                               int p0, int p1, Acc* s_rows,
-                                              Acc* s_prod, Acc* red) {
+                                              Acc* s_prod, Acc* red,
+                                              const unsigned short* __restrict__ slot =
+                                                  nullptr) {
     const int tid = threadIdx.x;
     const int nrows = r1 - r0;
     const int nz = p1 - p0;
@@ -194,13 +202,27 @@ __device__ __forceinline__ void rowblock_sums(const int* __restrict__ ptr,
             ci[u] = idx[p0 + q];
             vv[u] = val[p0 + q];
         }
-        Acc xv[U];
+        if (slot) {
+            int sq[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = static_cast<Acc>(gx(ci[u]));
+            for (int u = 0; u < U; ++u) sq[u] = slot[p0 + min(tid + u * BS, last)];
+            Acc xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = tid + u * BS;
-            if (q < nz) s_prod[q] = static_cast<Acc>(vv[u]) * xv[u];
+            for (int u = 0; u < U; ++u) xv[u] = static_cast<Acc>(gx(ci[u]));
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = tid + u * BS;
+                if (q < nz) s_prod[sq[u]] = static_cast<Acc>(vv[u]) * xv[u];
+            }
+        } else {
+            Acc xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = static_cast<Acc>(gx(ci[u]));
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = tid + u * BS;
+                if (q < nz) s_prod[q] = static_cast<Acc>(vv[u]) * xv[u];
+            }
         }
     }
     __syncthreads();
