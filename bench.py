@@ -53,12 +53,21 @@ def bytes_per_iteration(m, n, nnz, w_a, w):
     return 2 * nnz * (w_a + 4) + 4 * (m + n + 2) + w * (7 * n + 8 * m)
 
 
-def load_traffic(name):
+def load_traffic(name, alg_bytes=None):
+    """HBM bytes per launch from the committed rocprofv3 counter summary:
+    measured directly for a fixed launch, or, where the summary holds the
+    ratio of measured to algorithmic bytes on full launches, that ratio
+    times this launch's algorithmic bytes."""
     path = os.path.join(ROOT, "profiles", f"{name}_pmc.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        d = json.load(f)
+    if "hbm_bytes_per_launch" in d:
+        return d["hbm_bytes_per_launch"]
+    if alg_bytes is not None and "traffic_over_algorithmic" in d:
+        return int(round(d["traffic_over_algorithmic"] * alg_bytes))
+    return None
 
 
 # ---------------------------------------------------------------- batch --
@@ -212,8 +221,8 @@ def bench_batch(args):
                                 "lockstep rounds)"),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": load_traffic("k_batch_batch8192" if args.mode == "persistent"
-                                             else "lockstep_batch8192"),
+                     "traffic": (None if lk is None else
+                                 load_traffic("lockstep_batch8192", lk["dual_bytes"])),
                      "bytes_per_launch": alg if lk is None else lk["dual_bytes"],
                      "ms_per_launch": round(launch_ms if lk is None else lk["dual_ms"], 4),
                      "lockstep_samples": None if lk is None else lk,
