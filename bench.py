@@ -127,6 +127,7 @@ def bench_batch(args):
     value = solved / elapsed
     its = np.array(sol.iterations)
     att = np.array(sol.attempts)
+    ok32 = np.array([st == "near-optimal" for st in sol.status])
     alg = float((att * bytes_per_iteration(ms, ns, nnz, 4, 4)).sum())
     launch_ms = kernel_ms / args.steps
     achieved_run = alg / (launch_ms * 1e-3) / 1e9
@@ -140,12 +141,15 @@ def bench_batch(args):
     sample = list(range(0, count, max(1, count // args.fp64_sample)))
     t64 = 0.0
     solved64 = 0
+    rel32 = []  # batched FP32+refine objective vs the plain FP64 engine's
     for i in sample:
         e = igaos_amd.Engine(lps[i], precision=64)
         s = e.run(time_limit=args.time_limit, tolerance=args.tolerance)
         e.close()
         t64 += s.solve_time_ms * 1e-3
         solved64 += s.status == "near-optimal"
+        if s.status == "near-optimal" and sol.status[i] == "near-optimal":
+            rel32.append(abs(sol.objective[i] - s.objective) / (1 + abs(s.objective)))
     fp64_rate = solved64 / t64 if t64 > 0 else 0.0
     log(f"[bench] plain FP64 per instance: {solved64}/{len(sample)} solved "
         f"in {t64:.1f} s -> {fp64_rate:.2f} LPs/s")
@@ -207,12 +211,21 @@ def bench_batch(args):
                        "nnz_class_edges": sched.edges,
                        "waves_per_simd": sched.waves}},
         "solved": solved, "instances": count * args.steps,
+        "iterations_per_s": float(its.sum()) / (elapsed / args.steps),
         "iterations_median": float(np.median(its)),
         "iterations_p99": float(np.percentile(its, 99)),
         "launch_ms": launch_ms,
         "plain_fp64_per_instance": {
             "value": fp64_rate, "unit": "LPs/s", "sample": sample,
             "solved": solved64, "solve_seconds": round(t64, 3)},
+        "fp32_refined_error": {
+            "objective_vs_fp64_engine_rel_max": max(rel32) if rel32 else None,
+            "compared": len(rel32),
+            "certified_pinf_max": float(np.max(np.where(ok32, sol.pinf, 0.0))),
+            "certified_dinf_max": float(np.max(np.where(ok32, sol.dinf, 0.0))),
+            "certified_rel_gap_max": float(np.max(np.where(ok32, sol.rel_gap, 0.0))),
+            "note": ("FP64 KKT of every certified batched result (original units), "
+                     "and its objective against the plain FP64 engine on the sample")},
         "batched_vs_plain_fp64": (value / fp64_rate) if fp64_rate > 0 else None,
         "roofline": {"bound": "hbm",
                      "kernel": ("k_batch<float> (one persistent launch)"

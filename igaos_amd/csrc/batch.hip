@@ -629,7 +629,7 @@ int batch_occupancy(bool f32, int waves) {
 // ---------------------------------------------------------------------------
 // Lockstep mode.  Every active instance advances by one line-search attempt
 // per round; a round is three launches over the row blocks of all active
-// instances (tables of (instance, local block) pairs), so thousands of
+// instances (one table entry per block: instance, rows, nonzeros), so thousands of
 // small SpMVs run as a few wide, low-register kernels that keep HBM busy,
 // instead of one workgroup walking its instance's blocks one after another.
 // The attempt kernels follow the single-LP engine's k_primal / k_dual

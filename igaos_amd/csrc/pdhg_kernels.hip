@@ -2,7 +2,7 @@
 // dual steps with the on-device line-search decision, KKT evaluation in
 // original units, Ruiz equilibration and fixed-order reductions.
 //
-// Per accepted iteration the engine launches two kernels:
+// Per attempt the engine launches three kernels:
 //   k_primal : A'y (transposed CSR = CSC of A) fused with the primal step
 //              x' = clamp(x - tau (g + A'y), lo, hi), xbar = 2x' - x and the
 //              |dx|^2 partial (src/pdhg/pdhg.cu:32-45), plus the pending
@@ -11,11 +11,14 @@
 //              y' = t - sigma clamp((t + yoff)/sigma, rlo, rhi),
 //              t = y + sigma A xbar (pdhg.cu:47-61), Adx = (A xbar - Ax)/2,
 //              the |dy|^2 and dy'Adx partials, the pending averaging of y
-//              and the A x commit; its last workgroup reduces all partials
-//              in a fixed order and applies the line-search rule
-//              (pdhg.cu:751-794) to the device-resident LsState.
+//              and the A x commit;
+//   k_decide : one workgroup reduces all partials in a fixed order and
+//              applies the line-search rule (pdhg.cu:751-794) to the
+//              device-resident LsState.
 // A rejected attempt re-runs both kernels from the same (x, y); k_primal
-// then reuses the stored A'y.
+// then reuses the stored A'y.  On large LPs either product may instead run
+// column-sliced (k_primal_slices / k_dual_slices plus an epilogue kernel
+// with the same step, below).
 
 #include "dev_common.hpp"
 #include "pdhg_kernels.hpp"
