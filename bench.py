@@ -3,20 +3,24 @@
 Default workload ("batch"): 8,192 independent random LPs (seed 0) with
 m log-uniform in [32, 4096], n = round(m * U[1.5, 3]), 4-8 nonzeros per
 column with values N(0,1), rhs from a feasible x0 ~ U[0,1], box bounds
-[0, 10], c ~ N(0,1) (igaos_amd.lpgen.gen_batch).  A step is one persistent
-launch that solves every instance to relative KKT 1e-6 with FP32 iterations
-and FP64 refinement (per-instance budget --time-limit seconds); ``value``
-is instances certified near-optimal per second of that launch, with the
-batch already resident in HBM.  Beside it: the plain FP64 single-LP
+[0, 10], c ~ N(0,1) (igaos_amd.lpgen.gen_batch).  A step is one batched
+solve of every instance to relative KKT 1e-6 with FP32 iterations and FP64
+refinement (per-instance budget --time-limit seconds), in the execution
+mode the autotuner measures as faster on a probe of the batch (lockstep
+rounds or one persistent launch; --mode fixes it); ``value`` is instances
+certified near-optimal per wall second of the step, with the batch already
+resident in HBM.  Beside it: the plain FP64 single-LP
 engine run instance by instance on a fixed sample (every 512th instance),
 timed with its own HBM-resident setup excluded, extrapolated to LPs/s.
 
 ``roofline`` (bound: HBM) uses the algorithmic byte model per accepted
 iteration of an instance, B = 2 nnz (w_a + 4) + 4 (m + n + 2) +
-w (7n + 8m) with w_a = w = 4 (FP32), summed over every instance's attempts
-of the launch and divided by the launch's duration (HIP events on the
-batch stream).  ``traffic`` is read from a committed rocprofv3 --pmc
-summary for the same command when present.
+w (7n + 8m) with w_a = w = 4 (FP32).  Lockstep mode: the dual attempt
+kernel's share of it over the active instances of sampled rounds, divided
+by those launches' durations (HIP events on the batch stream); persistent
+mode: summed over every instance's attempts and divided by the launch's
+duration.  ``traffic`` comes from the committed rocprofv3 --pmc summary
+when present.
 
 ``--workload zipf`` instead measures the single-LP attempt kernels on the
 badly scaled LP (n = 1,000,000, m = 400,000, clip(Zipf(1.5), 2, 2000)
@@ -86,18 +90,22 @@ def bench_batch(args):
     log(f"[bench] {count} LPs generated in {time.perf_counter() - t:.1f} s: "
         f"sum m={ms.sum()} n={ns.sum()} nnz={nnz.sum()}")
     sched = None
-    if args.mode == "persistent":
+    mode = args.mode
+    tune_s = 0.0
+    if args.mode in ("persistent", "auto"):
         from igaos_amd import autotune
         t = time.perf_counter()
-        sched = autotune.tune(lps, log=log)
-        log(f"[bench] autotuned in {time.perf_counter() - t:.1f} s: tpr by class "
+        sched = autotune.tune(lps, log=log, modes=args.mode == "auto")
+        tune_s = time.perf_counter() - t
+        mode = sched.mode if args.mode == "auto" else "persistent"
+        log(f"[bench] autotuned in {tune_s:.1f} s: mode {mode}, tpr by class "
             f"{sched.tpr} (nnz edges {sched.edges}), waves/SIMD {sched.waves}")
-    b = BatchSolver(lps, precision=32, mode=args.mode)
+    b = BatchSolver(lps, precision=32, mode=mode)
     if sched is not None:
         autotune.apply(b, sched)
     tpr, grid, waves = b.schedule()
-    log(f"[bench] batch resident on the GPU, mode {args.mode}"
-        + (f", persistent grid {grid}" if args.mode == "persistent" else ""))
+    log(f"[bench] batch resident on the GPU, mode {mode}"
+        + (f", persistent grid {grid}" if mode == "persistent" else ""))
     t0 = [0.0]
 
     def prog(done, total):
@@ -122,7 +130,7 @@ def bench_batch(args):
         solved += sol.solved
         log(f"[bench] step {k}: {sol.solved}/{count} solved, launch "
             f"{sol.kernel_ms:.0f} ms")
-    lk = b.lockstep_timing() if args.mode == "lockstep" else None
+    lk = b.lockstep_timing() if mode == "lockstep" else None
     b.close()
     value = solved / elapsed
     its = np.array(sol.iterations)
@@ -204,12 +212,15 @@ def bench_batch(args):
                    "sum_n": int(ns.sum()), "sum_nnz": int(nnz.sum()),
                    "tolerance": args.tolerance,
                    "time_limit_per_instance_s": args.time_limit,
-                   "mode": args.mode,
+                   "mode": mode,
+                   "mode_choice": ("autotuned" if args.mode == "auto" else "fixed"),
                    "schedule": None if sched is None else {
                        "persistent_grid": int(grid),
                        "tpr_by_class": sched.tpr,
                        "nnz_class_edges": sched.edges,
-                       "waves_per_simd": sched.waves}},
+                       "waves_per_simd": sched.waves,
+                       "mode_probe_ms": sched.mode_ms,
+                       "tuning_s": round(tune_s, 1)}},
         "solved": solved, "instances": count * args.steps,
         "iterations_per_s": float(its.sum()) / (elapsed / args.steps),
         "iterations_median": float(np.median(its)),
@@ -229,7 +240,7 @@ def bench_batch(args):
         "batched_vs_plain_fp64": (value / fp64_rate) if fp64_rate > 0 else None,
         "roofline": {"bound": "hbm",
                      "kernel": ("k_batch<float> (one persistent launch)"
-                                if args.mode == "persistent" else
+                                if mode == "persistent" else
                                 "k_lk_dual<float> (HIP events around it in sampled "
                                 "lockstep rounds)"),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
@@ -265,7 +276,8 @@ def bench_diff(args):
                                       ratio_hi=2.0, k_lo=6, k_hi=6)
         lps.append(lpgen.perturb_nondegenerate(lp, seed=i, scale=1e-6))
     log(f"[bench] {count} LPs generated in {time.perf_counter() - t:.1f} s")
-    layer = DiffLPBatch(lps, precision=32, mode=args.mode, time_limit=args.time_limit)
+    mode = "lockstep" if args.mode == "auto" else args.mode
+    layer = DiffLPBatch(lps, precision=32, mode=mode, time_limit=args.time_limit)
     c, rmin, rmax, ax = layer.data(torch.float32)
     w = torch.from_numpy(np.random.default_rng(7).standard_normal(c.numel())).float()
     fwd = bwd = 0.0
@@ -296,7 +308,7 @@ def bench_diff(args):
         "data": "synthetic",
         "config": {"workload": "512 LPs, m log-uniform [256,1024], n=2m, 6 nnz/col, "
                                "nondegenerate perturbation 1e-6, seed 0",
-                   "tolerance": layer.tolerance, "mode": args.mode},
+                   "tolerance": layer.tolerance, "mode": mode},
         "forward_s": fwd / args.steps, "backward_s": bwd / args.steps,
         "certified": sol.solved, "polished": int(np.sum(pinfo["accepted"])),
         "adjoint_cgls_iterations_max": int(max(ainfo["iters_p"][ok].max(),
@@ -402,8 +414,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="batch", choices=["batch", "zipf", "diff"])
     ap.add_argument("--count", type=int, default=8192)
-    ap.add_argument("--mode", default="lockstep", choices=["lockstep", "persistent"],
-                    help="batched execution mode (igaos_batch_set_mode)")
+    ap.add_argument("--mode", default="auto", choices=["auto", "lockstep", "persistent"],
+                    help="batched execution mode (igaos_batch_set_mode); auto: "
+                         "the autotuner measures both on a probe and picks")
     ap.add_argument("--tolerance", type=float, default=1e-6)
     ap.add_argument("--time-limit", type=float, default=60.0)
     ap.add_argument("--attempts", type=int, default=20)

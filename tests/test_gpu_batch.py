@@ -125,6 +125,17 @@ def test_autotuner_picks_a_valid_schedule(tmp_path):
     s = b.run(tolerance=1e-6, time_limit=60)
     b.close()
     assert s.solved == len(lps)
+    # with the mode choice: both modes timed on the probe, the faster one set
+    sm = autotune.tune(lps, classes=2, per_class=6, iterations=64, modes=True,
+                       mode_probe=24)
+    assert set(sm.mode_ms) == set(autotune.MODES)
+    assert sm.mode == min(sm.mode_ms, key=sm.mode_ms.get)
+    b = BatchSolver(lps, precision=32)
+    autotune.apply(b, sm)
+    assert b.mode == sm.mode
+    s = b.run(tolerance=1e-6, time_limit=60)
+    b.close()
+    assert s.solved == len(lps)
 
 
 def test_lockstep_iteration_cap_and_modes_agree():
