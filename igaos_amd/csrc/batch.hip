@@ -725,26 +725,20 @@ __global__ __launch_bounds__(NT, LK_WAVES<V>) void k_lk_primal(const Arrays<V>* 
     const V* __restrict__ y = L.y(cur);
     const int r0 = e.r0, p0 = e.p0, r1 = e.r1, p1 = e.p1;
     V* rows = reinterpret_cast<V*>(S.rows);
-    if (fresh) {
-        if constexpr (GATHER_ORDERED<I>)
-            dev::rowblock_sums<V>(L.cp(), L.cis16(), L.facxs(), dev::GatherPlain<V>{y}, r0,
-                                  r1, p0, p1, rows, reinterpret_cast<V*>(S.prod),
-                                  reinterpret_cast<V*>(S.red), L.cslot());
-        else
-            dev::rowblock_sums<V>(L.cp(), L.ci(), L.facx(), dev::GatherPlain<V>{y}, r0, r1,
-                                  p0, p1, rows, reinterpret_cast<V*>(S.prod),
-                                  reinterpret_cast<V*>(S.red));
-    }
+    // A'y is recomputed on a retry too (same y, same sums) rather than
+    // stored for it: retries are rare, the store would be paid every round
+    if constexpr (GATHER_ORDERED<I>)
+        dev::rowblock_sums<V>(L.cp(), L.cis16(), L.facxs(), dev::GatherPlain<V>{y}, r0, r1,
+                              p0, p1, rows, reinterpret_cast<V*>(S.prod),
+                              reinterpret_cast<V*>(S.red), L.cslot());
+    else
+        dev::rowblock_sums<V>(L.cp(), L.ci(), L.facx(), dev::GatherPlain<V>{y}, r0, r1, p0,
+                              p1, rows, reinterpret_cast<V*>(S.prod),
+                              reinterpret_cast<V*>(S.red));
     double acc = 0.0;
     for (int t = threadIdx.x; t < r1 - r0; t += NT) {
         const int j = r0 + t;
-        V a;
-        if (fresh) {
-            a = rows[t];
-            L.ata()[j] = a;
-        } else {
-            a = L.ata()[j];
-        }
+        const V a = rows[t];
         const V xo = x[j];
         if (mix) {
             const V xa = L.xavg()[j];
