@@ -850,11 +850,12 @@ __global__ __launch_bounds__(NT) void k_lk_decide(const LsRange* rng, int nact,
 // The check of every active instance (all frozen): one workgroup each.
 template <typename V>
 __global__ __launch_bounds__(NT) void k_lk_check(const Arrays<V>* A, Params P,
-                                                 const LsRange* rng, Ctl* ctl,
-                                                 int* done) {
+                                                 const LsRange* rng, const int* ord,
+                                                 Ctl* ctl, int* done) {
     __shared__ Ctl C;
     __shared__ Lds S;
-    const int id = rng[blockIdx.x].lp;
+    const int a = ord[blockIdx.x];
+    const int id = rng[a].lp;
     if (threadIdx.x == 0) C = ctl[id];
     __syncthreads();
     const LPDesc D = P.lps[id];
@@ -874,7 +875,7 @@ __global__ __launch_bounds__(NT) void k_lk_check(const Arrays<V>* A, Params P,
         if (stop) C.done = 1;
         else C.st.frozen = 0;
         ctl[id] = C;
-        done[blockIdx.x] = C.done;
+        done[a] = C.done;
     }
 }
 
@@ -922,8 +923,8 @@ void lockstep_sort_values(const Arrays<double>* a, const LPDesc* lps, const LkEn
 
 template <typename V>
 static void lk_check_impl(const Arrays<V>* a, const Params& p, const LsRange* rng,
-                          int nact, void* ctl, int* done, hipStream_t s) {
-    k_lk_check<V><<<nact, NT, 0, s>>>(a, p, rng, static_cast<Ctl*>(ctl), done);
+                          const int* ord, int nact, void* ctl, int* done, hipStream_t s) {
+    k_lk_check<V><<<nact, NT, 0, s>>>(a, p, rng, ord, static_cast<Ctl*>(ctl), done);
 }
 
 void lockstep_setup(const Arrays<float>* a, const Params& p, void* ctl, int* done,
@@ -945,12 +946,12 @@ void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const LkEnt* tbl
                   idx16, s);
 }
 void lockstep_check(const Arrays<float>* a, const Params& p, const LsRange* rng,
-                    int nact, void* ctl, int* done, hipStream_t s) {
-    lk_check_impl(a, p, rng, nact, ctl, done, s);
+                    const int* ord, int nact, void* ctl, int* done, hipStream_t s) {
+    lk_check_impl(a, p, rng, ord, nact, ctl, done, s);
 }
 void lockstep_check(const Arrays<double>* a, const Params& p, const LsRange* rng,
-                    int nact, void* ctl, int* done, hipStream_t s) {
-    lk_check_impl(a, p, rng, nact, ctl, done, s);
+                    const int* ord, int nact, void* ctl, int* done, hipStream_t s) {
+    lk_check_impl(a, p, rng, ord, nact, ctl, done, s);
 }
 
 }  // namespace igaos::batch

@@ -124,9 +124,12 @@ void lockstep_round(const Arrays<double>* a, const LPDesc* lps, const LkEnt* tbl
                     int nbT, const LkEnt* tblA, int nbA, const LsRange* rng, int nact,
                     void* ctl, double* partT, double* partA, int* nfrozen,
                     hipEvent_t* ev, bool idx16, hipStream_t s);
+// The check of every active instance; workgroup b takes active instance
+// ord[b] (largest first, so the longest checks start first) and writes
+// done[ord[b]].
 void lockstep_check(const Arrays<float>* a, const Params& p, const LsRange* rng,
-                    int nact, void* ctl, int* done, hipStream_t s);
+                    const int* ord, int nact, void* ctl, int* done, hipStream_t s);
 void lockstep_check(const Arrays<double>* a, const Params& p, const LsRange* rng,
-                    int nact, void* ctl, int* done, hipStream_t s);
+                    const int* ord, int nact, void* ctl, int* done, hipStream_t s);
 
 }  // namespace igaos::batch

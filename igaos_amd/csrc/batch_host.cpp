@@ -94,6 +94,7 @@ struct igaos_batch {
     double* d_partT = nullptr;
     double* d_partA = nullptr;
     int* d_done = nullptr;
+    int* d_ord = nullptr;  // check launch order of the active instances
     int* d_nfrozen = nullptr;
     // lockstep timing samples: rounds 0, 16, 32, 48 of each check period
     // (every active instance works in those), HIP events around the two
@@ -288,6 +289,7 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
         return e;
     };
     std::vector<LsRange> rng;
+    std::vector<int> ord;
     while (!active.empty()) {
         // tables of the active instances' row blocks
         tT.clear();
@@ -315,6 +317,15 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
         BCK(hipMemcpyAsync(B->d_tblA, tA.data(), sizeof(LkEnt) * nbA, hipMemcpyHostToDevice, s));
         BCK(hipMemcpyAsync(B->d_rng, rng.data(), sizeof(LsRange) * nact,
                            hipMemcpyHostToDevice, s));
+        // checks of the largest instances first (longest workgroups start early)
+        ord.resize(nact);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
+            const LPDesc& du = B->desc[active[u]];
+            const LPDesc& dv = B->desc[active[v]];
+            return du.nnz + du.m + du.n > dv.nnz + dv.m + dv.n;
+        });
+        BCK(hipMemcpyAsync(B->d_ord, ord.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
         BCK(hipMemsetAsync(B->d_nfrozen, 0, sizeof(int), s));
         if (B->idx16 && periods_done == 0)  // values in gather order, once per run
             igaos::batch::lockstep_sort_values(A, B->d_desc, B->d_tblT, nbT, B->d_tblA,
@@ -350,7 +361,7 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
             chunk = 4;
         }
         ++periods_done;
-        igaos::batch::lockstep_check(A, P, B->d_rng, nact, B->d_ctl, B->d_done, s);
+        igaos::batch::lockstep_check(A, P, B->d_rng, B->d_ord, nact, B->d_ctl, B->d_done, s);
         BCK(hipGetLastError());
         BCK(hipMemcpyAsync(flags.data(), B->d_done, sizeof(int) * nact,
                            hipMemcpyDeviceToHost, s));
@@ -382,6 +393,7 @@ void lockstep_alloc(igaos_batch* B) {
     B->d_partT = M.alloc<double>(bt);
     B->d_partA = M.alloc<double>(2 * ba);
     B->d_done = M.alloc<int>(B->count);
+    B->d_ord = M.alloc<int>(B->count);
     B->d_nfrozen = M.alloc<int>(1);
 }
 
