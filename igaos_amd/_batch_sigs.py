@@ -50,4 +50,7 @@ SIGNATURES = {
     "igaos_batch_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, C.c_int,
                                       _dp, _dp, _dp, _dp, _dp, _dp,
                                       C.POINTER(AdjointInfo)]),
+    "igaos_batch_unroll": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "igaos_batch_unroll_backward": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                              _dp, _dp, _dp]),
 }

@@ -17,6 +17,16 @@ rmin and rmax (``solve_eq``): autograd adds the two halves to dL/db.
 
 The column bounds are fixed data; their gradients are available from
 ``BatchSolver.adjoint`` (gcl, gcu) but not through this function.
+
+``differentiation="unrolled"`` replaces the solve by ``unroll_iterations``
+steps of fixed-step PDHG from x = y = 0 (igaos_batch_unroll, FP64, iterates
+checkpointed on the device every 64 steps) and returns that iterate; the
+backward is reverse mode through those steps (igaos_batch_unroll_backward,
+each 64-step segment recomputed from its checkpoint).  The step sizes are
+``unroll_steps = (tau, sigma)`` or, by default, those of the first call's
+data, held fixed for later calls, so the layer is one fixed K-step map and
+its gradients are that map's own, exact to rounding, whether or not the
+iterate has converged.
 """
 from __future__ import annotations
 
@@ -50,6 +60,14 @@ class _SolveFn(torch.autograd.Function):
         if prob._generation != ctx.generation:
             # a later forward replaced the device state: solve this one again
             prob._forward(_host(c), _host(rmin), _host(rmax), _host(a_values))
+        t = lambda a, like: torch.from_numpy(a).to(like.device, like.dtype)  # noqa
+        if prob.differentiation == "unrolled":
+            g = prob.solver.unroll_backward(
+                gx=None if gx is None else _host(gx),
+                gy=None if gy is None else _host(gy),
+                gobj=None if gobj is None else _host(gobj))
+            return (None, t(g["gc"], c), t(g["grmin"], rmin), t(g["grmax"], rmax),
+                    t(g["gax"], a_values))
         g = prob.solver.adjoint(
             gx=None if gx is None else _host(gx),
             gy=None if gy is None else _host(gy),
@@ -57,7 +75,6 @@ class _SolveFn(torch.autograd.Function):
             tolerance=prob.adjoint_tolerance,
             max_iterations=prob.adjoint_max_iterations)
         prob.last_adjoint_info = g["info"]
-        t = lambda a, like: torch.from_numpy(a).to(like.device, like.dtype)  # noqa
         return (None, t(g["gc"], c), t(g["grmin"], rmin), t(g["grmax"], rmax),
                 t(g["gax"], a_values))
 
@@ -68,7 +85,14 @@ class DiffLPBatch:
 
     def __init__(self, lps, precision=32, tolerance=1e-6, time_limit=60.0,
                  max_iterations=500000, adjoint_tolerance=1e-12,
-                 adjoint_max_iterations=None, polish=True, mode="persistent"):
+                 adjoint_max_iterations=None, polish=True, mode="persistent",
+                 differentiation="implicit", unroll_iterations=1024, unroll_steps=None):
+        if differentiation not in ("implicit", "unrolled"):
+            raise ValueError("differentiation must be 'implicit' or 'unrolled'")
+        self.differentiation = differentiation
+        self.unroll_iterations = int(unroll_iterations)
+        self.unroll_steps = unroll_steps
+        self.last_unroll = None
         self.lps = list(lps)
         self.solver = BatchSolver(self.lps, precision=precision, mode=mode)
         self.polish = polish
@@ -87,6 +111,15 @@ class DiffLPBatch:
 
     def _forward(self, c, rmin, rmax, ax):
         self.solver.update(c=c, rmin=rmin, rmax=rmax, ax=ax)
+        if self.differentiation == "unrolled":
+            u = self.solver.unroll(self.unroll_iterations, steps=self.unroll_steps)
+            if self.unroll_steps is None:  # from the first call's data, then held
+                self.unroll_steps = (u["tau"].copy(), u["sigma"].copy())
+            self._generation += 1
+            self.last_unroll = u
+            self._x, self._y = u["x"], u["y"]
+            self._obj = u["objective"]
+            return
         sol = self.solver.run(tolerance=self.tolerance, time_limit=self.time_limit,
                               max_iterations=self.max_iterations)
         self._generation += 1

@@ -251,6 +251,48 @@ class BatchSolver:
         out["info"] = self._info(info)
         return out
 
+    def unroll(self, iterations, steps=None) -> dict:
+        """Fixed-step PDHG unrolled for ``iterations`` steps on the current
+        data (include/igaos_pdhg.h, igaos_batch_unroll), with step sizes
+        ``steps = (tau, sigma)`` (per instance) or, if None, the ones the
+        data give: returns x, y (concatenated), objective, tau and sigma."""
+        L = _core.lib()
+        n, m = int(self._coff[-1]), int(self._roff[-1])
+        out = dict(x=np.zeros(n), y=np.zeros(m), objective=np.zeros(self.count),
+                   tau=np.zeros(self.count), sigma=np.zeros(self.count))
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+        st = None
+        if steps is not None:
+            st = np.ascontiguousarray(np.concatenate([
+                np.broadcast_to(np.asarray(s, dtype=np.float64), (self.count,))
+                for s in steps]))
+        rc = L.igaos_batch_unroll(self._h, int(iterations), None if st is None else dp(st),
+                                  *[dp(out[k]) for k in ("x", "y", "objective", "tau",
+                                                         "sigma")])
+        if rc != 0:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+        return out
+
+    def unroll_backward(self, gx=None, gy=None, gobj=None) -> dict:
+        """Reverse mode through the last ``unroll`` (igaos_batch_unroll_backward):
+        returns gc, grmin, grmax, gcl, gcu, gax (gax in CSR order)."""
+        L = _core.lib()
+        n, m, z = int(self._coff[-1]), int(self._roff[-1]), int(self._zoff[-1])
+        ins = {}
+        for key, v, size in (("gx", gx, n), ("gy", gy, m), ("gobj", gobj, self.count)):
+            if v is not None:
+                ins[key] = np.ascontiguousarray(np.broadcast_to(
+                    np.asarray(v, dtype=np.float64), (size,)))
+        out = dict(gc=np.zeros(n), grmin=np.zeros(m), grmax=np.zeros(m),
+                   gcl=np.zeros(n), gcu=np.zeros(n), gax=np.zeros(z))
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+        rc = L.igaos_batch_unroll_backward(
+            self._h, *[dp(ins[k]) if k in ins else None for k in ("gx", "gy", "gobj")],
+            *[dp(out[k]) for k in ("gc", "grmin", "grmax", "gcl", "gcu", "gax")])
+        if rc != 0:
+            raise RuntimeError(L.igaos_batch_last_error().decode())
+        return out
+
     def close(self):
         h = getattr(self, "_h", None)
         lib = getattr(_core, "_lib", None) if _core is not None else None

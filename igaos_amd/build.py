@@ -21,7 +21,7 @@ CLI = os.path.join(HERE, "igaos")
 
 LIB_SOURCES = ["mps.cpp", "repair.cpp", "engine.cpp", "capi.cpp",
                "pdhg_kernels.hip", "batch.hip", "batch_host.cpp",
-               "adjoint.hip"]
+               "adjoint.hip", "unroll.hip"]
 CLI_SOURCES = ["cli.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstring>
 #include <memory>
@@ -16,6 +17,7 @@
 
 #include "adjoint.hpp"
 #include "batch.hpp"
+#include "unroll.hpp"
 #include "igaos_pdhg.h"
 #include "pdhg_kernels.hpp"
 
@@ -103,6 +105,12 @@ struct igaos_batch {
     long lk_samples = 0;
     double lk_primal_ms = 0.0, lk_dual_ms = 0.0;
     double lk_primal_bytes = 0.0, lk_dual_bytes = 0.0;
+    Mem umem;             // unrolled-mode workspace, allocated on first use
+    std::unique_ptr<Mem> ckmem;  // its checkpoints (sized for un_cap steps)
+    igaos::batch::UnrollArrays un{};
+    igaos::batch::UnrollArrays* d_un = nullptr;
+    int un_cap = -1;      // steps the checkpoint slabs hold
+    int un_K = -1;        // steps of the last forward on the current data (-1: none)
     Mem amem;             // adjoint workspace, allocated on first use
     igaos::batch::AdjArrays adj{};
     igaos::batch::AdjArrays* d_adj = nullptr;
@@ -688,6 +696,7 @@ int igaos_batch_update(igaos_batch* B, const double* c, const double* cl,
         if (B->precision == 32) go(B->a32);
         else go(B->a64);
         B->solved = false;
+        B->un_K = -1;
     } catch (const std::exception& e) {
         return bfail(e.what());
     }
@@ -695,6 +704,40 @@ int igaos_batch_update(igaos_batch* B, const double* c, const double* cl,
 }
 
 namespace {
+
+void unroll_alloc(igaos_batch* B, int K) {
+    using igaos::batch::UNROLL_SEG;
+    const long m = B->sum_m, n = B->sum_n, z = B->sum_z;
+    igaos::batch::UnrollArrays& U = B->un;
+    if (!B->d_un) {
+        Mem& M = B->umem;
+        U.segx = M.alloc<double>(static_cast<size_t>(UNROLL_SEG) * n);
+        U.segy = M.alloc<double>(static_cast<size_t>(UNROLL_SEG) * m);
+        for (double** p : {&U.ax, &U.xb, &U.gb, &U.tmpn, &U.xK, &U.gc, &U.gcl, &U.gcu})
+            *p = M.alloc<double>(n);
+        for (double** p : {&U.ay, &U.tb, &U.tmpm, &U.yK, &U.grmin, &U.grmax})
+            *p = M.alloc<double>(m);
+        U.cx = M.alloc<unsigned char>(n);
+        U.cy = M.alloc<unsigned char>(m);
+        for (double** p : {&U.tau, &U.sigma, &U.obj}) *p = M.alloc<double>(B->count);
+        U.gx = M.alloc<double>(n);
+        U.gy = M.alloc<double>(m);
+        U.gobj = M.alloc<double>(B->count);
+        U.gax = M.alloc<double>(z);
+        U.sum_n = n;
+        U.sum_m = m;
+        B->d_un = M.alloc<igaos::batch::UnrollArrays>(1);
+    }
+    if (K > B->un_cap) {  // checkpoints at steps 0, 64, ... < K
+        const long nck = std::max(1L, (static_cast<long>(K) + UNROLL_SEG - 1) / UNROLL_SEG);
+        B->ckmem.reset();
+        B->ckmem = std::make_unique<Mem>();
+        U.ckx = B->ckmem->alloc<double>(nck * n);
+        U.cky = B->ckmem->alloc<double>(nck * m);
+        B->un_cap = K;
+    }
+    U.K = K;
+}
 
 void adjoint_alloc(igaos_batch* B) {
     if (B->d_adj) return;
@@ -775,6 +818,87 @@ int igaos_batch_polish(igaos_batch* B, double tolerance, int max_iterations,
         if (info)
             BCK(hipMemcpy(info, J.info, sizeof(igaos::batch::AdjInfo) * B->count,
                           hipMemcpyDeviceToHost));
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
+
+int igaos_batch_unroll(igaos_batch* B, int iterations, const double* steps, double* x,
+                       double* y, double* objective, double* tau, double* sigma) {
+    if (!B) return bfail("null batch");
+    if (B->running) return bfail("batch is running");
+    if (iterations < 0) return bfail("iterations must be >= 0");
+    if (steps)
+        for (int i = 0; i < 2 * B->count; ++i)
+            if (!(steps[i] > 0.0) || !std::isfinite(steps[i]))
+                return bfail("step sizes must be positive and finite");
+    try {
+        B->un_K = -1;
+        unroll_alloc(B, iterations);
+        igaos::batch::UnrollArrays& U = B->un;
+        U.fixed = steps != nullptr;
+        if (steps) {
+            BCK(hipMemcpyAsync(U.tau, steps, sizeof(double) * B->count,
+                               hipMemcpyHostToDevice, B->s));
+            BCK(hipMemcpyAsync(U.sigma, steps + B->count, sizeof(double) * B->count,
+                               hipMemcpyHostToDevice, B->s));
+        }
+        BCK(hipMemcpyAsync(B->d_un, &U, sizeof(U), hipMemcpyHostToDevice, B->s));
+        if (B->precision == 32)
+            igaos::batch::launch_unroll_forward(B->d_a32, B->d_desc, B->count, B->d_un, B->s);
+        else
+            igaos::batch::launch_unroll_forward(B->d_a64, B->d_desc, B->count, B->d_un, B->s);
+        BCK(hipGetLastError());
+        BCK(hipStreamSynchronize(B->s));
+        auto out = [&](double* dst, const double* src, long k) {
+            if (dst) BCK(hipMemcpy(dst, src, sizeof(double) * k, hipMemcpyDeviceToHost));
+        };
+        out(x, U.xK, B->sum_n);
+        out(y, U.yK, B->sum_m);
+        out(objective, U.obj, B->count);
+        out(tau, U.tau, B->count);
+        out(sigma, U.sigma, B->count);
+        B->un_K = iterations;
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
+
+int igaos_batch_unroll_backward(igaos_batch* B, const double* gx, const double* gy,
+                                const double* gobj, double* gc, double* grmin,
+                                double* grmax, double* gcl, double* gcu, double* gax) {
+    if (!B) return bfail("null batch");
+    if (B->running) return bfail("batch is running");
+    if (B->un_K < 0) return bfail("needs igaos_batch_unroll on the current data first");
+    try {
+        igaos::batch::UnrollArrays& U = B->un;
+        auto in = [&](const double* src, const double* dst, long k) {
+            double* d = const_cast<double*>(dst);
+            if (src) BCK(hipMemcpyAsync(d, src, sizeof(double) * k, hipMemcpyHostToDevice, B->s));
+            else BCK(hipMemsetAsync(d, 0, sizeof(double) * k, B->s));
+        };
+        in(gx, U.gx, B->sum_n);
+        in(gy, U.gy, B->sum_m);
+        in(gobj, U.gobj, B->count);
+        U.K = B->un_K;
+        BCK(hipMemcpyAsync(B->d_un, &U, sizeof(U), hipMemcpyHostToDevice, B->s));
+        if (B->precision == 32)
+            igaos::batch::launch_unroll_backward(B->d_a32, B->d_desc, B->count, B->d_un, B->s);
+        else
+            igaos::batch::launch_unroll_backward(B->d_a64, B->d_desc, B->count, B->d_un, B->s);
+        BCK(hipGetLastError());
+        BCK(hipStreamSynchronize(B->s));
+        auto out = [&](double* dst, const double* src, long k) {
+            if (dst) BCK(hipMemcpy(dst, src, sizeof(double) * k, hipMemcpyDeviceToHost));
+        };
+        out(gc, U.gc, B->sum_n);
+        out(gcl, U.gcl, B->sum_n);
+        out(gcu, U.gcu, B->sum_n);
+        out(grmin, U.grmin, B->sum_m);
+        out(grmax, U.grmax, B->sum_m);
+        out(gax, U.gax, B->sum_z);
     } catch (const std::exception& e) {
         return bfail(e.what());
     }

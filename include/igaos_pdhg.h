@@ -271,6 +271,29 @@ int igaos_batch_adjoint(igaos_batch* batch, const double* gx, const double* gy,
                         int max_iterations, double* gc, double* grmin,
                         double* grmax, double* gcl, double* gcu, double* gax,
                         igaos_adjoint_info* info);
+/* Unrolled differentiation (no reference counterpart): `iterations` steps
+ * of fixed-step PDHG on every instance of the current data, in FP64 and
+ * original units, from x = y = 0 (the step of src/pdhg/pdhg.cu:32-61
+ * without line search, restarts or averaging).  The step sizes are
+ * `steps` (host, tau of every instance then sigma of every instance) or,
+ * when it is NULL, tau = 0.9 / (omega |A|), sigma = 0.9 omega / |A| from
+ * the data (|A| from 30 power iterations, omega the engine's primal weight
+ * rule in original units); either way they are constants of the map (no
+ * gradient flows through them).  The iterates are checkpointed every 64
+ * steps on the device.  Outputs (host; any may be NULL): x (sum n), y
+ * (sum m) after the last step, objective c'x + obj_const, and the tau and
+ * sigma used (count each). */
+int igaos_batch_unroll(igaos_batch* batch, int iterations, const double* steps,
+                       double* x, double* y, double* objective, double* tau,
+                       double* sigma);
+/* Reverse mode through the last igaos_batch_unroll on the current data,
+ * recomputing each 64-step segment from its checkpoint: gradients of
+ * L(x_K, y_K, objective) for upstream gx (sum n), gy (sum m), gobj (count),
+ * NULL = zero.  Outputs as for igaos_batch_adjoint (gax in CSR order); a
+ * clamp at a tie passes the gradient to the bound (the lower one first). */
+int igaos_batch_unroll_backward(igaos_batch* batch, const double* gx, const double* gy,
+                                const double* gobj, double* gc, double* grmin,
+                                double* grmax, double* gcl, double* gcu, double* gax);
 const char* igaos_batch_last_error(void);
 void igaos_batch_destroy(igaos_batch* batch);
 
