@@ -277,7 +277,7 @@ def bench_diff(args):
         lps.append(lpgen.perturb_nondegenerate(lp, seed=i, scale=1e-6))
     log(f"[bench] {count} LPs generated in {time.perf_counter() - t:.1f} s")
     mode = "lockstep" if args.mode == "auto" else args.mode
-    layer = DiffLPBatch(lps, precision=32, mode=mode, time_limit=args.time_limit)
+    layer = DiffLPBatch(lps, precision=32, mode=mode, time_limit=args.diff_time_limit)
     c, rmin, rmax, ax = layer.data(torch.float32)
     w = torch.from_numpy(np.random.default_rng(7).standard_normal(c.numel())).float()
     fwd = bwd = 0.0
@@ -297,6 +297,21 @@ def bench_diff(args):
     pinfo = layer.last_polish_info
     ainfo = layer.last_adjoint_info
     layer.close()
+    # the same loss through the unrolled mode: K fixed-step iterations with
+    # device checkpoints every 64, reverse mode through recomputed segments
+    K = args.unroll_iterations
+    ul = DiffLPBatch(lps, precision=32, differentiation="unrolled", unroll_iterations=K)
+    cc, bb, aa = (v.clone().double().requires_grad_(True) for v in (c, rmin, ax))
+    t0 = time.perf_counter()
+    x, y, obj = ul.solve_eq(cc, bb, aa)
+    t1 = time.perf_counter()
+    ((w.double() * x).sum() + 0.5 * (y * y).sum()).backward()
+    t2 = time.perf_counter()
+    ul.close()
+    log(f"[bench] unrolled K={K}: forward {t1 - t0:.2f} s backward {t2 - t1:.2f} s")
+    unrolled = {"iterations": K, "forward_s": t1 - t0, "backward_s": t2 - t1,
+                "checkpoint_every": 64,
+                "instance_iterations_per_s": count * K / (t1 - t0)}
     el = fwd + bwd
     ok = np.array([st == "near-optimal" for st in sol.status]) & (pinfo["accepted"] == 1)
     return {
@@ -308,7 +323,8 @@ def bench_diff(args):
         "data": "synthetic",
         "config": {"workload": "512 LPs, m log-uniform [256,1024], n=2m, 6 nnz/col, "
                                "nondegenerate perturbation 1e-6, seed 0",
-                   "tolerance": layer.tolerance, "mode": mode},
+                   "tolerance": layer.tolerance, "mode": mode,
+                   "time_limit_per_instance_s": args.diff_time_limit},
         "forward_s": fwd / args.steps, "backward_s": bwd / args.steps,
         "certified": sol.solved, "polished": int(np.sum(pinfo["accepted"])),
         "adjoint_cgls_iterations_max": int(max(ainfo["iters_p"][ok].max(),
@@ -316,6 +332,7 @@ def bench_diff(args):
         "adjoint_residual_max": float(max(ainfo["res_p"][ok].max(),
                                           ainfo["res_q"][ok].max())),
         "note": "adjoint statistics over the certified and polished instances",
+        "unrolled_mode": unrolled,
     }
 
 
@@ -418,6 +435,10 @@ def main():
                     help="batched execution mode (igaos_batch_set_mode); auto: "
                          "the autotuner measures both on a probe and picks")
     ap.add_argument("--tolerance", type=float, default=1e-6)
+    ap.add_argument("--diff-time-limit", type=float, default=10.0,
+                    help="diff workload: per-instance budget of the forward solve")
+    ap.add_argument("--unroll-iterations", type=int, default=1024,
+                    help="diff workload: steps of the unrolled-mode measurement")
     ap.add_argument("--time-limit", type=float, default=60.0)
     ap.add_argument("--attempts", type=int, default=20)
     ap.add_argument("--fp64-sample", type=int, default=16)
