@@ -175,6 +175,33 @@ def test_infeasible_and_unbounded_instances_do_not_disturb_the_batch(mode):
         assert np.array_equal(sol.x[i], ref.x[k])  # independent of the batch
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_degenerate_shapes_in_a_batch(mode):
+    # an instance without rows (box-constrained: x_j at the bound c_j
+    # points away from, objective in closed form), one whose matrix has an
+    # empty row and an empty column, and ordinary ones around them
+    n0 = 7
+    rng = np.random.default_rng(5)
+    c0 = rng.standard_normal(n0)
+    cl0, cu0 = -rng.uniform(0, 2, n0), rng.uniform(0, 3, n0)
+    norows = lpgen.LP.from_triplets(0, n0, [], [], [], c0, cl0, cu0, [], [])
+    x0 = np.where(c0 > 0, cl0, cu0)
+    # rows 0..2, row 1 empty; column 3 appears in no row (its bound decides)
+    gaps = lpgen.LP.from_triplets(3, 4, [0, 0, 2, 2], [0, 1, 1, 2], [1.0, 2.0, -1.0, 1.0],
+                                  [1.0, -1.0, 0.5, 2.0], [0.0] * 4, [5.0] * 4,
+                                  [1.0, -np.inf, 0.0], [1.0, np.inf, 0.0])
+    good = [lpgen.gen_batch_instance(8, i, m_hi=200) for i in range(2)]
+    lps = [good[0], norows, gaps, good[1]]
+    sol = solve_batch(lps, precision=32, mode=mode, tolerance=1e-6, time_limit=60)
+    assert all(st == "near-optimal" for st in sol.status), sol.status
+    assert np.allclose(sol.x[1], x0, rtol=0, atol=1e-6)
+    assert abs(sol.objective[1] - c0 @ x0) <= 1e-6 * (1 + abs(c0 @ x0))
+    # gaps: x3 = 0 (c3 > 0, free of rows); rows force x0 + 2 x1 = 1 and
+    # x2 = x1; c.x = x0 - x1 + 0.5 x1 = 1 - 2.5 x1 -> x1 = 0.5, x0 = 0
+    assert np.allclose(sol.x[2], [0.0, 0.5, 0.5, 0.0], rtol=0, atol=1e-5)
+    assert abs(sol.objective[2] - (-0.25)) <= 1e-5
+
+
 @pytest.mark.parametrize("precision,min_certified", [(32, 30), (64, 31)])
 def test_netlib_sweep_as_one_lockstep_batch(precision, min_certified):
     # the 36 small Netlib LPs of the reference's sweep as one batch (60 s,
