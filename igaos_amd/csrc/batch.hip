@@ -68,8 +68,26 @@ __device__ void refresh_refs(const LP<V>& L, Lds& S) {
 template <typename V>
 __device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
                            double cmax, Lds& S) {
-    const double* xr = L.xref();
-    const double* yr = L.yref();
+    // the FP64 iterate gathered by the two SpMVs: an FP32 iterate is
+    // combined with its reference point once, into the output arrays (free
+    // until finish_lp writes them), so that each nonzero gathers one double
+    // instead of a double and a float (same values: x_ref + double(dx))
+    const double* xg;
+    const double* yg;
+    if constexpr (sizeof(V) == 4) {
+        const double* xr = L.xref();
+        const double* yr = L.yref();
+        double* xf = L.xout();
+        double* yf = L.yout();
+        for (int j = threadIdx.x; j < L.n; j += NT) xf[j] = full(xr, xd, j);
+        for (int i = threadIdx.x; i < L.m; i += NT) yf[i] = full(yr, yd, i);
+        __syncthreads();
+        xg = xf;
+        yg = yf;
+    } else {
+        xg = xd;
+        yg = yd;
+    }
     const double dz = 1e-8 * (1.0 + cmax);
     double dv = 0.0, op = 0.0, odc = 0.0, unbc = 0.0;
     double pv = 0.0, pa = 0.0, prow = 0.0, ymax = 0.0;
@@ -78,8 +96,8 @@ __device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
         const double* c = L.c();
         const double* clp = L.cl();
         const double* cup = L.cu();
-        spmv_T<double>(L, L.sacx(), dev::GatherRef<V>{yr, yd}, S, [&](int j, double ata) {
-            const double hx = full(xr, xd, j) * dc[j];
+        spmv_T<double>(L, L.sacx(), dev::GatherPlain<double>{yg}, S, [&](int j, double ata) {
+            const double hx = xg[j] * dc[j];
             const double hd = c[j] + ata / dc[j];
             const double cl = clp[j], cu = cup[j];
             const double bnd = fabs(__builtin_isfinite(cu) ? cu : cl);
@@ -103,7 +121,7 @@ __device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
         const double* dr = L.dr();
         const double* rminp = L.rmin();
         const double* rmaxp = L.rmax();
-        spmv_A<double>(L, L.sax(), dev::GatherRef<V>{xr, xd}, S, [&](int i, double tm) {
+        spmv_A<double>(L, L.sax(), dev::GatherPlain<double>{xg}, S, [&](int i, double tm) {
             const double ax = tm * dr[i];
             const double lo = rminp[i], hi = rmaxp[i];
             const double e = fmax(lo - ax, ax - hi);
@@ -113,7 +131,7 @@ __device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
             if (__builtin_isfinite(lo)) sc = fmax(sc, fabs(lo));
             if (__builtin_isfinite(hi)) sc = fmax(sc, fabs(hi));
             prow = fmax(prow, fmax(0.0, e) / (1.0 + sc));
-            ymax = fmax(ymax, fabs(full(yr, yd, i) * (1.0 / dr[i])));
+            ymax = fmax(ymax, fabs(yg[i] * (1.0 / dr[i])));
         });
     }
     dv = bmax(dv, S.red);
@@ -127,7 +145,7 @@ __device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
     const double yz = 1e-8 * (1.0 + ymax);
     double odr = 0.0, unbr = 0.0;
     for (int i = threadIdx.x; i < L.m; i += NT) {
-        const double hy = full(yr, yd, i) * (1.0 / L.dr()[i]);
+        const double hy = yg[i] * (1.0 / L.dr()[i]);
         if (hy > yz) {
             if (__builtin_isfinite(L.rmax()[i])) odr -= hy * L.rmax()[i];
             else unbr = 1.0;
