@@ -201,6 +201,7 @@ class Engine final : public EngineBase {
     DBuf<V> lo_, hi_, gref_, rlo_, rhi_, yoff_;  // FP32 correction data
     // FP64 references of the FP32 engine and scratch
     DBuf<double> xref_, yref_, aref_, atref_;
+    DBuf<double> kx_[2], ky_[2];  // FP64 iterates of a check (FP32 engine)
     DBuf<double> xb_, yb_, pesx_, pesy_, tmpm_, tmpn_;
     DBuf<double> partT_, partA_, partC_, partR_, partR2_, rowres_, red_, scal_;
     DBuf<unsigned int> tickets_;
@@ -390,6 +391,10 @@ void Engine<V>::setup() {
         ib_.ax = vax_.p;
         ib_.acx = vacx_.p;
         xref_.alloc(n_); yref_.alloc(m_); aref_.alloc(m_); atref_.alloc(n_);
+        for (int q = 0; q < 2; ++q) {
+            kx_[q].alloc(n_);
+            ky_[q].alloc(m_);
+        }
         lo_.alloc(n_); hi_.alloc(n_); gref_.alloc(n_);
         rlo_.alloc(m_); rhi_.alloc(m_); yoff_.alloc(m_);
         ib_.lo = lo_.p; ib_.hi = hi_.p; ib_.gref = gref_.p;
@@ -504,16 +509,28 @@ void Engine<V>::recompute_refs() {
 
 template <typename V>
 void Engine<V>::check_metrics(int cur, Metrics& mc, Metrics& ma) {
-    dev::KktIter<V> it;
     if constexpr (F32) {
-        it.xref[0] = it.xref[1] = xref_.p;
-        it.yref[0] = it.yref[1] = yref_.p;
+        // both FP64 iterates (reference + FP32 correction) formed once, so
+        // the KKT SpMVs gather one double per nonzero instead of a double
+        // and a float (same values)
+        dev::combine(xref_.p, xcur(cur), kx_[0].p, n_, s_);
+        dev::combine(yref_.p, ycur(cur), ky_[0].p, m_, s_);
+        dev::combine(xref_.p, xavg_.p, kx_[1].p, n_, s_);
+        dev::combine(yref_.p, yavg_.p, ky_[1].p, m_, s_);
+        dev::KktIter<double> it;
+        for (int q = 0; q < 2; ++q) {
+            it.xd[q] = kx_[q].p;
+            it.yd[q] = ky_[q].p;
+        }
+        dev::launch_kkt(kd_, it, met_.p, s_);
+    } else {
+        dev::KktIter<V> it;
+        it.xd[0] = xcur(cur);
+        it.yd[0] = ycur(cur);
+        it.xd[1] = xavg_.p;
+        it.yd[1] = yavg_.p;
+        dev::launch_kkt(kd_, it, met_.p, s_);
     }
-    it.xd[0] = xcur(cur);
-    it.yd[0] = ycur(cur);
-    it.xd[1] = xavg_.p;
-    it.yd[1] = yavg_.p;
-    dev::launch_kkt(kd_, it, met_.p, s_);
     Metrics h[2];
     HIPCK(hipMemcpyAsync(h, met_.p, sizeof(h), hipMemcpyDeviceToHost, s_));
     sync();
