@@ -204,7 +204,6 @@ class Engine final : public EngineBase {
     DBuf<double> kx_[2], ky_[2];  // FP64 iterates of a check (FP32 engine)
     DBuf<double> xb_, yb_, pesx_, pesy_, tmpm_, tmpn_;
     DBuf<double> partT_, partA_, partC_, partR_, partR2_, rowres_, red_, scal_;
-    DBuf<unsigned int> tickets_;
     DBuf<LsState> st_;
     DBuf<Metrics> met_;
     // column-sliced copies of A (gathers x) and A' (gathers y), when used
@@ -262,7 +261,7 @@ void Engine<V>::full(double* dst, const double* ref, const V* d, long k) {
 
 template <typename V>
 double Engine<V>::dsum(const double* a, const double* b, long n) {
-    dev::sumsq_diff(a, b, n, red_.p, tickets_.p + 2, scal_.p, s_);
+    dev::sumsq_diff(a, b, n, red_.p, scal_.p, s_);
     double h = 0.0;
     HIPCK(hipMemcpyAsync(&h, scal_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
     sync();
@@ -334,14 +333,12 @@ void Engine<V>::setup() {
     tmpn_.alloc(n_);
     red_.alloc(1024);
     scal_.alloc(4);
-    tickets_.alloc(8);
-    tickets_.zero(s_);
 
     lam2_ = power_iteration();
     eta0_ = 1.2 / std::sqrt(lam2_);
     {  // pdhg.cu:647-663
         const double cn = dsum(sc_.p, nullptr, n_);
-        dev::sumsq_q(srmin_.p, srmax_.p, m_, red_.p, tickets_.p + 2, scal_.p, s_);
+        dev::sumsq_q(srmin_.p, srmax_.p, m_, red_.p, scal_.p, s_);
         double qn = 0.0;
         HIPCK(hipMemcpyAsync(&qn, scal_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
         sync();
@@ -350,7 +347,7 @@ void Engine<V>::setup() {
             omega0_ = std::clamp(std::sqrt(cn / qn), 0.3, 3.0);
     }
     double cmax = 0.0;
-    dev::absmax(c_.p, n_, red_.p, tickets_.p + 2, scal_.p, s_);
+    dev::absmax(c_.p, n_, red_.p, scal_.p, s_);
     HIPCK(hipMemcpyAsync(&cmax, scal_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
     sync();
 
@@ -382,7 +379,7 @@ void Engine<V>::setup() {
     ib_.xbar = xbar_.p; ib_.ata = ata_.p; ib_.xavg = xavg_.p; ib_.yavg = yavg_.p;
     ib_.axcur = axcur_.p; ib_.adx = adx_.p;
     ib_.partT = partT_.p; ib_.partA = partA_.p;
-    ib_.st = st_.p; ib_.ticket = tickets_.p + 3;
+    ib_.st = st_.p;
     if constexpr (F32) {
         vax_.alloc(nnz);
         vacx_.alloc(nnz);
@@ -416,7 +413,7 @@ void Engine<V>::setup() {
     kd_.dr = dr_.p; kd_.dc = dc_.p;
     kd_.cmax = cmax;
     kd_.partC = partC_.p; kd_.partR = partR_.p; kd_.partR2 = partR2_.p;
-    kd_.rowres = rowres_.p; kd_.tickets = tickets_.p;
+    kd_.rowres = rowres_.p;
 
     const int G = cfg_.graph_slots;
     if (G > 0) {  // G attempt slots, replayed until the state freezes

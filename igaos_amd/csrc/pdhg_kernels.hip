@@ -817,14 +817,14 @@ __global__ __launch_bounds__(FIN) void k_fin(const double* part, int cnt,
     if (threadIdx.x == 0) *out = t;
 }
 
-void sumsq_diff(const double* a, const double* b, long n, double* part,
-                unsigned int*, double* out, hipStream_t s) {
+void sumsq_diff(const double* a, const double* b, long n, double* part, double* out,
+                hipStream_t s) {
     const int g = reduce_blocks(n);
     k_sumsq_diff<double><<<g, BS, 0, s>>>(a, b, n, part);
     k_fin<<<1, FIN, 0, s>>>(part, g, 0, out);
 }
-void sumsq_diff(const float* a, const float* b, long n, double* part,
-                unsigned int*, double* out, hipStream_t s) {
+void sumsq_diff(const float* a, const float* b, long n, double* part, double* out,
+                hipStream_t s) {
     const int g = reduce_blocks(n);
     k_sumsq_diff<float><<<g, BS, 0, s>>>(a, b, n, part);
     k_fin<<<1, FIN, 0, s>>>(part, g, 0, out);
@@ -846,7 +846,7 @@ __global__ __launch_bounds__(BS) void k_sumsq_q(const double* srmin,
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 void sumsq_q(const double* srmin, const double* srmax, long m, double* part,
-             unsigned int*, double* out, hipStream_t s) {
+             double* out, hipStream_t s) {
     const int g = reduce_blocks(m);
     k_sumsq_q<<<g, BS, 0, s>>>(srmin, srmax, m, part);
     k_fin<<<1, FIN, 0, s>>>(part, g, 0, out);
@@ -862,7 +862,7 @@ __global__ __launch_bounds__(BS) void k_absmax(const double* v, long n,
     s = block_max(s, red);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-void absmax(const double* v, long n, double* part, unsigned int*,
+void absmax(const double* v, long n, double* part,
             double* out, hipStream_t s) {
     const int g = reduce_blocks(n);
     k_absmax<<<g, BS, 0, s>>>(v, n, part);

@@ -84,7 +84,6 @@ struct IterBufs {
     // the fused row-block kernels run instead
     Sliced<V> sT, sA;
     pdhg::LsState* st = nullptr;
-    unsigned int* ticket = nullptr;
 };
 
 // One line-search attempt: fused A'y + primal step, fused A xbar + dual
@@ -115,7 +114,6 @@ struct KktData {
     double cmax = 0.0;
     double *partC = nullptr, *partR = nullptr, *partR2 = nullptr;
     double* rowres = nullptr;        // 2 x 4 row-pass results
-    unsigned int* tickets = nullptr; // 2 counters
 };
 template <typename V>
 struct KktIter {
@@ -144,19 +142,19 @@ void scale_data(int m, int n, const int* ap, const int* ai, const double* ax,
                 double* scl, double* scu, double* srmin, double* srmax,
                 hipStream_t s);
 
-// out[0] = sum (a - b)^2 (b may be null), fixed order; `part` holds at
-// least reduce_blocks(n) doubles and `ticket` one zeroed counter.
+// out[0] = sum (a - b)^2 (b may be null), fixed order (per-workgroup
+// partials, then one workgroup); `part` holds at least reduce_blocks(n)
+// doubles.
 int reduce_blocks(long n);
-void sumsq_diff(const double* a, const double* b, long n, double* part,
-                unsigned int* ticket, double* out, hipStream_t s);
-void sumsq_diff(const float* a, const float* b, long n, double* part,
-                unsigned int* ticket, double* out, hipStream_t s);
+void sumsq_diff(const double* a, const double* b, long n, double* part, double* out,
+                hipStream_t s);
+void sumsq_diff(const float* a, const float* b, long n, double* part, double* out,
+                hipStream_t s);
 // out[0] = sum_j q_j^2 with q = srmax if finite, else srmin if finite, else 0
 void sumsq_q(const double* srmin, const double* srmax, long m, double* part,
-             unsigned int* ticket, double* out, hipStream_t s);
+             double* out, hipStream_t s);
 // out[0] = max |v|
-void absmax(const double* v, long n, double* part, unsigned int* ticket,
-            double* out, hipStream_t s);
+void absmax(const double* v, long n, double* part, double* out, hipStream_t s);
 
 // Elementwise helpers.
 void fill(double* v, long n, double val, hipStream_t s);
