@@ -29,9 +29,11 @@ FP32 solve (the LP needs well over 10^4 iterations to reach 1e-6).
 
 ``cpu_baseline`` is the oracle (a single-threaded C host replay of the
 reference engine) on a bounded sample of the same workload, named in
-``sample``.  Beside it, ``cpu_baseline_reference_simplex`` times the
-reference's own CPU path for these LPs, its revised simplex built from its
-sources (``oracle/_ref``, present when it was built), on the same sample.
+``sample``.  Beside it, ``cpu_baseline_all_cores`` runs the same replay on
+up to 16 host threads (one instance per thread) over a larger sample, and
+``cpu_baseline_reference_simplex`` times the reference's own CPU path for
+these LPs, its revised simplex built from its sources (``oracle/_ref``,
+present when it was built), on the single-thread sample.
 """
 from __future__ import annotations
 
@@ -100,7 +102,9 @@ def bench_batch(args):
         mode = sched.mode if args.mode == "auto" else "persistent"
         log(f"[bench] autotuned in {tune_s:.1f} s: mode {mode}, tpr by class "
             f"{sched.tpr} (nnz edges {sched.edges}), waves/SIMD {sched.waves}")
+    t_create = time.perf_counter()
     b = BatchSolver(lps, precision=32, mode=mode)
+    t_create = time.perf_counter() - t_create
     if sched is not None:
         autotune.apply(b, sched)
     tpr, grid, waves = b.schedule()
@@ -181,6 +185,26 @@ def bench_batch(args):
                           f"{cs} of the batch, {sc}/{len(cs)} certified in "
                           f"{tc:.1f} s of solve time ({wall:.1f} s wall incl. "
                           f"setup), budget {args.cpu_time_limit} s each")}
+    cpu_par = None
+    if not args.no_cpu_baseline:
+        # the same host replay on all the host cores this job may use, one
+        # instance per thread (ctypes releases the GIL during each solve)
+        import oracle
+        from concurrent.futures import ThreadPoolExecutor
+        cores = max(1, min(16, os.cpu_count() or 1))
+        step = max(1, count // (4 * cores))
+        cs = list(range(0, count, step))[:4 * cores]
+        wall = time.perf_counter()
+        with ThreadPoolExecutor(cores) as ex:
+            res = list(ex.map(lambda i: oracle.solve(lps[i], tolerance=args.tolerance,
+                                                     time_limit=args.cpu_time_limit), cs))
+        wall = time.perf_counter() - wall
+        sc = sum(r.status == "near-optimal" for r in res)
+        cpu_par = {"value": sc / wall, "unit": "LPs/s", "cores": cores, "kind": "port",
+                   "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+                   "sample": (f"oracle host replay, {cores} threads, one instance each: "
+                              f"{len(cs)} instances (every {step}th), {sc} certified in "
+                              f"{wall:.1f} s wall, budget {args.cpu_time_limit} s each")}
     cpu_ref = None
     if not args.no_cpu_baseline:
         import oracle
@@ -223,6 +247,12 @@ def bench_batch(args):
                        "tuning_s": round(tune_s, 1)}},
         "solved": solved, "instances": count * args.steps,
         "iterations_per_s": float(its.sum()) / (elapsed / args.steps),
+        "end_to_end": {
+            "value": solved / (t_create + elapsed), "unit": "LPs/s",
+            "setup_s": round(t_create, 2),
+            "note": ("from host arrays to certified results: upload, row blocks "
+                     "and gather-ordered copies, then the step(s); the autotuner's "
+                     "probe runs are not included")},
         "iterations_median": float(np.median(its)),
         "iterations_p99": float(np.percentile(its, 99)),
         "launch_ms": launch_ms,
@@ -252,6 +282,7 @@ def bench_batch(args):
                      "lockstep_samples": None if lk is None else lk,
                      "whole_run_gbs": round(achieved_run, 1)},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_par,
         "cpu_baseline_reference_simplex": cpu_ref,
     }
 
@@ -409,6 +440,17 @@ def bench_zipf(args):
                      "ms_per_launch": round(ms_dom, 4)},
         "cpu_baseline": cpu,
     }
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def step_kernels(step, slices, prec):
