@@ -217,7 +217,7 @@ class Engine final : public EngineBase {
     double lam2_ = 1.0, eta0_ = 1.0, omega0_ = 1.0;
 
     void setup();
-    int slices_for(int cols, long nnz) const;
+    int slices_for(int cols, int rows, long nnz) const;
     void make_slices(dev::Sliced<V>& sl, SliceBufs& B, int rows, int cols,
                      const std::vector<int>& ptr, const std::vector<int>& idx,
                      const double* sval, int S);
@@ -356,7 +356,7 @@ void Engine<V>::setup() {
     y0_.alloc(m_); y1_.alloc(m_); yavg_.alloc(m_); axcur_.alloc(m_); adx_.alloc(m_);
     xb_.alloc(n_); yb_.alloc(m_); pesx_.alloc(n_); pesy_.alloc(m_);
     {
-        const int sT = slices_for(m_, nnz), sA = slices_for(n_, nnz);
+        const int sT = slices_for(m_, n_, nnz), sA = slices_for(n_, m_, nnz);
         if (sT) make_slices(ib_.sT, slT_, n_, m_, lp_.cp, lp_.ci, sacx_.p, sT);
         if (sA) make_slices(ib_.sA, slA_, m_, n_, lp_.ap, lp_.ai, sax_.p, sA);
         ib_.nT_part = sT ? (n_ + dev::BS - 1) / dev::BS : rT_.nblk;
@@ -425,25 +425,26 @@ void Engine<V>::setup() {
     sync();
 }
 
-// Column slicing of the product that gathers a vector of `cols` entries:
-// used when that vector (cols * sizeof(V) bytes) outgrows what an XCD's
-// 4 MB L2 keeps under the gathers (3 MB and up) and the matrix has enough
-// nonzeros (8M) that every slice still fills the GPU.  Slices hold at least
-// 0.5 MB of the vector each, at most 8 of them: on the 1M-column Zipf LP
-// (m = 400k, 68.5M nonzeros) 8 slices of x took the A xbar step from 450
-// to 403 us (FP64) and 382 to 284 us (FP32), and 4 slices of the FP64 y
-// the A'y step from 467 to 434 us, while 8 slices of y (0.4 MB each) or 16
-// of either were slower than 4 / 8: each slice adds a pass over the
-// partial row sums.
+// Column slicing of the product that gathers a vector of `cols` entries
+// over `rows` rows: used when that vector (cols * sizeof(V) bytes)
+// outgrows what an XCD's 4 MB L2 keeps under the gathers (3 MB and up) and
+// the matrix has enough nonzeros (8M) that every slice still fills the GPU.
+// Every slice adds a pass over the rows (pointers and partial sums), so the
+// slice size follows the row length: 0.5 MB of the vector per slice when
+// rows average 128 nonzeros or more, 1.6 MB otherwise, 2 to 8 slices.  On
+// the 1M-column Zipf LP (m = 400k, 68.5M nonzeros; A rows average 171,
+// A' rows 68.5) this gives 8 slices of x, which took the A xbar step from
+// 450 to 401 us (FP64) and 382 to 285 us (FP32; 4 slices: 326 us), and 2
+// slices of the FP64 y, which took the A'y step from 467 to 416 us (4
+// slices: 435 us, 6: 463 us).
 template <typename V>
-int Engine<V>::slices_for(int cols, long nnz) const {
+int Engine<V>::slices_for(int cols, int rows, long nnz) const {
     const int k = cfg_.column_slices;
     if (k >= 0) return std::min(k, dev::MAX_SLICES);
     const double bytes = static_cast<double>(cols) * sizeof(V);
-    if (bytes < 3.0e6 || nnz < 8000000) return 0;
-    int S = 8;
-    while (S > 2 && bytes / S < 0.5e6) S /= 2;
-    return S;
+    if (bytes < 3.0e6 || nnz < 8000000 || rows <= 0) return 0;
+    const double per = static_cast<double>(nnz) / rows >= 128.0 ? 0.5e6 : 1.6e6;
+    return static_cast<int>(std::clamp(std::lround(bytes / per), 2L, 8L));
 }
 
 template <typename V>
