@@ -78,6 +78,8 @@ SIGNATURES = {
     "igaos_pdhg_profile_attempts": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip,
                                               _ip]),
     "igaos_rowblocks_build": (C.c_int, [C.c_int, _ip, _ip, C.c_int]),
+    "igaos_slices_build": (C.c_int, [C.c_int, C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip,
+                                     _ip, _ip, C.c_int]),
     "igaos_spmv_csr_f64": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),
     "igaos_spmv_csr_f32": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -367,6 +367,27 @@ int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap) {
     return k;
 }
 
+int igaos_slices_build(int nrows, int ncols, const int* ptr, const int* idx, int slices,
+                       int* sptr, int* sidx, int* src, int* blocks, int* bslice, int cap) {
+    if (nrows < 0 || ncols < 0 || !ptr || (ptr[nrows] > 0 && !idx))
+        return fail("invalid CSR arrays");
+    if (slices < 1 || slices > igaos::dev::MAX_SLICES) return fail("slices must be 1..16");
+    try {
+        const igaos::dev::HostSlices h =
+            igaos::dev::build_slices(nrows, ncols, ptr, idx, slices);
+        const int nb = static_cast<int>(h.bslice.size());
+        if (cap < nb) return nb;
+        if (sptr) std::memcpy(sptr, h.ptr.data(), sizeof(int) * h.ptr.size());
+        if (sidx) std::memcpy(sidx, h.idx.data(), sizeof(int) * h.idx.size());
+        if (src) std::memcpy(src, h.src.data(), sizeof(int) * h.src.size());
+        if (blocks) std::memcpy(blocks, h.quad.data(), sizeof(int) * h.quad.size());
+        if (bslice) std::memcpy(bslice, h.bslice.data(), sizeof(int) * nb);
+        return nb;
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+}
+
 int igaos_spmv_csr_f64(int nblk, const int* rowblocks, const int* ptr,
                        const int* idx, const double* val, const double* x,
                        double* y, void* stream) {

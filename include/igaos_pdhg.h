@@ -305,6 +305,17 @@ void igaos_batch_destroy(igaos_batch* batch);
  * ints; returns the number of ints, or the required count if cap is too
  * small.  The device copy must be 8-byte aligned. */
 int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap);
+/* Host: the column-sliced copy of a CSR matrix that the engine's large-LP
+ * products use (no reference counterpart): the columns cut into `slices`
+ * (1..16) ranges of equal nonzero count, each row's nonzeros split by
+ * range in their order, slices stored one after the other.  Writes sptr
+ * (slices * (nrows + 1) offsets), sidx and src (nnz each: column index,
+ * position in the input), and per row block (slice-major) its (first row,
+ * end row, first nonzero, end nonzero) into blocks and its slice into
+ * bslice.  Returns the number of blocks; when cap (blocks' capacity in
+ * blocks) is smaller, writes nothing and returns the count needed. */
+int igaos_slices_build(int nrows, int ncols, const int* ptr, const int* idx, int slices,
+                       int* sptr, int* sidx, int* src, int* blocks, int* bslice, int cap);
 /* y = A x with A in CSR (device arrays); replaces the cusparseSpMV calls
  * of src/pdhg/pdhg.cu:150-156 for A and A' (CSR of A' = CSC of A). */
 int igaos_spmv_csr_f64(int nblk, const int* rowblocks, const int* ptr,

@@ -198,3 +198,62 @@ def test_rowblock_invariants(gen):
             nz = ptr[b] - ptr[a]
             assert b - a <= 256
             assert nz <= 2048 or b - a == 1
+
+
+def _slices(ptr, idx, ncols, S):
+    import ctypes as C
+    L = _core.lib()
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    nrows, nnz = len(ptr) - 1, int(ptr[-1])
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa
+    nb = L.igaos_slices_build(nrows, ncols, ip(ptr), ip(idx), S, None, None, None, None,
+                              None, 0)
+    assert nb >= 0
+    sptr = np.zeros(S * (nrows + 1), np.int32)
+    sidx, src = np.zeros(max(nnz, 1), np.int32), np.zeros(max(nnz, 1), np.int32)
+    blocks, bslice = np.zeros(4 * max(nb, 1), np.int32), np.zeros(max(nb, 1), np.int32)
+    assert L.igaos_slices_build(nrows, ncols, ip(ptr), ip(idx), S, ip(sptr), ip(sidx),
+                                ip(src), ip(blocks), ip(bslice), nb) == nb
+    return (sptr.reshape(S, nrows + 1), sidx[:nnz], src[:nnz], blocks[:4 * nb].reshape(-1, 4),
+            bslice[:nb])
+
+
+@pytest.mark.parametrize("S", [1, 3, 8])
+def test_column_slice_invariants(S):
+    # the sliced copy of the large-LP products (host builder): a
+    # permutation of the nonzeros, each slice a contiguous column range with
+    # ranges in order, each row's nonzeros in their own order, and row
+    # blocks per slice that cover every row once within the block limits
+    lp = lpgen.gen_zipf_badly_scaled(seed=2, m=2000, n=9000, kmax=600)
+    for ptr, idx, ncols in ((lp.ap, lp.ai, lp.n), (lp.cp, lp.ci, lp.m)):
+        ptr, idx = np.asarray(ptr), np.asarray(idx)
+        nrows = len(ptr) - 1
+        sptr, sidx, src, blocks, bslice = _slices(ptr, idx, ncols, S)
+        assert np.array_equal(np.sort(src), np.arange(ptr[-1]))
+        assert np.array_equal(sidx, idx[src])
+        rows_of = np.repeat(np.arange(nrows), np.diff(ptr))
+        last_max = -1
+        for sc in range(S):
+            p = sptr[sc]
+            assert np.all(np.diff(p) >= 0)
+            seg = slice(p[0], p[-1])
+            if p[-1] > p[0]:
+                assert sidx[seg].min() > last_max
+                last_max = sidx[seg].max()
+            # each row's entries in this slice: the row's own, in order
+            r = np.repeat(np.arange(nrows), np.diff(p))
+            assert np.array_equal(rows_of[src[seg]], r)
+            for i in range(0, nrows, max(1, nrows // 50)):
+                assert np.all(np.diff(src[p[i]:p[i + 1]]) > 0)
+            b = blocks[bslice == sc]
+            assert np.array_equal(b[:, 0], np.concatenate([[0], b[:-1, 1]]))
+            assert b[-1, 1] == nrows and np.all(b[:, 1] > b[:, 0])
+            assert np.array_equal(b[:, 2], p[b[:, 0]]) and np.array_equal(b[:, 3], p[b[:, 1]])
+            assert np.all((b[:, 1] - b[:, 0] <= 256))
+            assert np.all((b[:, 3] - b[:, 2] <= 2048) | (b[:, 1] - b[:, 0] == 1))
+        assert np.all(np.diff(bslice) >= 0)
+        # equal nonzero shares: no slice holds more than its share plus one column
+        counts = np.array([sptr[sc][-1] - sptr[sc][0] for sc in range(S)])
+        colmax = np.bincount(idx, minlength=ncols).max()
+        assert counts.max() <= ptr[-1] / S + colmax
