@@ -97,6 +97,11 @@ struct igaos_batch {
     double* d_partA = nullptr;
     int* d_done = nullptr;
     int* d_ord = nullptr;  // check launch order of the active instances
+    // pinned host staging of the per-period tables (asynchronous uploads)
+    igaos::batch::LkEnt* h_tblT = nullptr;
+    igaos::batch::LkEnt* h_tblA = nullptr;
+    igaos::batch::LsRange* h_rng = nullptr;
+    int* h_ord = nullptr;
     int* d_nfrozen = nullptr;
     // lockstep timing samples: rounds 0, 16, 32, 48 of each check period
     // (every active instance works in those), HIP events around the two
@@ -123,6 +128,9 @@ struct igaos_batch {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (h_prog) (void)hipHostFree(h_prog);
+        for (void* p : {static_cast<void*>(h_tblT), static_cast<void*>(h_tblA),
+                        static_cast<void*>(h_rng), static_cast<void*>(h_ord)})
+            if (p) (void)hipHostFree(p);
         if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -284,7 +292,6 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
     long periods_done = 0;
     B->lk_done = count - static_cast<int>(active.size());
     using igaos::batch::LkEnt;
-    std::vector<LkEnt> tT, tA;
     // one entry per row block b of an instance's table rb (pairs of first
     // row, first nonzero)
     auto entry = [](int lp, const int* rb, int b) {
@@ -296,23 +303,31 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
         e.p1 = rb[2 * b + 3];
         return e;
     };
-    std::vector<LsRange> rng;
-    std::vector<int> ord;
+    // the checks run the largest instances first (their workgroups take
+    // longest): one order of all instances, filtered each period
+    std::vector<int> lpt(count), pos(count, -1);
+    std::iota(lpt.begin(), lpt.end(), 0);
+    std::stable_sort(lpt.begin(), lpt.end(), [&](int u, int v) {
+        const LPDesc& du = B->desc[u];
+        const LPDesc& dv = B->desc[v];
+        return du.nnz + du.m + du.n > dv.nnz + dv.m + dv.n;
+    });
     while (!active.empty()) {
-        // tables of the active instances' row blocks
-        tT.clear();
-        tA.clear();
-        rng.clear();
-        for (int lp : active) {
-            const LPDesc& d = B->desc[lp];
-            LsRange r{lp, static_cast<int>(tT.size()), d.nblkT,
-                      static_cast<int>(tA.size()), d.nblkA};
-            for (int b = 0; b < d.nblkT; ++b) tT.push_back(entry(lp, B->rbT.data() + d.rbto, b));
-            for (int b = 0; b < d.nblkA; ++b) tA.push_back(entry(lp, B->rbA.data() + d.rbao, b));
-            rng.push_back(r);
-        }
+        // tables of the active instances' row blocks, built in pinned host
+        // memory (the previous period's uploads have completed: the host
+        // synchronised on the check)
         const int nact = static_cast<int>(active.size());
-        const int nbT = static_cast<int>(tT.size()), nbA = static_cast<int>(tA.size());
+        int nbT = 0, nbA = 0;
+        for (int a = 0; a < nact; ++a) {
+            const int lp = active[a];
+            const LPDesc& d = B->desc[lp];
+            B->h_rng[a] = LsRange{lp, nbT, d.nblkT, nbA, d.nblkA};
+            const int* rbT = B->rbT.data() + d.rbto;
+            const int* rbA = B->rbA.data() + d.rbao;
+            for (int b = 0; b < d.nblkT; ++b) B->h_tblT[nbT++] = entry(lp, rbT, b);
+            for (int b = 0; b < d.nblkA; ++b) B->h_tblA[nbA++] = entry(lp, rbA, b);
+            pos[lp] = a;
+        }
         // algorithmic bytes of one round's attempt kernels over the active set
         const double wa = sizeof(V), wv = sizeof(V);
         double bp = 0.0, bd = 0.0;
@@ -321,19 +336,16 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
             bp += d.nnz * (wa + 4) + 4.0 * (d.n + 1) + 7.0 * d.n * wv;
             bd += d.nnz * (wa + 4) + 4.0 * (d.m + 1) + 8.0 * d.m * wv;
         }
-        BCK(hipMemcpyAsync(B->d_tblT, tT.data(), sizeof(LkEnt) * nbT, hipMemcpyHostToDevice, s));
-        BCK(hipMemcpyAsync(B->d_tblA, tA.data(), sizeof(LkEnt) * nbA, hipMemcpyHostToDevice, s));
-        BCK(hipMemcpyAsync(B->d_rng, rng.data(), sizeof(LsRange) * nact,
-                           hipMemcpyHostToDevice, s));
-        // checks of the largest instances first (longest workgroups start early)
-        ord.resize(nact);
-        std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
-            const LPDesc& du = B->desc[active[u]];
-            const LPDesc& dv = B->desc[active[v]];
-            return du.nnz + du.m + du.n > dv.nnz + dv.m + dv.n;
-        });
-        BCK(hipMemcpyAsync(B->d_ord, ord.data(), sizeof(int) * nact, hipMemcpyHostToDevice, s));
+        int no = 0;
+        for (int lp : lpt)
+            if (pos[lp] >= 0) B->h_ord[no++] = pos[lp];
+        for (int lp : active) pos[lp] = -1;
+        if (nbT) BCK(hipMemcpyAsync(B->d_tblT, B->h_tblT, sizeof(LkEnt) * nbT,
+                                    hipMemcpyHostToDevice, s));
+        if (nbA) BCK(hipMemcpyAsync(B->d_tblA, B->h_tblA, sizeof(LkEnt) * nbA,
+                                    hipMemcpyHostToDevice, s));
+        BCK(hipMemcpyAsync(B->d_rng, B->h_rng, sizeof(LsRange) * nact, hipMemcpyHostToDevice, s));
+        BCK(hipMemcpyAsync(B->d_ord, B->h_ord, sizeof(int) * nact, hipMemcpyHostToDevice, s));
         BCK(hipMemsetAsync(B->d_nfrozen, 0, sizeof(int), s));
         if (B->idx16 && periods_done == 0)  // values in gather order, once per run
             igaos::batch::lockstep_sort_values(A, B->d_desc, B->d_tblT, nbT, B->d_tblA,
@@ -402,6 +414,13 @@ void lockstep_alloc(igaos_batch* B) {
     B->d_partA = M.alloc<double>(2 * ba);
     B->d_done = M.alloc<int>(B->count);
     B->d_ord = M.alloc<int>(B->count);
+    BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_tblT),
+                      sizeof(igaos::batch::LkEnt) * std::max(bt, 1L)));
+    BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_tblA),
+                      sizeof(igaos::batch::LkEnt) * std::max(ba, 1L)));
+    BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_rng),
+                      sizeof(igaos::batch::LsRange) * std::max(B->count, 1)));
+    BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_ord), sizeof(int) * std::max(B->count, 1)));
     B->d_nfrozen = M.alloc<int>(1);
 }
 
