@@ -492,8 +492,9 @@ def main():
                          "kernel for scaling rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
-                                   "sharding is described in DESIGN.md only",
+        print(json.dumps({"error": "the engine runs on one GPU; sharding a batch "
+                                   "by instance or one LP by rows across GPUs is "
+                                   "not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
