@@ -257,3 +257,14 @@ def test_column_slice_invariants(S):
         counts = np.array([sptr[sc][-1] - sptr[sc][0] for sc in range(S)])
         colmax = np.bincount(idx, minlength=ncols).max()
         assert counts.max() <= ptr[-1] / S + colmax
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import json
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert json.loads(r.stdout.strip().splitlines()[-1])["n_gpus"] == 2
