@@ -61,6 +61,23 @@ def test_batch_fp64_trajectory_matches_oracle(mode):
         assert err <= 1e-9, (i, err)
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_wide_instance_takes_the_32_bit_index_path(mode):
+    # an instance with n > 65,536 keeps the whole batch on 32-bit column
+    # indices in row order (no gather-ordered 16-bit copies): same
+    # one-period trajectory as the oracle
+    wide = lpgen.gen_random_eq(seed=3, m=20_000, n=70_000, per_col=3)
+    lps = [lpgen.gen_batch_instance(0, 1, m_hi=600), wide]
+    sol = solve_batch(lps, precision=64, mode=mode, tolerance=0.0, time_limit=1e7,
+                      max_iterations=64)
+    for i, lp in enumerate(lps):
+        r = oracle.solve(lp, tolerance=0.0, time_limit=1e7, max_iterations=64)
+        assert sol.iterations[i] == 64 and sol.attempts[i] == r.attempts
+        err = max(np.abs(sol.x[i] - r.x).max() / max(1, np.abs(r.x).max()),
+                  np.abs(sol.y[i] - r.y).max() / max(1, np.abs(r.y).max()))
+        assert err <= 1e-9, (i, err)
+
+
 def test_batch_fp64_matches_single_engine():
     lps = [lpgen.gen_batch_instance(3, i, m_hi=800) for i in range(4)]
     sol = solve_batch(lps, precision=64, tolerance=1e-6, time_limit=60)
