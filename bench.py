@@ -22,6 +22,8 @@ mode: summed over every instance's attempts and divided by the launch's
 duration.  ``traffic`` comes from the committed rocprofv3 --pmc summary
 when present.
 
+``--workload c1`` solves the reference's CPU-runnable LP (m = 1,000,
+n = 3,000) with the single-LP engine and reports iterations/s.
 ``--workload zipf`` instead measures the single-LP attempt kernels on the
 badly scaled LP (n = 1,000,000, m = 400,000, clip(Zipf(1.5), 2, 2000)
 nonzeros per column, 68.5M nonzeros) in FP64 and FP32, plus a bounded
@@ -461,6 +463,61 @@ def step_kernels(step, slices, prec):
     return f"k_{step}<{t}>"
 
 
+# ------------------------------------------------------------------- c1 --
+def bench_c1(args):
+    """The reference's CPU-runnable case: one random equality LP (m = 1,000,
+    n = 3,000, 5 nonzeros per column, seed 0) to 1e-6, plain FP64 and FP32 +
+    refinement, each solved `steps` times from a prepared engine.  At 0.6 MB
+    per iteration the LP lives in cache: the figure is iterations/s (launch
+    latency), not an HBM fraction."""
+    import numpy as np
+    import igaos_amd
+    from igaos_amd import lpgen
+
+    lp = lpgen.gen_random_eq(seed=0)
+    out = {}
+    for prec in (64, 32):
+        e = igaos_amd.Engine(lp, precision=prec)
+        for _ in range(args.warmup):
+            e.run(time_limit=args.time_limit, tolerance=args.tolerance)
+        its, ms, st, ob = 0, 0.0, None, None
+        for _ in range(args.steps):
+            s = e.run(time_limit=args.time_limit, tolerance=args.tolerance)
+            its += s.iterations
+            ms += s.solve_time_ms
+            st, ob = s.status, s.objective
+        e.close()
+        out[prec] = {"status": st, "objective": ob, "iterations": its // args.steps,
+                     "solve_ms": ms / args.steps, "iterations_per_s": its / (ms * 1e-3),
+                     "solves_per_s": args.steps / (ms * 1e-3)}
+        log(f"[bench] C1 FP{prec}: {st} obj {ob:.10g}, {its // args.steps} it, "
+            f"{ms / args.steps:.1f} ms per solve")
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        t0 = time.perf_counter()
+        r = oracle.solve(lp, tolerance=args.tolerance, time_limit=args.cpu_time_limit)
+        wall = time.perf_counter() - t0
+        cpu = {"value": r.iterations / (r.solve_time_ms * 1e-3), "unit": "iterations/s",
+               "cores": 1, "kind": "port",
+               "sample": (f"oracle host replay (1 thread) of the same LP: {r.status} after "
+                          f"{r.iterations} iterations, {wall:.1f} s wall, budget "
+                          f"{args.cpu_time_limit} s")}
+    k64 = out[64]
+    return {
+        "metric": "C1 single-LP solve: iterations/s (plain FP64 engine)",
+        "value": k64["iterations_per_s"], "unit": "iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": k64["solve_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "random equality LP m=1000 n=3000, 5 nnz/col, seed 0",
+                   "tolerance": args.tolerance},
+        "engines": out, "roofline": None,
+        "note": "0.6 MB per iteration stays in cache: bound by launch latency",
+        "cpu_baseline": cpu,
+    }
+
+
 def torch_sync():
     import torch
     torch.cuda.synchronize()
@@ -471,7 +528,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--workload", default="batch", choices=["batch", "zipf", "diff"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "zipf", "diff", "c1"])
     ap.add_argument("--count", type=int, default=8192)
     ap.add_argument("--mode", default="auto", choices=["auto", "lockstep", "persistent"],
                     help="batched execution mode (igaos_batch_set_mode); auto: "
@@ -507,7 +564,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    line = {"batch": bench_batch, "zipf": bench_zipf, "diff": bench_diff}[args.workload](args)
+    line = {"batch": bench_batch, "zipf": bench_zipf, "diff": bench_diff,
+            "c1": bench_c1}[args.workload](args)
     print(json.dumps(line), flush=True)
 
 
