@@ -51,6 +51,9 @@ SIGNATURES = {
                                       _dp, _dp, _dp, _dp, _dp, _dp,
                                       C.POINTER(AdjointInfo)]),
     "igaos_batch_unroll": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "igaos_gather_order_build": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int), C.POINTER(C.c_ushort),
+                                           C.POINTER(C.c_ushort)]),
     "igaos_batch_unroll_backward": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp,
                                               _dp, _dp, _dp]),
 }

@@ -148,6 +148,26 @@ int bfail(const std::string& w) {
 // or A' alike): inside every row block of more than one row the nonzeros
 // sorted by index (stable), as 16-bit indices plus each one's position in
 // the block's row order.  Instances are spread over host threads.
+// One matrix's row blocks rb (nb blocks, pairs of first row, first
+// nonzero): within every block of more than one row the nonzeros' order
+// sorted by index (stable), as 16-bit indices and slots.
+void gather_order_blocks(const int* rb, int nb, const int* ix, unsigned short* si,
+                         unsigned short* sl, std::vector<int>& ord) {
+    for (int k = 0; k < nb; ++k) {
+        const int r0 = rb[2 * k], p0 = rb[2 * k + 1];
+        const int r1 = rb[2 * k + 2], p1 = rb[2 * k + 3];
+        ord.resize(p1 - p0);
+        std::iota(ord.begin(), ord.end(), 0);
+        if (r1 - r0 > 1)
+            std::stable_sort(ord.begin(), ord.end(),
+                             [&](int u, int v) { return ix[p0 + u] < ix[p0 + v]; });
+        for (int q = 0; q < p1 - p0; ++q) {
+            si[p0 + q] = static_cast<unsigned short>(ix[p0 + ord[q]]);
+            sl[p0 + q] = static_cast<unsigned short>(ord[q]);
+        }
+    }
+}
+
 void gather_order(const std::vector<LPDesc>& desc, const std::vector<int>& rb, bool trans,
                   const int* idx, long z, std::vector<unsigned short>& sidx,
                   std::vector<unsigned short>& slot) {
@@ -160,25 +180,9 @@ void gather_order(const std::vector<LPDesc>& desc, const std::vector<int>& rb, b
         std::vector<int> ord;
         for (int lp = next++; lp < static_cast<int>(desc.size()); lp = next++) {
             const LPDesc& d = desc[lp];
-            const int* b = rb.data() + (trans ? d.rbto : d.rbao);
-            const int nb = trans ? d.nblkT : d.nblkA;
-            const int* ix = idx + d.zoff;
-            unsigned short* si = sidx.data() + d.zoff;
-            unsigned short* sl = slot.data() + d.zoff;
-            for (int k = 0; k < nb; ++k) {
-                const int r0 = b[2 * k], p0 = b[2 * k + 1];
-                const int r1 = b[2 * k + 2], p1 = b[2 * k + 3];
-                ord.resize(p1 - p0);
-                std::iota(ord.begin(), ord.end(), 0);
-                if (r1 - r0 > 1)
-                    std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) {
-                        return ix[p0 + u] < ix[p0 + v];
-                    });
-                for (int q = 0; q < p1 - p0; ++q) {
-                    si[p0 + q] = static_cast<unsigned short>(ix[p0 + ord[q]]);
-                    sl[p0 + q] = static_cast<unsigned short>(ord[q]);
-                }
-            }
+            gather_order_blocks(rb.data() + (trans ? d.rbto : d.rbao),
+                                trans ? d.nblkT : d.nblkA, idx + d.zoff,
+                                sidx.data() + d.zoff, slot.data() + d.zoff, ord);
         }
     };
     std::vector<std::thread> th;
@@ -429,6 +433,22 @@ void lockstep_alloc(igaos_batch* B) {
 extern "C" {
 
 const char* igaos_batch_last_error(void) { return g_berr.c_str(); }
+
+int igaos_gather_order_build(int nrows, int ncols, const int* ptr, const int* idx,
+                             unsigned short* sidx, unsigned short* slot) {
+    if (nrows < 0 || ncols < 0 || ncols > 65536 || !ptr || !sidx || !slot ||
+        (ptr[nrows] > 0 && !idx))
+        return bfail("invalid CSR arrays (at most 65,536 columns)");
+    try {
+        const std::vector<int> rb = igaos::dev::build_row_blocks(ptr, nrows);
+        std::vector<int> ord;
+        gather_order_blocks(rb.data(), static_cast<int>(rb.size() / 2) - 1, idx, sidx, slot,
+                            ord);
+    } catch (const std::exception& e) {
+        return bfail(e.what());
+    }
+    return 0;
+}
 
 int igaos_batch_set_mode(igaos_batch* B, int mode) {
     if (!B) return bfail("null batch");

@@ -295,6 +295,14 @@ int igaos_batch_unroll_backward(igaos_batch* batch, const double* gx, const doub
                                 const double* gobj, double* gc, double* grmin,
                                 double* grmax, double* gcl, double* gcu, double* gax);
 const char* igaos_batch_last_error(void);
+/* Host: the gather order the lockstep kernels read (no reference
+ * counterpart): over the row blocks of a CSR matrix with at most 65,536
+ * columns, within every block of more than one row the nonzeros sorted by
+ * column (stable); sidx[q] is the column of the q-th nonzero in that order
+ * (16-bit) and slot[q] its position within the block's row order; blocks
+ * of one row keep their order.  sidx, slot: nnz each.  0 on success. */
+int igaos_gather_order_build(int nrows, int ncols, const int* ptr, const int* idx,
+                             unsigned short* sidx, unsigned short* slot);
 void igaos_batch_destroy(igaos_batch* batch);
 
 /* ---- kernel layer (device pointers, HIP stream or NULL) ----------------- */

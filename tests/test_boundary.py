@@ -268,3 +268,36 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert json.loads(r.stdout.strip().splitlines()[-1])["n_gpus"] == 2
+
+
+def test_gather_order_invariants():
+    # the lockstep kernels' gather order (host builder): inside each
+    # multi-row block a stable sort by column with slots back to row order;
+    # single-row blocks unchanged
+    import ctypes as C
+    L = _core.lib()
+    lp = lpgen.gen_batch_instance(0, 3)
+    for ptr, idx, ncols in ((lp.ap, lp.ai, lp.n), (lp.cp, lp.ci, lp.m)):
+        ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        nnz = int(ptr[-1])
+        sidx, slot = np.zeros(nnz, np.uint16), np.zeros(nnz, np.uint16)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa
+        up = lambda a: a.ctypes.data_as(C.POINTER(C.c_ushort))  # noqa
+        assert L.igaos_gather_order_build(len(ptr) - 1, ncols, ip(ptr), ip(idx), up(sidx),
+                                          up(slot)) == 0
+        pairs = _rowblocks(ptr).reshape(-1, 2)
+        for (r0, p0), (r1, p1) in zip(pairs[:-1], pairs[1:]):
+            sl = slot[p0:p1].astype(np.int64)
+            assert np.array_equal(np.sort(sl), np.arange(p1 - p0))
+            assert np.array_equal(sidx[p0:p1], idx[p0 + sl])
+            if r1 - r0 > 1:
+                assert np.all(np.diff(sidx[p0:p1].astype(np.int64)) >= 0)
+                # stable: equal columns keep their row order
+                eq = np.diff(sidx[p0:p1].astype(np.int64)) == 0
+                assert np.all(np.diff(sl)[eq] > 0)
+            else:
+                assert np.array_equal(sl, np.arange(p1 - p0))
+    assert L.igaos_gather_order_build(2, 70000, ip(np.array([0, 0, 0], np.int32)), None,
+                                      up(np.zeros(1, np.uint16)),
+                                      up(np.zeros(1, np.uint16))) != 0
