@@ -212,16 +212,18 @@ bool to_options(const igaos_options* opt, igaos::Options& o) {
     return true;
 }
 
-bool to_config(const igaos_engine_config* cfg, igaos::pdhg::EngineConfig& ec) {
-    if (!cfg) return true;
-    if (cfg->precision != 32 && cfg->precision != 64) return false;
+// nullptr on success, else what is wrong with the configuration
+const char* to_config(const igaos_engine_config* cfg, igaos::pdhg::EngineConfig& ec) {
+    if (!cfg) return nullptr;
+    if (cfg->precision != 32 && cfg->precision != 64) return "precision must be 32 or 64";
+    if (cfg->column_slices < -1 || cfg->column_slices > 16)
+        return "column_slices must be -1 (engine's choice), 0 (off) or 1..16";
     ec.precision = cfg->precision;
     ec.graph_slots = std::max(0, cfg->graph_slots);
     ec.max_repair_m = cfg->max_repair_m;
     ec.device = cfg->device;
-    if (cfg->column_slices < -1 || cfg->column_slices > 16) return false;
     ec.column_slices = cfg->column_slices;
-    return true;
+    return nullptr;
 }
 
 void export_solution(const igaos::Solution& s, const igaos::pdhg::SolveInfo& si,
@@ -275,7 +277,7 @@ int igaos_pdhg_solve(const igaos_model* model, const igaos_options* opt,
     igaos::Options o;
     igaos::pdhg::EngineConfig ec;
     if (!model || !to_options(opt, o)) return fail("null model or options");
-    if (!to_config(cfg, ec)) return fail("precision must be 32 or 64");
+    if (const char* err = to_config(cfg, ec)) return fail(err);
     igaos::pdhg::SolveInfo si;
     const igaos::Solution s = igaos::pdhg::solve_ex(model->m, o, ec, &si);
     export_solution(s, si, model->m.m, model->m.n, x, y, info, msg, msglen);
@@ -289,8 +291,8 @@ igaos_engine* igaos_pdhg_create(const igaos_model* model,
         fail("null model");
         return nullptr;
     }
-    if (!to_config(cfg, ec)) {
-        fail("precision must be 32 or 64");
+    if (const char* err = to_config(cfg, ec)) {
+        fail(err);
         return nullptr;
     }
     try {

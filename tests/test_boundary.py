@@ -301,3 +301,20 @@ def test_gather_order_invariants():
     assert L.igaos_gather_order_build(2, 70000, ip(np.array([0, 0, 0], np.int32)), None,
                                       up(np.zeros(1, np.uint16)),
                                       up(np.zeros(1, np.uint16))) != 0
+
+
+def test_layer_and_engine_argument_errors():
+    # argument errors surface before any device work
+    from igaos_amd.autograd import DiffLPBatch
+    lp = lpgen.gen_batch_instance(0, 0, m_hi=40)
+    with pytest.raises(ValueError):
+        DiffLPBatch([lp], differentiation="both")
+    # the engine configuration rejects an out-of-range slice count
+    import ctypes as C
+    cfg = igaos_amd._core.EngineConfig()
+    _core.lib().igaos_engine_config_default(C.byref(cfg))
+    assert cfg.column_slices == -1 and cfg.precision == 64
+    cfg.column_slices = 17
+    m = igaos_amd.api.model_from_lp(lp)
+    assert not _core.lib().igaos_pdhg_create(m._h, C.byref(cfg))
+    assert "column_slices" in _core.last_error()
