@@ -32,7 +32,8 @@ FP32 solve (the LP needs well over 10^4 iterations to reach 1e-6).
 ``cpu_baseline`` is the oracle (a single-threaded C host replay of the
 reference engine) on a bounded sample of the same workload, named in
 ``sample``.  Beside it, ``cpu_baseline_all_cores`` runs the same replay on
-up to 16 host threads (one instance per thread) over a larger sample, and
+up to 16 host threads (one instance per thread for the batch, the rows of
+each sparse product split among the threads for the Zipf LP), and
 ``cpu_baseline_reference_simplex`` times the reference's own CPU path for
 these LPs, its revised simplex built from its sources (``oracle/_ref``,
 present when it was built), on the single-thread sample.
@@ -410,7 +411,7 @@ def bench_zipf(args):
     b_dom = k64[f"bytes_{dom}"]
     achieved = b_dom / ms_dom / 1e6
     solved = 1 if s.status == "near-optimal" else 0
-    cpu = None
+    cpu = cpu_par = None
     if not args.no_cpu_baseline:
         import oracle
         t0 = time.perf_counter()
@@ -421,6 +422,23 @@ def bench_zipf(args):
                "kind": "port",
                "sample": (f"oracle host replay (1 thread): setup + 64 "
                           f"iterations of the same LP, {wall:.1f} s wall")}
+        log(f"[bench] oracle 1 thread: {ips:.2f} it/s")
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        try:
+            t0 = time.perf_counter()
+            r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=64)
+            wall = time.perf_counter() - t0
+        finally:
+            oracle.set_threads(1)
+        ips = r.iterations / (r.solve_time_ms * 1e-3)
+        cpu_par = {"value": ips, "unit": "iterations/s", "cores": threads,
+                   "kind": "port",
+                   "sample": (f"oracle host replay, sparse products row-split "
+                              f"over {threads} threads: setup + 64 iterations "
+                              f"of the same LP, {wall:.1f} s wall"),
+                   "cpu_model": cpu_model()}
+        log(f"[bench] oracle {threads} threads: {ips:.2f} it/s")
     return {
         "metric": METRIC, "value": solved / el, "unit": "LPs/s", "n_gpus": 1,
         "steps": 1, "warmup": 0, "ms_per_step": el * 1000.0,
@@ -441,6 +459,7 @@ def bench_zipf(args):
                      "bytes_per_launch": b_dom,
                      "ms_per_launch": round(ms_dom, 4)},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_par,
     }
 
 

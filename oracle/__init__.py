@@ -90,6 +90,7 @@ def load():
         lib.orc_dual_repair.argtypes = [C.POINTER(_Model), _dp, _dp, _dp,
                                         C.c_double, _dp]
         lib.orc_set_mutation.argtypes = [C.c_int]
+        lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_avg_weight.restype = C.c_double
         lib.orc_avg_weight.argtypes = [C.c_long]
         lib.orc_mix_weights.argtypes = [C.c_int, _dp]
@@ -158,6 +159,11 @@ class Result:
 
 def set_mutation(code: int):
     load().orc_set_mutation(int(code))
+
+
+def set_threads(k: int):
+    """Host threads of the oracle's SpMV (rows split; results unchanged)."""
+    load().orc_set_threads(int(k))
 
 
 def solve(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,

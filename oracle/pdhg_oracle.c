@@ -31,6 +31,11 @@
 static int g_mut = 0;
 void orc_set_mutation(int code) { g_mut = code; }
 
+/* Host threads of the SpMV (rows split among them; each row still sums in
+ * storage order, so results do not depend on the count).  Default 1. */
+static int g_threads = 1;
+void orc_set_threads(int k) { g_threads = k > 0 ? k : 1; }
+
 static double dmax(double a, double b) { return a > b ? a : b; }
 static double dmin(double a, double b) { return a < b ? a : b; }
 /* CUDA fmin/fmax semantics (NaN-ignoring), as used in pdhg.cu:41,56 */
@@ -51,6 +56,7 @@ static double now_s(void) {
  * pdhg.cu:150-156.  Sequential per-row sum in storage order. */
 void orc_spmv(int nrows, const int* ptr, const int* idx, const double* val,
               const double* x, double* y) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1 && nrows > 4096)
     for (int i = 0; i < nrows; ++i) {
         double s = 0.0;
         for (int p = ptr[i]; p < ptr[i + 1]; ++p) s += val[p] * x[idx[p]];

@@ -47,6 +47,7 @@ typedef struct {
 
 /* mutation switch for pin tests: 0 = faithful.  See pdhg_oracle.c. */
 void orc_set_mutation(int code);
+void orc_set_threads(int k);
 
 void orc_spmv(int nrows, const int* ptr, const int* idx, const double* val,
               const double* x, double* y);

@@ -56,6 +56,26 @@ def test_spmv_integer_exact_and_transpose():
                           A.T @ yv)
 
 
+def test_spmv_threads_do_not_change_results():
+    # rows are split among threads, each row still sums in storage order
+    rng = np.random.default_rng(2)
+    A = _rand_csr(rng, 9000, 3000, 0.004, empty_rows=(0, 4500, 8999))
+    x = rng.standard_normal(3000)
+    y1 = oracle.spmv(A.indptr, A.indices, A.data, x)
+    lp = lpgen.gen_random_eq(seed=5, m=5000, n=12000)
+    r1 = oracle.solve(lp, tolerance=0.0, time_limit=1e7, max_iterations=128)
+    oracle.set_threads(4)
+    try:
+        y4 = oracle.spmv(A.indptr, A.indices, A.data, x)
+        r4 = oracle.solve(lp, tolerance=0.0, time_limit=1e7, max_iterations=128)
+    finally:
+        oracle.set_threads(1)
+    assert np.array_equal(y1, y4)
+    np.testing.assert_allclose(y1, A @ x, rtol=1e-12, atol=1e-12)
+    assert r1.iterations == r4.iterations == 128
+    assert np.array_equal(r1.x, r4.x) and np.array_equal(r1.y, r4.y)
+
+
 # ---------------------------------------------------------------- Ruiz --
 def test_ruiz_equilibrates_rows_and_columns():
     lp = lpgen.gen_zipf_badly_scaled(seed=3, m=300, n=700)
