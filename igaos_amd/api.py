@@ -172,6 +172,8 @@ def model_from_lp(lp) -> Model:
             else a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
     h = _core.lib().igaos_model_from_csr_csc(lp.m, lp.n, *ptrs,
                                              float(lp.obj_const))
+    if not h:
+        raise ValueError(_core.last_error())
     return Model(h)
 
 

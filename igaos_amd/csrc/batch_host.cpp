@@ -19,6 +19,7 @@
 #include "batch.hpp"
 #include "unroll.hpp"
 #include "igaos_pdhg.h"
+#include "model.hpp"
 #include "pdhg_kernels.hpp"
 
 namespace {
@@ -488,6 +489,52 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
         bfail("precision must be 32 or 64");
         return nullptr;
     }
+    if (!c || !cl || !cu || !rmin || !rmax || !ap || !ai || !ax || !cp || !ci || !acx) {
+        bfail("null data array");
+        return nullptr;
+    }
+    try {
+        // every instance's pattern is checked before anything is uploaded:
+        // the kernels index the vectors by these arrays unchecked
+        std::vector<long> rpo(count), cpo(count), zo(count);
+        long r = 0, q = 0, z = 0;
+        for (int i = 0; i < count; ++i) {
+            if (m[i] < 0 || n[i] < 0 || nnz[i] < 0) throw std::invalid_argument("negative size");
+            rpo[i] = r;
+            cpo[i] = q;
+            zo[i] = z;
+            if (ap[r + m[i]] != nnz[i] || cp[q + n[i]] != nnz[i])
+                throw std::invalid_argument("pointer arrays do not match nnz of instance " +
+                                            std::to_string(i));
+            r += m[i] + 1;
+            q += n[i] + 1;
+            z += nnz[i];
+        }
+        std::vector<std::string> bad(count);
+        const int nt = static_cast<int>(
+            std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+        std::atomic<int> next{0};
+        auto work = [&]() {
+            for (int i = next++; i < count; i = next++) {
+                try {
+                    bad[i] = igaos::io::check_csr_csc(m[i], n[i], ap + rpo[i], ai + zo[i],
+                                                      cp + cpo[i], ci + zo[i]);
+                } catch (const std::exception& e) {
+                    bad[i] = e.what();
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+        for (int i = 0; i < count; ++i)
+            if (!bad[i].empty())
+                throw std::invalid_argument("instance " + std::to_string(i) + ": " + bad[i]);
+    } catch (const std::exception& e) {
+        bfail(e.what());
+        return nullptr;
+    }
     try {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -500,7 +547,6 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
         B->desc.resize(count);
         long ro = 0, co = 0, zo = 0, rpo = 0, cpo = 0;
         for (int i = 0; i < count; ++i) {
-            if (m[i] < 0 || n[i] < 0 || nnz[i] < 0) throw std::invalid_argument("negative size");
             LPDesc& d = B->desc[i];
             d.m = m[i];
             d.n = n[i];
@@ -512,9 +558,6 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
             d.rpo = rpo;
             d.cpo = cpo;
             d.obj_const = obj_const ? obj_const[i] : 0.0;
-            if (ap[rpo + m[i]] != nnz[i] || cp[cpo + n[i]] != nnz[i])
-                throw std::invalid_argument("pointer arrays do not match nnz of instance " +
-                                            std::to_string(i));
             const std::vector<int> ba = igaos::dev::build_row_blocks(ap + rpo, m[i]);
             const std::vector<int> bt = igaos::dev::build_row_blocks(cp + cpo, n[i]);
             d.rbao = static_cast<long>(B->rbA.size());
@@ -553,10 +596,10 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
         B->d_order = B->mem.upload(B->order.data(), count);
         B->d_queue = B->mem.alloc<int>(1);
         B->d_res = B->mem.alloc<LPResult>(count);
-        hipDeviceProp_t prop;
-        BCK(hipGetDeviceProperties(&prop, 0));
-        B->cus = prop.multiProcessorCount;
         BCK(hipGetDevice(&B->device));
+        hipDeviceProp_t prop;
+        BCK(hipGetDeviceProperties(&prop, B->device));
+        B->cus = prop.multiProcessorCount;
         const int occ = igaos::batch::batch_occupancy(precision == 32, B->waves);
         B->grid = std::min(count, B->cus * occ);
         BCK(hipEventCreate(&B->e0));

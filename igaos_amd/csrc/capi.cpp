@@ -76,27 +76,37 @@ igaos_model* igaos_model_from_triplets(int m, int n, int nnz,
         fail("negative size");
         return nullptr;
     }
-    std::vector<igaos::io::Triplet> t(nnz);
-    for (int p = 0; p < nnz; ++p) {
-        if (rows[p] < 0 || rows[p] >= m || cols[p] < 0 || cols[p] >= n) {
-            fail("triplet index out of range");
-            return nullptr;
-        }
-        t[p] = {rows[p], cols[p], vals[p]};
+    if ((nnz > 0 && (!rows || !cols || !vals)) || (n > 0 && (!c || !cl || !cu)) ||
+        (m > 0 && (!rmin || !rmax))) {
+        fail("null data array");
+        return nullptr;
     }
-    auto* h = new igaos_model;
-    auto& md = h->m;
-    md.m = m;
-    md.n = n;
-    md.c.assign(c, c + n);
-    md.cl.assign(cl, cl + n);
-    md.cu.assign(cu, cu + n);
-    md.rmin.assign(rmin, rmin + m);
-    md.rmax.assign(rmax, rmax + m);
-    md.integ.assign(n, 0);
-    md.obj_const = obj_const;
-    igaos::io::counting_sort(t, m, n, md.ap, md.ai, md.ax, md.cp, md.ci, md.acx);
-    return h;
+    try {
+        std::vector<igaos::io::Triplet> t(nnz);
+        for (int p = 0; p < nnz; ++p) {
+            if (rows[p] < 0 || rows[p] >= m || cols[p] < 0 || cols[p] >= n) {
+                fail("triplet index out of range");
+                return nullptr;
+            }
+            t[p] = {rows[p], cols[p], vals[p]};
+        }
+        auto h = std::make_unique<igaos_model>();
+        auto& md = h->m;
+        md.m = m;
+        md.n = n;
+        md.c.assign(c, c + n);
+        md.cl.assign(cl, cl + n);
+        md.cu.assign(cu, cu + n);
+        md.rmin.assign(rmin, rmin + m);
+        md.rmax.assign(rmax, rmax + m);
+        md.integ.assign(n, 0);
+        md.obj_const = obj_const;
+        igaos::io::counting_sort(t, m, n, md.ap, md.ai, md.ax, md.cp, md.ci, md.acx);
+        return h.release();
+    } catch (const std::exception& e) {
+        fail(e.what());
+        return nullptr;
+    }
 }
 
 igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
@@ -110,29 +120,40 @@ igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
         fail("invalid sizes or pointers");
         return nullptr;
     }
-    const int nnz = ap[m];
-    if (cp[n] != nnz || nnz < 0) {
-        fail("CSR and CSC nonzero counts differ");
+    const std::string bad = igaos::io::check_csr_csc(m, n, ap, ai, cp, ci);
+    if (!bad.empty()) {
+        fail(bad);
         return nullptr;
     }
-    auto* h = new igaos_model;
-    auto& md = h->m;
-    md.m = m;
-    md.n = n;
-    md.c.assign(c, c + n);
-    md.cl.assign(cl, cl + n);
-    md.cu.assign(cu, cu + n);
-    md.rmin.assign(rmin, rmin + m);
-    md.rmax.assign(rmax, rmax + m);
-    md.ap.assign(ap, ap + m + 1);
-    md.ai.assign(ai, ai + nnz);
-    md.ax.assign(ax, ax + nnz);
-    md.cp.assign(cp, cp + n + 1);
-    md.ci.assign(ci, ci + nnz);
-    md.acx.assign(acx, acx + nnz);
-    md.integ.assign(n, 0);
-    md.obj_const = obj_const;
-    return h;
+    const int nnz = ap[m];
+    if ((nnz > 0 && (!ax || !acx)) || (n > 0 && (!c || !cl || !cu)) ||
+        (m > 0 && (!rmin || !rmax))) {
+        fail("null data array");
+        return nullptr;
+    }
+    try {
+        auto h = std::make_unique<igaos_model>();
+        auto& md = h->m;
+        md.m = m;
+        md.n = n;
+        md.c.assign(c, c + n);
+        md.cl.assign(cl, cl + n);
+        md.cu.assign(cu, cu + n);
+        md.rmin.assign(rmin, rmin + m);
+        md.rmax.assign(rmax, rmax + m);
+        md.ap.assign(ap, ap + m + 1);
+        md.ai.assign(ai, ai + nnz);
+        md.ax.assign(ax, ax + nnz);
+        md.cp.assign(cp, cp + n + 1);
+        md.ci.assign(ci, ci + nnz);
+        md.acx.assign(acx, acx + nnz);
+        md.integ.assign(n, 0);
+        md.obj_const = obj_const;
+        return h.release();
+    } catch (const std::exception& e) {
+        fail(e.what());
+        return nullptr;
+    }
 }
 
 void igaos_model_free(igaos_model* model) { delete model; }

@@ -49,4 +49,16 @@ void counting_sort(const std::vector<Triplet>& trips, int rows, int cols,
 // std::out_of_range for a BOUNDS entry naming an unknown column.
 void parse_mps(const std::string& path, Model& model);
 
+// Empty when (ptr, idx) is a compressed pattern of `rows` rows over `cols`
+// columns (ptr[0] = 0, ptr non-decreasing, every index in [0, cols)), else
+// what is wrong with it.  The kernels index vectors by these arrays
+// unchecked, so every array that comes in from a caller passes here first.
+std::string check_compressed(int rows, int cols, const int* ptr, const int* idx);
+
+// check_compressed of the CSR (ap, ai) and of the CSC (cp, ci) of an m x n
+// matrix, and that both hold the same number of entries in every row and
+// in every column.
+std::string check_csr_csc(int m, int n, const int* ap, const int* ai, const int* cp,
+                          const int* ci);
+
 }  // namespace igaos::io

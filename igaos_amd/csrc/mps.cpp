@@ -222,4 +222,39 @@ void parse_mps(const std::string& path, Model& model) {
                   model.cp, model.ci, model.acx);
 }
 
+std::string check_compressed(int rows, int cols, const int* ptr, const int* idx) {
+    if (rows < 0 || cols < 0) return "negative size";
+    if (!ptr) return "null pointer array";
+    if (ptr[0] != 0) return "pointer array does not start at 0";
+    for (int i = 0; i < rows; ++i)
+        if (ptr[i + 1] < ptr[i])
+            return "pointer array decreases at " + std::to_string(i);
+    const int nz = ptr[rows];
+    if (nz > 0 && !idx) return "null index array";
+    for (int p = 0; p < nz; ++p)
+        if (idx[p] < 0 || idx[p] >= cols)
+            return "index " + std::to_string(idx[p]) + " at entry " + std::to_string(p) +
+                   " is outside [0, " + std::to_string(cols) + ")";
+    return {};
+}
+
+std::string check_csr_csc(int m, int n, const int* ap, const int* ai, const int* cp,
+                          const int* ci) {
+    std::string e = check_compressed(m, n, ap, ai);
+    if (!e.empty()) return "CSR: " + e;
+    e = check_compressed(n, m, cp, ci);
+    if (!e.empty()) return "CSC: " + e;
+    if (ap[m] != cp[n]) return "CSR and CSC nonzero counts differ";
+    std::vector<int> per_col(n, 0), per_row(m, 0);
+    for (int p = 0; p < ap[m]; ++p) per_col[ai[p]]++;
+    for (int p = 0; p < cp[n]; ++p) per_row[ci[p]]++;
+    for (int j = 0; j < n; ++j)
+        if (per_col[j] != cp[j + 1] - cp[j])
+            return "CSR and CSC differ in the entries of column " + std::to_string(j);
+    for (int i = 0; i < m; ++i)
+        if (per_row[i] != ap[i + 1] - ap[i])
+            return "CSR and CSC differ in the entries of row " + std::to_string(i);
+    return {};
+}
+
 }  // namespace igaos::io

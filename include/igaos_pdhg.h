@@ -50,7 +50,9 @@ igaos_model* igaos_model_from_triplets(int m, int n, int nnz,
                                        const double* cl, const double* cu,
                                        const double* rmin, const double* rmax,
                                        double obj_const);
-/* Builds a model from ready CSR + CSC arrays (copied). */
+/* Builds a model from ready CSR + CSC arrays (copied).  Returns NULL when
+ * a pointer array does not start at 0 or decreases, an index is out of
+ * range, or CSR and CSC disagree in an entry count of a row or column. */
 igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
                                       const double* cl, const double* cu,
                                       const double* rmin, const double* rmax,

@@ -318,3 +318,32 @@ def test_layer_and_engine_argument_errors():
     m = igaos_amd.api.model_from_lp(lp)
     assert not _core.lib().igaos_pdhg_create(m._h, C.byref(cfg))
     assert "column_slices" in _core.last_error()
+
+
+def test_malformed_patterns_are_rejected_before_device_work():
+    # the kernels index vectors by the pattern arrays unchecked, so both
+    # entry points that take caller arrays refuse a malformed pattern
+    import dataclasses
+    from igaos_amd.batch import BatchSolver
+    lp = lpgen.gen_batch_instance(0, 0, m_hi=40)
+    assert lp.nnz > 2
+
+    def bad(**kw):
+        return dataclasses.replace(lp, **{k: np.array(v, dtype=np.int32) for k, v in kw.items()})
+
+    ai = lp.ai.copy()
+    ai[1] = lp.n                       # column index out of range
+    ap = lp.ap.copy()
+    ap[1], ap[2] = ap[2], ap[1] - 1    # row pointer decreases
+    ci = lp.ci.copy()
+    ci[0] = (ci[0] + 1) % lp.m         # CSC entry moved to another row
+    cases = [(bad(ai=ai), "outside"), (bad(ap=ap), "decreases"),
+             (bad(ci=ci), "CSR and CSC differ")]
+    for b, what in cases:
+        with pytest.raises(ValueError, match=what):
+            igaos_amd.api.model_from_lp(b)
+        with pytest.raises(RuntimeError, match="instance 1: .*" + what):
+            BatchSolver([lp, b])
+    # the well-formed instance passes the checks (and then needs a device)
+    m = igaos_amd.api.model_from_lp(lp)
+    assert m.n == lp.n
