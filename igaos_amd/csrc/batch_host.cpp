@@ -402,16 +402,19 @@ void lockstep_loop(igaos_batch* B, const igaos::batch::Arrays<V>* A,
 }
 
 void lockstep_alloc(igaos_batch* B) {
+    // d_ctl is set last: after a failed allocation it stays null and the
+    // next run tries again, keeping what was already allocated
     if (B->d_ctl) return;
     for (auto& t : B->lk_ev)
-        for (hipEvent_t& e : t) BCK(hipEventCreate(&e));
+        for (hipEvent_t& e : t)
+            if (!e) BCK(hipEventCreate(&e));
     Mem& M = B->lkmem;
     long bt = 0, ba = 0;
     for (const LPDesc& d : B->desc) {
         bt += d.nblkT;
         ba += d.nblkA;
     }
-    B->d_ctl = M.alloc<unsigned char>(igaos::batch::lockstep_ctl_bytes() * B->count);
+    unsigned char* ctl = M.alloc<unsigned char>(igaos::batch::lockstep_ctl_bytes() * B->count);
     B->d_tblT = M.alloc<igaos::batch::LkEnt>(bt);
     B->d_tblA = M.alloc<igaos::batch::LkEnt>(ba);
     B->d_rng = M.alloc<igaos::batch::LsRange>(B->count);
@@ -419,14 +422,20 @@ void lockstep_alloc(igaos_batch* B) {
     B->d_partA = M.alloc<double>(2 * ba);
     B->d_done = M.alloc<int>(B->count);
     B->d_ord = M.alloc<int>(B->count);
-    BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_tblT),
-                      sizeof(igaos::batch::LkEnt) * std::max(bt, 1L)));
-    BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_tblA),
-                      sizeof(igaos::batch::LkEnt) * std::max(ba, 1L)));
-    BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_rng),
-                      sizeof(igaos::batch::LsRange) * std::max(B->count, 1)));
-    BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_ord), sizeof(int) * std::max(B->count, 1)));
+    if (!B->h_tblT)
+        BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_tblT),
+                          sizeof(igaos::batch::LkEnt) * std::max(bt, 1L)));
+    if (!B->h_tblA)
+        BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_tblA),
+                          sizeof(igaos::batch::LkEnt) * std::max(ba, 1L)));
+    if (!B->h_rng)
+        BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_rng),
+                          sizeof(igaos::batch::LsRange) * std::max(B->count, 1)));
+    if (!B->h_ord)
+        BCK(hipHostMalloc(reinterpret_cast<void**>(&B->h_ord),
+                          sizeof(int) * std::max(B->count, 1)));
     B->d_nfrozen = M.alloc<int>(1);
+    B->d_ctl = ctl;
 }
 
 }  // namespace

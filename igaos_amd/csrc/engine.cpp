@@ -881,8 +881,15 @@ AttemptTiming Engine<V>::profile(int attempts) {
     st.no_freeze = 1;
     write_state(st);
     sync();
-    hipEvent_t e[3];
-    for (auto& ev : e) HIPCK(hipEventCreate(&ev));
+    struct Events {  // destroyed on every exit, thrown errors included
+        hipEvent_t ev[3] = {};
+        ~Events() {
+            for (hipEvent_t x : ev)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs;
+    hipEvent_t* e = evs.ev;
+    for (int k = 0; k < 3; ++k) HIPCK(hipEventCreate(&e[k]));
     AttemptTiming t;
     t.nblk_primal = rT_.nblk;
     t.nblk_dual = rA_.nblk;
@@ -904,7 +911,6 @@ AttemptTiming Engine<V>::profile(int attempts) {
             b += ms2;
         }
     }
-    for (auto& ev : e) (void)hipEventDestroy(ev);
     t.ms_primal = a / attempts;
     t.ms_dual = b / attempts;
     return t;
