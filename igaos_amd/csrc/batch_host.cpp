@@ -449,6 +449,8 @@ int igaos_gather_order_build(int nrows, int ncols, const int* ptr, const int* id
     if (nrows < 0 || ncols < 0 || ncols > 65536 || !ptr || !sidx || !slot ||
         (ptr[nrows] > 0 && !idx))
         return bfail("invalid CSR arrays (at most 65,536 columns)");
+    const std::string bad = igaos::io::check_compressed(nrows, ncols, ptr, idx);
+    if (!bad.empty()) return bfail(bad);
     try {
         const std::vector<int> rb = igaos::dev::build_row_blocks(ptr, nrows);
         std::vector<int> ord;

@@ -384,6 +384,9 @@ int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
 
 int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap) {
     if (nrows < 0 || !ptr) return fail("invalid row pointer");
+    if (ptr[0] != 0) return fail("pointer array does not start at 0");
+    for (int i = 0; i < nrows; ++i)
+        if (ptr[i + 1] < ptr[i]) return fail("pointer array decreases at " + std::to_string(i));
     const std::vector<int> rb = igaos::dev::build_row_blocks(ptr, nrows);
     const int k = static_cast<int>(rb.size());
     if (out && cap >= k) std::memcpy(out, rb.data(), sizeof(int) * k);
@@ -395,6 +398,8 @@ int igaos_slices_build(int nrows, int ncols, const int* ptr, const int* idx, int
     if (nrows < 0 || ncols < 0 || !ptr || (ptr[nrows] > 0 && !idx))
         return fail("invalid CSR arrays");
     if (slices < 1 || slices > igaos::dev::MAX_SLICES) return fail("slices must be 1..16");
+    const std::string bad = igaos::io::check_compressed(nrows, ncols, ptr, idx);
+    if (!bad.empty()) return fail(bad);
     try {
         const igaos::dev::HostSlices h =
             igaos::dev::build_slices(nrows, ncols, ptr, idx, slices);

@@ -302,7 +302,8 @@ const char* igaos_batch_last_error(void);
  * columns, within every block of more than one row the nonzeros sorted by
  * column (stable); sidx[q] is the column of the q-th nonzero in that order
  * (16-bit) and slot[q] its position within the block's row order; blocks
- * of one row keep their order.  sidx, slot: nnz each.  0 on success. */
+ * of one row keep their order.  sidx, slot: nnz each.  0 on success; -1
+ * (igaos_batch_last_error) for a malformed pattern or more columns. */
 int igaos_gather_order_build(int nrows, int ncols, const int* ptr, const int* idx,
                              unsigned short* sidx, unsigned short* slot);
 void igaos_batch_destroy(igaos_batch* batch);
@@ -313,7 +314,8 @@ void igaos_batch_destroy(igaos_batch* batch);
  * pair (nrows, nnz), i.e. 2 * (blocks + 1) ints.  A block holds at most
  * 2048 nonzeros and 256 rows, or a single longer row.  Writes at most cap
  * ints; returns the number of ints, or the required count if cap is too
- * small.  The device copy must be 8-byte aligned. */
+ * small, or -1 (igaos_last_error) when ptr does not start at 0 or
+ * decreases.  The device copy must be 8-byte aligned. */
 int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap);
 /* Host: the column-sliced copy of a CSR matrix that the engine's large-LP
  * products use (no reference counterpart): the columns cut into `slices`
@@ -323,7 +325,8 @@ int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap);
  * position in the input), and per row block (slice-major) its (first row,
  * end row, first nonzero, end nonzero) into blocks and its slice into
  * bslice.  Returns the number of blocks; when cap (blocks' capacity in
- * blocks) is smaller, writes nothing and returns the count needed. */
+ * blocks) is smaller, writes nothing and returns the count needed; -1
+ * (igaos_last_error) for a malformed pattern. */
 int igaos_slices_build(int nrows, int ncols, const int* ptr, const int* idx, int slices,
                        int* sptr, int* sidx, int* src, int* blocks, int* bslice, int cap);
 /* y = A x with A in CSR (device arrays); replaces the cusparseSpMV calls

@@ -347,3 +347,31 @@ def test_malformed_patterns_are_rejected_before_device_work():
     # the well-formed instance passes the checks (and then needs a device)
     m = igaos_amd.api.model_from_lp(lp)
     assert m.n == lp.n
+
+
+def test_layout_builders_refuse_malformed_patterns():
+    # the host builders index their own arrays by the caller's pattern
+    import ctypes as C
+    L = _core.lib()
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa
+    up = lambda a: a.ctypes.data_as(C.POINTER(C.c_ushort))  # noqa
+    ptr = np.array([0, 2, 3], np.int32)
+    good = np.array([0, 4, 1], np.int32)
+    out = np.zeros(64, np.int32)
+    nb = np.zeros(1, np.int32)
+    assert L.igaos_rowblocks_build(2, ip(ptr), ip(out), 64) == 4
+    assert L.igaos_rowblocks_build(2, ip(np.array([1, 2, 3], np.int32)), ip(out), 64) < 0
+    assert "start at 0" in _core.last_error()
+    assert L.igaos_rowblocks_build(2, ip(np.array([0, 3, 2], np.int32)), ip(out), 64) < 0
+    assert "decreases" in _core.last_error()
+    big = np.zeros(16, np.int32)
+    args = lambda idx: (2, 5, ip(ptr), ip(idx), 2, ip(big), ip(big), ip(big), ip(big),  # noqa
+                        ip(nb), 1)
+    assert L.igaos_slices_build(*args(good)) > 0
+    assert L.igaos_slices_build(*args(np.array([0, 5, 1], np.int32))) < 0
+    assert "outside" in _core.last_error()
+    s16 = np.zeros(3, np.uint16)
+    assert L.igaos_gather_order_build(2, 5, ip(ptr), ip(good), up(s16), up(s16)) == 0
+    assert L.igaos_gather_order_build(2, 5, ip(ptr), ip(np.array([0, -1, 1], np.int32)),
+                                      up(s16), up(s16)) < 0
+    assert "outside" in L.igaos_batch_last_error().decode()
