@@ -120,17 +120,22 @@ class BatchSolver:
         if L.igaos_batch_launch(self._h, C.byref(opt)) != 0:
             raise RuntimeError(L.igaos_batch_last_error().decode())
         if progress is not None:
-            last = time.perf_counter()
-            progress(0, self.count)
-            while True:
-                done = L.igaos_batch_progress(self._h)
-                if done >= self.count:
-                    progress(done, self.count)
-                    break
-                if time.perf_counter() - last >= poll_s:
-                    progress(done, self.count)
-                    last = time.perf_counter()
-                time.sleep(0.05)
+            try:
+                last = time.perf_counter()
+                progress(0, self.count)
+                while True:
+                    done = L.igaos_batch_progress(self._h)
+                    if done >= self.count:
+                        progress(done, self.count)
+                        break
+                    if time.perf_counter() - last >= poll_s:
+                        progress(done, self.count)
+                        last = time.perf_counter()
+                    time.sleep(0.05)
+            except BaseException:
+                # the launched run still ends before the batch is usable again
+                L.igaos_batch_wait(self._h, None, None, None)
+                raise
         rc = L.igaos_batch_wait(
             self._h,
             x.ctypes.data_as(C.POINTER(C.c_double)) if want_xy else None,
@@ -180,6 +185,9 @@ class BatchSolver:
         L = _core.lib()
         conv = lambda v: None if v is None else np.ascontiguousarray(np.concatenate(v) if isinstance(v, list) else v, np.float64)  # noqa
         xa, ya = conv(x), conv(y)
+        for key, a, size in (("x", xa, self._coff[-1]), ("y", ya, self._roff[-1])):
+            if a is not None and a.shape != (int(size),):
+                raise ValueError(f"{key} must have {int(size)} entries in all, got {a.shape}")
         dp = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
         if L.igaos_batch_set_solution(self._h, dp(xa), dp(ya)) != 0:
             raise RuntimeError(L.igaos_batch_last_error().decode())
