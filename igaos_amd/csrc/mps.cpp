@@ -6,16 +6,22 @@
 // into two-sided rows by the MPS sign rules; BOUNDS types UP (a negative
 // UP with a zero lower bound frees the lower bound), LO, FX, FR, MI, PL,
 // BV, LI, UI; 'MARKER' INTORG/INTEND lines flag integer columns.  The
-// tokenizer splits each line on blanks itself instead of going through an
-// istringstream per line.
+// file is read whole and tokenised in place (string views into the buffer,
+// name tables keyed by those views, numbers converted where they lie)
+// instead of going through an istringstream and string copies per line.
+#include <charconv>
+#include <cerrno>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
+#include <iterator>
 #include <limits>
 #include <stdexcept>
 #include <string>
-#include <unordered_map>
+#include <string_view>
 #include <utility>
 #include <vector>
 
@@ -54,21 +60,130 @@ void counting_sort(const std::vector<Triplet>& trips, int rows, int cols,
 
 namespace {
 
-void split_blanks(const std::string& line, std::vector<std::string>& tok) {
+// The whole file in one string (NUL-terminated, so strtod stops at its end).
+std::string slurp(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open " + path);
+    std::string buf;
+    in.seekg(0, std::ios::end);
+    const std::streamoff size = in.tellg();
+    if (size > 0) {
+        buf.resize(static_cast<size_t>(size));
+        in.seekg(0, std::ios::beg);
+        in.read(&buf[0], size);
+        buf.resize(static_cast<size_t>(in.gcount()));
+    } else {  // not seekable (a pipe): read it as a stream
+        in.clear();
+        in.seekg(0, std::ios::beg);
+        buf.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+    }
+    return buf;
+}
+
+// Token separators: the blanks of the classic locale (space, \t, \n, \v,
+// \f, \r), the ones a default-constructed istringstream splits words on.
+struct Blanks {
+    bool is[256] = {};
+    Blanks() {
+        for (unsigned char c : {' ', '\t', '\n', '\v', '\f', '\r'}) is[c] = true;
+    }
+    bool operator()(char c) const { return is[static_cast<unsigned char>(c)]; }
+};
+
+void split_blanks(std::string_view line, const Blanks& blank,
+                  std::vector<std::string_view>& tok) {
     tok.clear();
-    size_t i = 0, L = line.size();
+    size_t i = 0;
+    const size_t L = line.size();
     while (i < L) {
-        while (i < L && std::isspace(static_cast<unsigned char>(line[i]))) ++i;
+        while (i < L && blank(line[i])) ++i;
         size_t j = i;
-        while (j < L && !std::isspace(static_cast<unsigned char>(line[j]))) ++j;
-        if (j > i) tok.emplace_back(line, i, j - i);
+        while (j < L && !blank(line[j])) ++j;
+        if (j > i) tok.push_back(line.substr(i, j - i));
         i = j;
     }
 }
 
-double to_num(const std::string& s) {
-    size_t used = 0;
-    double v = std::stod(s, &used);  // throws std::invalid_argument like stod
+// Name -> index table with open addressing (linear probing over a
+// power-of-two array, load <= 1/2): keys are views into the file buffer,
+// each slot keeps the key's 64-bit FNV-1a hash so most probes compare one
+// word.  A chained std::unordered_map spent most of a large file's reading
+// time walking its bucket nodes.
+class NameTable {
+  public:
+    // pointer to the value of `key`, or nullptr
+    int* find(std::string_view key) {
+        const uint64_t h = hash(key);
+        for (size_t s = h & mask_;; s = (s + 1) & mask_) {
+            Slot& e = slots_[s];
+            if (!e.used) return nullptr;
+            if (e.h == h && e.key == key) return &e.val;
+        }
+    }
+    // value of `key`, inserted as `val` when absent; `inserted` tells which
+    int& get(std::string_view key, int val, bool& inserted) {
+        if (2 * (count_ + 1) > slots_.size()) grow();
+        const uint64_t h = hash(key);
+        size_t s = h & mask_;
+        for (;; s = (s + 1) & mask_) {
+            Slot& e = slots_[s];
+            if (!e.used) break;
+            if (e.h == h && e.key == key) {
+                inserted = false;
+                return e.val;
+            }
+        }
+        slots_[s] = Slot{key, h, val, true};
+        ++count_;
+        inserted = true;
+        return slots_[s].val;
+    }
+    size_t size() const { return count_; }
+
+  private:
+    struct Slot {
+        std::string_view key;
+        uint64_t h = 0;
+        int val = 0;
+        bool used = false;
+    };
+    static uint64_t hash(std::string_view k) {
+        uint64_t h = 1469598103934665603ull;
+        for (char c : k) h = (h ^ static_cast<unsigned char>(c)) * 1099511628211ull;
+        return h;
+    }
+    void grow() {
+        std::vector<Slot> old;
+        old.swap(slots_);
+        slots_.assign(old.empty() ? 1024 : 2 * old.size(), Slot{});
+        mask_ = slots_.size() - 1;
+        for (const Slot& e : old) {
+            if (!e.used) continue;
+            size_t s = e.h & mask_;
+            while (slots_[s].used) s = (s + 1) & mask_;
+            slots_[s] = e;
+        }
+    }
+    std::vector<Slot> slots_;
+    size_t mask_ = 0, count_ = 0;
+};
+
+// std::stod on the token.  Plain decimals go through std::from_chars
+// (correctly rounded, as strtod is, so the value is the same double); a
+// result that is not a normal nonzero number, or a form from_chars does not
+// take (a leading '+', hexadecimal, inf, nan), goes through strtod from the
+// token's first character (a token ends at a blank or at the end of the
+// buffer, where strtod stops as well) with stod's exceptions for no
+// conversion and for ERANGE.
+double to_num(std::string_view s) {
+    double v = 0.0;
+    const auto r = std::from_chars(s.data(), s.data() + s.size(), v);
+    if (r.ec == std::errc() && std::isnormal(v)) return v;
+    char* end = nullptr;
+    errno = 0;
+    v = std::strtod(s.data(), &end);
+    if (end == s.data()) throw std::invalid_argument("stod");
+    if (errno == ERANGE) throw std::out_of_range("stod");
     return v;
 }
 
@@ -76,21 +191,31 @@ double to_num(const std::string& s) {
 
 void parse_mps(const std::string& path, Model& model) {
     const double INF = std::numeric_limits<double>::infinity();
-    std::ifstream in(path);
-    if (!in) throw std::runtime_error("cannot open " + path);
-    std::string line, section, objname;
-    std::unordered_map<std::string, int> rowidx, colidx;
+    const std::string buf = slurp(path);
+    const Blanks blank;
+    std::string_view section, objname;
+    NameTable rowidx, colidx;
+    bool fresh = false;
+    std::string_view last_col;  // COLUMNS lines come grouped by column
+    int last_j = -1;
     std::vector<char> rsense;
     std::vector<double> rhs;
     std::vector<Triplet> trips;
     std::vector<std::pair<int, double>> ranges;
-    std::vector<std::string> t;
+    std::vector<std::string_view> t;
     bool pending_int = false;
 
-    while (std::getline(in, line)) {
+    const char* const base = buf.data();
+    size_t pos = 0;
+    const size_t size = buf.size();
+    while (pos < size) {
+        const void* nl = std::memchr(base + pos, '\n', size - pos);
+        const size_t eol = nl ? static_cast<size_t>(static_cast<const char*>(nl) - base) : size;
+        const std::string_view line(base + pos, eol - pos);
+        pos = eol + 1;
         if (line.empty() || line[0] == '*') continue;
         const bool header = !(line[0] == ' ' || line[0] == '\t');
-        split_blanks(line, t);
+        split_blanks(line, blank, t);
         if (t.empty()) continue;
         if (header) {
             section = t[0];
@@ -101,28 +226,29 @@ void parse_mps(const std::string& path, Model& model) {
             if (t.size() < 2) continue;
             if (s == 'N') {
                 if (objname.empty()) objname = t[1];
-                rowidx[t[1]] = -1;
+                rowidx.get(t[1], -1, fresh) = -1;
             } else {
-                rowidx[t[1]] = static_cast<int>(rsense.size());
+                const int i = static_cast<int>(rsense.size());
+                rowidx.get(t[1], i, fresh) = i;  // a repeated name now means the new row
                 rsense.push_back(s);
                 rhs.push_back(0.0);
             }
         } else if (section == "COLUMNS") {
-            if (line.find("'MARKER'") != std::string::npos) {
-                pending_int = line.find("INTORG") != std::string::npos;
+            if (line.find("'MARKER'") != std::string_view::npos) {
+                pending_int = line.find("INTORG") != std::string_view::npos;
                 continue;
             }
-            auto it = colidx.find(t[0]);
-            int j;
-            if (it == colidx.end()) {
-                j = static_cast<int>(colidx.size());
-                colidx.emplace(t[0], j);
-                model.c.push_back(0.0);
-                model.cl.push_back(0.0);
-                model.cu.push_back(INF);
-                model.integ.push_back(pending_int ? 1 : 0);
-            } else {
-                j = it->second;
+            int j = last_j;
+            if (last_j < 0 || t[0] != last_col) {
+                j = colidx.get(t[0], static_cast<int>(colidx.size()), fresh);
+                if (fresh) {
+                    model.c.push_back(0.0);
+                    model.cl.push_back(0.0);
+                    model.cu.push_back(INF);
+                    model.integ.push_back(pending_int ? 1 : 0);
+                }
+                last_col = t[0];
+                last_j = j;
             }
             for (size_t k = 1; k + 1 < t.size(); k += 2) {
                 const double v = to_num(t[k + 1]);
@@ -130,9 +256,9 @@ void parse_mps(const std::string& path, Model& model) {
                     model.c[j] += v;
                     continue;
                 }
-                auto r = rowidx.find(t[k]);
-                if (r == rowidx.end() || r->second < 0) continue;
-                trips.push_back({r->second, j, v});
+                const int* r = rowidx.find(t[k]);
+                if (!r || *r < 0) continue;
+                trips.push_back({*r, j, v});
             }
         } else if (section == "RHS") {
             for (size_t k = 1; k + 1 < t.size(); k += 2) {
@@ -140,31 +266,33 @@ void parse_mps(const std::string& path, Model& model) {
                     model.obj_const -= to_num(t[k + 1]);
                     continue;
                 }
-                auto r = rowidx.find(t[k]);
-                if (r == rowidx.end() || r->second < 0) continue;
-                rhs[r->second] = to_num(t[k + 1]);
+                const int* r = rowidx.find(t[k]);
+                if (!r || *r < 0) continue;
+                rhs[*r] = to_num(t[k + 1]);
             }
         } else if (section == "RANGES") {
             for (size_t k = 1; k + 1 < t.size(); k += 2) {
-                auto r = rowidx.find(t[k]);
-                if (r == rowidx.end() || r->second < 0) continue;
-                ranges.emplace_back(r->second, to_num(t[k + 1]));
+                const int* r = rowidx.find(t[k]);
+                if (!r || *r < 0) continue;
+                ranges.emplace_back(*r, to_num(t[k + 1]));
                 ++model.n_ranges_parsed;
             }
         } else if (section == "QUADOBJ" || section == "QSECTION" ||
                    section == "QMATRIX") {
             for (size_t k = 0; k + 2 < t.size(); k += 3) {
-                auto c1 = colidx.find(t[k]);
-                auto c2 = colidx.find(t[k + 1]);
-                if (c1 == colidx.end() || c2 == colidx.end()) continue;
-                model.q_i.push_back(c1->second);
-                model.q_j.push_back(c2->second);
+                const int* c1 = colidx.find(t[k]);
+                const int* c2 = colidx.find(t[k + 1]);
+                if (!c1 || !c2) continue;
+                model.q_i.push_back(*c1);
+                model.q_j.push_back(*c2);
                 model.q_v.push_back(to_num(t[k + 2]));
             }
         } else if (section == "BOUNDS") {
             if (t.size() < 3) throw std::out_of_range("short BOUNDS line");
-            const std::string& type = t[0];
-            const int j = colidx.at(t[2]);
+            const std::string_view type = t[0];
+            const int* jp = colidx.find(t[2]);
+            if (!jp) throw std::out_of_range("BOUNDS names an unknown column");
+            const int j = *jp;
             const double v = t.size() > 3 ? to_num(t[3]) : 0.0;
             if (type == "UP") {
                 model.cu[j] = v;

@@ -107,6 +107,31 @@ def test_mps_ranges_sign_rules_and_markers(tmp_path):
     assert m.n_int == 1 and a["cu"][0] == 1.0
 
 
+def test_mps_tokens_numbers_and_layout(tmp_path):
+    # the reference tokenises with istringstream (classic blanks) and
+    # converts with std::stod: '+', hexadecimal and zero values parse as
+    # stod parses them; CRLF lines, a missing final newline, a column
+    # whose entries are not contiguous and a repeated row name read as
+    # the reference reads them
+    text = ("NAME T\r\nROWS\r\n N obj\r\n E r1\r\n L r2\r\n G r1\r\nCOLUMNS\r\n"
+            " x obj +2.5 r1 0x1p3\r\n y\tobj -0.0 r2 1e2\r\n x r2 0\r\n"
+            " y r1 -.5\r\nRHS\r\n B r1 +1 r2 4.\r\nBOUNDS\r\n UP BND y 1.5E1\r\nENDATA")
+    m = _mps(tmp_path, text)
+    assert (m.m, m.n) == (3, 2)
+    a = m.arrays()
+    # r1 names the G row (index 2); the E row keeps its sense, loses its name
+    assert list(a["rmin"]) == [0.0, -INF, 1.0] and list(a["rmax"]) == [0.0, 4.0, INF]
+    assert list(a["c"]) == [2.5, 0.0] and a["cu"][1] == 15.0
+    lp = m.to_lp()
+    A = lp.to_scipy().toarray()
+    assert A.tolist() == [[0.0, 0.0], [0.0, 100.0], [8.0, -0.5]]
+    assert lp.nnz == 4 and list(lp.ai) == [1, 0, 0, 1]   # explicit zero kept
+    # stod's range error (a subnormal) and an unknown BOUNDS column are errors
+    for bad in (text.replace("1e2", "1e-310"), text.replace("BND y", "BND z")):
+        with pytest.raises(RuntimeError):
+            _mps(tmp_path, bad)
+
+
 def test_mps_afiro_stats_and_csc_is_transpose():
     m = igaos_amd.read_mps(netlib_path("afiro"))
     assert (m.m, m.n, m.nnz()) == (27, 32, 83)
