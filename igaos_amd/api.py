@@ -195,7 +195,14 @@ def _run(model: Model, opt: Options, cfg: EngineConfig | None,
                               msg, 512)
     if rc != 0:
         raise ValueError(_core.last_error())
-    return _solution(info, x, y, msg)
+    act = np.zeros(0)
+    if Status(info.status) != Status.Error:
+        # A x as the reference simplex fills it (igaos_model_row_activity)
+        act = np.zeros(model.m)
+        if L.igaos_model_row_activity(model._h, xp, act.ctypes.data_as(
+                C.POINTER(C.c_double))) != 0:
+            raise RuntimeError(_core.last_error())
+    return _solution(info, x, y, msg, act)
 
 
 class Engine:
@@ -253,12 +260,12 @@ class Engine:
         self.close()
 
 
-def _solution(info, x, y, msg) -> Solution:
+def _solution(info, x, y, msg, row_activity) -> Solution:
     st = Status(info.status)
     extra = {k: getattr(info, k) for k, _ in SolveInfo._fields_}
     return Solution(status=STATUS_NAMES[st], status_enum=st,
                     objective=info.objective, x=x, y=y,
-                    row_activity=np.zeros(0), pinf=info.pinf, dinf=info.dinf,
+                    row_activity=row_activity, pinf=info.pinf, dinf=info.dinf,
                     rel_gap=info.rel_gap, iterations=info.iterations,
                     solve_time_ms=info.solve_time_ms,
                     message=msg.value.decode(errors="replace"), info=extra)

@@ -158,6 +158,13 @@ igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
 
 void igaos_model_free(igaos_model* model) { delete model; }
 
+int igaos_model_row_activity(const igaos_model* model, const double* x, double* out) {
+    if (!model || (model->m.n > 0 && !x) || (model->m.m > 0 && !out))
+        return fail("null model or array");
+    igaos::io::row_activity(model->m, x, out);
+    return 0;
+}
+
 void igaos_model_dims(const igaos_model* h, int* m, int* n, int* nnz,
                       double* obj_const, int* n_fr, int* n_ranges,
                       int* n_int) {

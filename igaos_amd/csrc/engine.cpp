@@ -954,7 +954,12 @@ Solution solve_ex(const io::Model& model, const Options& options,
 }
 
 Solution solve(const io::Model& model, const Options& options) {
-    return solve_ex(model, options, EngineConfig{}, nullptr);
+    Solution sol = solve_ex(model, options, EngineConfig{}, nullptr);
+    if (sol.status != Status::Error && sol.x.size() == static_cast<size_t>(model.n)) {
+        sol.row_activity.resize(model.m);
+        io::row_activity(model, sol.x.data(), sol.row_activity.data());
+    }
+    return sol;
 }
 
 }  // namespace igaos::pdhg

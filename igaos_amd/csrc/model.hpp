@@ -49,6 +49,11 @@ void counting_sort(const std::vector<Triplet>& trips, int rows, int cols,
 // std::out_of_range for a BOUNDS entry naming an unknown column.
 void parse_mps(const std::string& path, Model& model);
 
+// Row activity A x of a solution, in FP64: each row summed from 0 in CSR
+// order, the computation of the reference simplex (src/simplex/
+// simplex.cpp:1727-1730).  out has m entries.
+void row_activity(const Model& model, const double* x, double* out);
+
 // Empty when (ptr, idx) is a compressed pattern of `rows` rows over `cols`
 // columns (ptr[0] = 0, ptr non-decreasing, every index in [0, cols)), else
 // what is wrong with it.  The kernels index vectors by these arrays

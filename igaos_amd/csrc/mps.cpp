@@ -350,6 +350,14 @@ void parse_mps(const std::string& path, Model& model) {
                   model.cp, model.ci, model.acx);
 }
 
+void row_activity(const Model& model, const double* x, double* out) {
+    for (int i = 0; i < model.m; ++i) {
+        double s = 0.0;
+        for (int p = model.ap[i]; p < model.ap[i + 1]; ++p) s += model.ax[p] * x[model.ai[p]];
+        out[i] = s;
+    }
+}
+
 std::string check_compressed(int rows, int cols, const int* ptr, const int* idx) {
     if (rows < 0 || cols < 0) return "negative size";
     if (!ptr) return "null pointer array";

@@ -34,6 +34,9 @@ struct SolveInfo {
     double kernel_ms = 0.0;    // device time of the attempt kernels (events)
 };
 
+// The reference's entry point (src/pdhg/pdhg.hpp:9).  Besides the
+// reference's fields it fills row_activity = A x as the reference simplex
+// does (io::row_activity); solve_ex leaves that to its caller.
 Solution solve(const io::Model& model, const Options& options);
 
 // A prepared engine: construction uploads the model and runs the setup

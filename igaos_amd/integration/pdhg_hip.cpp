@@ -40,13 +40,18 @@ Solution solve(const io::Model& md, const Options& o) {
     // cfg NULL: the plain FP64 engine
     const int rc = igaos_pdhg_solve(h, &opt, nullptr, sol.x.data(), sol.y.data(), &info,
                                     msg.data(), static_cast<int>(msg.size()));
-    igaos_model_free(h);
     if (rc != 0) {
+        igaos_model_free(h);
         sol.status = Status::Error;
         sol.message = igaos_last_error();
         return sol;
     }
     sol.status = static_cast<Status>(info.status);  // same enumerator order
+    if (sol.status != Status::Error) {  // A x, as the reference simplex fills it
+        sol.row_activity.assign(md.m, 0.0);
+        igaos_model_row_activity(h, sol.x.data(), sol.row_activity.data());
+    }
+    igaos_model_free(h);
     sol.objective = info.objective;
     sol.pinf = info.pinf;
     sol.dinf = info.dinf;

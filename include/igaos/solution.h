@@ -2,7 +2,8 @@
 // Solve result, same fields as the reference's include/igaos/solution.h:10-22.
 // The pdhg engine fills status, message, objective, pinf, dinf, rel_gap,
 // iterations, solve_time_ms (setup excluded), x (n) and y (m, sign
-// convention d = c + A'y); row_activity stays empty, as in the reference.
+// convention d = c + A'y), and row_activity = A x (m) computed as the
+// reference simplex computes it (the reference's pdhg leaves it empty).
 
 #include <string>
 #include <vector>

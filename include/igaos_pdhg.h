@@ -61,6 +61,12 @@ igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
                                       const int* ci, const double* acx,
                                       double obj_const);
 void igaos_model_free(igaos_model* model);
+/* Host: row activity A x (m entries) of a point x (n entries), each row
+ * summed in FP64 from 0 in CSR order -- how the reference simplex fills
+ * Solution::row_activity (src/simplex/simplex.cpp:1727-1730).  The
+ * reference's pdhg leaves that field empty; igaos::pdhg::solve here fills
+ * it this way.  0 on success. */
+int igaos_model_row_activity(const igaos_model* model, const double* x, double* out);
 /* Sizes and MPS counters (any output may be NULL). */
 void igaos_model_dims(const igaos_model* model, int* m, int* n, int* nnz,
                       double* obj_const, int* n_fr_parsed,

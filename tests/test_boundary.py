@@ -400,3 +400,18 @@ def test_layout_builders_refuse_malformed_patterns():
     assert L.igaos_gather_order_build(2, 5, ip(ptr), ip(np.array([0, -1, 1], np.int32)),
                                       up(s16), up(s16)) < 0
     assert "outside" in L.igaos_batch_last_error().decode()
+
+
+def test_row_activity_is_csr_ordered_product():
+    # igaos_model_row_activity sums each row from 0 in CSR order, as the
+    # reference simplex fills Solution::row_activity; scipy's CSR product
+    # sums in the same order, so the two agree bit for bit
+    import ctypes as C
+    m = igaos_amd.read_mps(netlib_path("sc205"))
+    lp = m.to_lp()
+    x = np.random.default_rng(0).standard_normal(lp.n)
+    out = np.zeros(lp.m)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+    assert _core.lib().igaos_model_row_activity(m._h, dp(x), dp(out)) == 0
+    assert np.array_equal(out, lp.to_scipy() @ x)
+    assert _core.lib().igaos_model_row_activity(m._h, None, dp(out)) != 0

@@ -180,6 +180,10 @@ def test_netlib_solves_match_reference_highs(name, objectives):
     ref = objectives[f"netlib/{name}"]["highs_ref"]
     assert s.status == "near-optimal", s.message
     assert abs(s.objective - ref) <= 1e-6 * (1 + abs(ref)) + 5e-10 * abs(ref)
+    # row_activity is A x summed in CSR order, as the reference simplex
+    # fills it (scipy's CSR product sums in the same order)
+    lp = igaos_amd.read_mps(netlib_path(name)).to_lp()
+    assert np.array_equal(s.row_activity, lp.to_scipy() @ s.x)
 
 
 ENGINE_SMOKE = [
