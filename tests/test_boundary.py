@@ -415,3 +415,18 @@ def test_row_activity_is_csr_ordered_product():
     assert _core.lib().igaos_model_row_activity(m._h, dp(x), dp(out)) == 0
     assert np.array_equal(out, lp.to_scipy() @ x)
     assert _core.lib().igaos_model_row_activity(m._h, None, dp(out)) != 0
+
+
+def test_bench_reads_committed_traffic_summaries():
+    # bench.py's roofline "traffic" comes from the committed counter
+    # summaries; each must parse and give bytes near the algorithmic ones
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    t = bench.load_traffic("lockstep_batch8192", 1.0e9)
+    assert t is not None and 1.0e9 <= t <= 1.3e9
+    for dom in ("primal", "dual"):
+        t = bench.load_traffic(f"{dom}_step_zipf")
+        assert t is not None and 8.0e8 <= t <= 1.5e9
+    assert bench.load_traffic("no_such_kernel") is None
