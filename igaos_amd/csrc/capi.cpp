@@ -389,6 +389,19 @@ int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
     return 0;
 }
 
+int igaos_dual_repair(const igaos_model* model, const double* x, const double* Ax,
+                      const double* y_it, double col_tol, double* y) {
+    if (!model) return fail("null model");
+    const auto& md = model->m;
+    if ((md.n > 0 && !x) || (md.m > 0 && (!Ax || !y_it || !y)))
+        return fail("null array");
+    try {
+        return igaos::pdhg::host_dual_repair(md, x, Ax, y_it, col_tol, y) ? 1 : 0;
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+}
+
 int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap) {
     if (nrows < 0 || !ptr) return fail("invalid row pointer");
     if (ptr[0] != 0) return fail("pointer array does not start at 0");

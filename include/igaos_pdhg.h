@@ -315,6 +315,17 @@ int igaos_gather_order_build(int nrows, int ncols, const int* ptr, const int* id
 void igaos_batch_destroy(igaos_batch* batch);
 
 /* ---- kernel layer (device pointers, HIP stream or NULL) ----------------- */
+/* Host: the check-time dual repair of src/pdhg/pdhg.cu:288-448 that the
+ * engine runs when only the duality-gap certificate lags (exported so that
+ * host tests can pin it): from the point x (n) with activity Ax (m) and the
+ * iterate's dual y_it (m), POCS projections onto {d_J = 0 for columns
+ * strictly inside their bounds (tolerance col_tol)} and the sign cones,
+ * with a dense Cholesky of A_J A_J' + ridge.  y (m) receives the dual with
+ * the best finite gap (returns 1), or the least-violating projection when
+ * none has one (returns 0); -1 for invalid arguments. */
+int igaos_dual_repair(const igaos_model* model, const double* x, const double* Ax,
+                      const double* y_it, double col_tol, double* y);
+
 /* Host: row-block layout of a CSR row pointer (host) for the SpMV
  * kernels: pairs (first row, first nonzero) of each block followed by the
  * pair (nrows, nnz), i.e. 2 * (blocks + 1) ints.  A block holds at most

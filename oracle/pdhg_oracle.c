@@ -16,7 +16,8 @@
  * squares (the reference calls cuBLAS Dnrm2, whose order is not published).
  *
  * Pinning (what each function is checked against) is stated in
- * tests/test_oracle.py; a mutation switch (orc_set_mutation) plants
+ * tests/test_oracle.py (the dual repair: tests/test_boundary.py); a
+ * mutation switch (orc_set_mutation) plants
  * plausible mistakes so those tests can show the pins catch them.
  */
 #include "pdhg_oracle.h"
@@ -452,7 +453,8 @@ int orc_dual_repair(const orc_model* lp, const double* x, const double* Ax,
                     }
                     if (res2 > 0.0) {
                         chol_apply(G, m, rhs);
-                        for (int i = 0; i < m; ++i) yc[i] += rhs[i];
+                        const double sg = g_mut == 6 ? -1.0 : 1.0;
+                        for (int i = 0; i < m; ++i) yc[i] += sg * rhs[i];
                     }
                 }
                 for (int j = 0; j < n; ++j) {

@@ -430,3 +430,73 @@ def test_bench_reads_committed_traffic_summaries():
         t = bench.load_traffic(f"{dom}_step_zipf")
         assert t is not None and 8.0e8 <= t <= 1.5e9
     assert bench.load_traffic("no_such_kernel") is None
+
+
+@pytest.mark.parametrize("name", ["afiro", "sc50a", "kb2", "adlittle", "sc205"])
+def test_dual_repair_matches_oracle_and_weak_duality(name, objectives):
+    # the engine's host dual repair (pdhg.cu:288-448) against the oracle's
+    # restatement on the same point, bit for bit, and against the
+    # mathematics: the repaired dual is feasible to the repair's tolerance,
+    # its Lagrangian bound is finite, lies below the known optimum (weak
+    # duality) and close to it for a near-optimal x
+    import ctypes as C
+    import oracle
+    m = igaos_amd.read_mps(netlib_path(name))
+    lp = m.to_lp()
+    r = oracle.solve(lp, tolerance=1e-4, time_limit=60)
+    x, y = np.ascontiguousarray(r.x), np.ascontiguousarray(r.y)
+    A = lp.to_scipy()
+    Ax = A @ x
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+    yp = np.zeros(lp.m)
+    ok = _core.lib().igaos_dual_repair(m._h, dp(x), dp(Ax), dp(y), 1e-6, dp(yp))
+    ok_o, yo = oracle.dual_repair(lp, x, Ax, y, 1e-6)
+    assert ok == 1 and ok_o and np.array_equal(yp, yo)
+    mt = oracle.evaluate(lp.rmin, lp.rmax, lp.cl, lp.cu, lp.c, x, yp, Ax, lp.c + A.T @ yp)
+    ref = objectives[f"netlib/{name}"]["highs_ref"]
+    bound = mt.od + lp.obj_const
+    assert math.isfinite(mt.gap) and mt.dinf <= 1e-5
+    assert bound <= ref + 1e-7 * (1 + abs(ref))
+    assert ref - bound <= 1e-4 * (1 + abs(ref))
+
+
+def test_dual_repair_closed_forms():
+    # every column strictly inside its bounds: the repair must reach the
+    # unique y with c + A'y = 0 from a far (and, for the G row, cone-
+    # violating) iterate: y = -1 for min x1 + x2, x1 + x2 = 2 at x = (1, 1);
+    # y = -1.5 for min 3x, 2x >= 2 at x = 1 (sign cone y <= 0)
+    import ctypes as C
+    import oracle
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+    cases = [
+        (lpgen.LP.from_triplets(1, 2, [0, 0], [0, 1], [1.0, 1.0], np.ones(2), np.zeros(2),
+                                np.full(2, 10.0), np.array([2.0]), np.array([2.0])),
+         np.array([1.0, 1.0]), np.array([0.5]), -1.0),
+        (lpgen.LP.from_triplets(1, 1, [0], [0], [2.0], np.array([3.0]), np.zeros(1),
+                                np.full(1, INF), np.array([2.0]), np.array([INF])),
+         np.array([1.0]), np.array([0.7]), -1.5),
+    ]
+    for lp, x, y_it, want in cases:
+        m = igaos_amd.api.model_from_lp(lp)
+        Ax = lp.to_scipy() @ x
+        y = np.zeros(lp.m)
+        assert _core.lib().igaos_dual_repair(m._h, dp(x), dp(Ax), dp(y_it), 1e-6, dp(y)) == 1
+        assert abs(y[0] - want) <= 1e-12
+        ok, yo = oracle.dual_repair(lp, x, Ax, y_it, 1e-6)
+        assert ok and abs(yo[0] - want) <= 1e-12
+
+
+def test_dual_repair_closed_form_catches_a_flipped_projection():
+    # the closed form fails when the oracle's affine projection step is
+    # applied with the wrong sign (mutation 6)
+    import oracle
+    lp = lpgen.LP.from_triplets(1, 2, [0, 0], [0, 1], [1.0, 1.0], np.ones(2), np.zeros(2),
+                                np.full(2, 10.0), np.array([2.0]), np.array([2.0]))
+    x = np.array([1.0, 1.0])
+    Ax = lp.to_scipy() @ x
+    oracle.set_mutation(6)
+    try:
+        ok, y = oracle.dual_repair(lp, x, Ax, np.array([0.5]), 1e-6)
+    finally:
+        oracle.set_mutation(0)
+    assert not (ok and abs(y[0] + 1.0) <= 1e-12)
