@@ -62,6 +62,7 @@ SIGNATURES = {
     "igaos_model_free": (None, [_vp]),
     "igaos_model_row_activity": (C.c_int, [_vp, _dp, _dp]),
     "igaos_dual_repair": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, _dp]),
+    "igaos_kkt_evaluate": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "igaos_model_dims": (None, [_vp, _ip, _ip, _ip, _dp, _ip, _ip, _ip]),
     "igaos_model_get": (None, [_vp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp,
                                _ip, _ip, _dp]),

@@ -389,6 +389,21 @@ int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
     return 0;
 }
 
+int igaos_kkt_evaluate(const igaos_model* model, const double* x, const double* y,
+                       const double* Ax, const double* d, double* out) {
+    if (!model || !out) return fail("null model or output");
+    const auto& md = model->m;
+    if ((md.n > 0 && (!x || !d)) || (md.m > 0 && (!y || !Ax))) return fail("null array");
+    const igaos::pdhg::HostMetrics t = igaos::pdhg::host_evaluate(md, x, y, Ax, d);
+    out[0] = t.pinf;
+    out[1] = t.dinf;
+    out[2] = t.gap;
+    out[3] = t.op;
+    out[4] = t.od;
+    out[5] = t.prow;
+    return 0;
+}
+
 int igaos_dual_repair(const igaos_model* model, const double* x, const double* Ax,
                       const double* y_it, double col_tol, double* y) {
     if (!model) return fail("null model");

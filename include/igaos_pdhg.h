@@ -315,6 +315,14 @@ int igaos_gather_order_build(int nrows, int ncols, const int* ptr, const int* id
 void igaos_batch_destroy(igaos_batch* batch);
 
 /* ---- kernel layer (device pointers, HIP stream or NULL) ----------------- */
+/* Host: the KKT evaluation of src/pdhg/pdhg.cu:173-254 in original units
+ * that the dual repair runs (exported so that host tests can pin it): for
+ * x (n), y (m), Ax (m) and d = c + A'y (n), out receives pinf, dinf, gap,
+ * primal objective, dual objective (without obj_const) and the largest
+ * relative row violation.  0 on success. */
+int igaos_kkt_evaluate(const igaos_model* model, const double* x, const double* y,
+                       const double* Ax, const double* d, double* out);
+
 /* Host: the check-time dual repair of src/pdhg/pdhg.cu:288-448 that the
  * engine runs when only the duality-gap certificate lags (exported so that
  * host tests can pin it): from the point x (n) with activity Ax (m) and the

@@ -500,3 +500,29 @@ def test_dual_repair_closed_form_catches_a_flipped_projection():
     finally:
         oracle.set_mutation(0)
     assert not (ok and abs(y[0] + 1.0) <= 1e-12)
+
+
+@pytest.mark.parametrize("name", ["afiro", "kb2", "sc205", "israel"])
+def test_host_kkt_evaluate_matches_oracle(name):
+    # the host evaluation the repair uses (pdhg.cu:173-254) equals the
+    # oracle's, bit for bit, at a near-optimal pair and at a perturbed one
+    # (whose Lagrangian bound is -inf and gap inf)
+    import ctypes as C
+    import oracle
+    m = igaos_amd.read_mps(netlib_path(name))
+    lp = m.to_lp()
+    r = oracle.solve(lp, tolerance=1e-4, time_limit=60)
+    A = lp.to_scipy()
+    rng = np.random.default_rng(0)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa
+    pairs = ((r.x, r.y), (r.x + 0.01 * rng.standard_normal(lp.n),
+                          r.y + 0.01 * rng.standard_normal(lp.m)))
+    for k, (x, y) in enumerate(pairs):
+        x, y = np.ascontiguousarray(x), np.ascontiguousarray(y)
+        Ax, d = A @ x, lp.c + A.T @ y
+        out = np.zeros(6)
+        assert _core.lib().igaos_kkt_evaluate(m._h, dp(x), dp(y), dp(Ax), dp(d), dp(out)) == 0
+        t = oracle.evaluate(lp.rmin, lp.rmax, lp.cl, lp.cu, lp.c, x, y, Ax, d)
+        assert np.array_equal(out, [t.pinf, t.dinf, t.gap, t.op, t.od, t.prow])
+        if k == 1:
+            assert out[2] == INF and out[4] == -INF
