@@ -930,6 +930,8 @@ std::unique_ptr<EngineBase> make_engine(const io::Model& model,
     if (model.m < 0 || model.n < 0 || static_cast<int>(model.ap.size()) != model.m + 1 ||
         static_cast<int>(model.cp.size()) != model.n + 1)
         throw std::invalid_argument("malformed model: CSR/CSC pointer sizes do not match m, n");
+    // the primal step's launch covers the rows of A' (one per column)
+    if (model.n == 0) throw std::invalid_argument("the LP has no columns (n = 0)");
     if (cfg.precision == 32) return std::make_unique<Engine<float>>(model, cfg);
     return std::make_unique<Engine<double>>(model, cfg);
 }
