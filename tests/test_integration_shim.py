@@ -43,3 +43,48 @@ def test_reference_side_shim_builds_and_solves(tmp_path, objectives):
         assert abs(float(obj) - ref) <= 1e-6 * (1 + abs(ref))
     else:
         assert status == "error" and "CUDA" in out.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "src/io/model.hpp")),
+                    reason="reference sources not present")
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
+def test_reference_side_batch_binding_builds_and_runs(tmp_path, objectives):
+    # igaos_amd/integration/batch_bindings.cpp: the pybind11 binding of the
+    # batched engine over the reference's io::Model, compiled against the
+    # reference's headers and loaded as an extension module
+    import json
+    import sys
+    import sysconfig
+    pybind11 = pytest.importorskip("pybind11")
+    lib = os.path.join(ROOT, "igaos_amd")
+    ext = str(tmp_path / ("igaos_batch_ext" + sysconfig.get_config_var("EXT_SUFFIX")))
+    cmd = ["g++", "-std=c++20", "-O1", "-shared", "-fPIC", "-DIGAOS_BATCH_STANDALONE",
+           "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+           "-I", os.path.join(REF, "src/io"), "-I", os.path.join(REF, "include"),
+           "-I", os.path.join(ROOT, "include"),
+           os.path.join(lib, "integration", "batch_bindings.cpp"),
+           "-L", lib, "-ligaos_pdhg", f"-Wl,-rpath,{lib}", "-o", ext]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    script = (
+        "import json, sys\n"
+        f"sys.path.insert(0, {str(tmp_path)!r})\n"
+        "import igaos_batch_ext as e\n"
+        f"ms = [e.read_mps({netlib_path('afiro')!r}), e.read_mps({netlib_path('sc50a')!r})]\n"
+        "try:\n"
+        "    out = e.solve_batch(ms, tolerance=1e-6, time_limit=60.0)\n"
+        "    print(json.dumps([[d['status'], d['objective'], len(d['x'])] for d in out]))\n"
+        "except RuntimeError as ex:\n"
+        "    print(json.dumps({'error': str(ex)}))\n")
+    out = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    import igaos_amd
+    if igaos_amd.has_pdhg:
+        for (status, obj, nx), name, n in zip(res, ("afiro", "sc50a"), (32, 48)):
+            ref = objectives[f"netlib/{name}"]["highs_ref"]
+            assert status == "near-optimal" and nx == n
+            assert abs(obj - ref) <= 2e-6 * (1 + abs(ref))
+    else:
+        assert "no HIP GPU device" in res["error"]
