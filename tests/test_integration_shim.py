@@ -4,7 +4,9 @@ headers (include/igaos/*.h, src/io/model.hpp, src/pdhg/pdhg.hpp), links with
 libigaos_pdhg.so and solves an MPS file parsed by the reference's
 io::parse_mps.  Needs the reference sources (skipped where they are
 absent) and g++; without a GPU the engine reports status "error" with the
-reference's no-GPU wording ("CUDA"), as src/api/engine.hpp:37 does."""
+reference's no-GPU wording ("CUDA"), as src/api/engine.hpp:37 does.
+The pybind11 binding of the batched engine (batch_bindings.cpp) is built
+the same way and loaded as an extension module."""
 import os
 import shutil
 import subprocess
