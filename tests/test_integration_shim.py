@@ -90,3 +90,42 @@ def test_reference_side_batch_binding_builds_and_runs(tmp_path, objectives):
             assert abs(obj - ref) <= 2e-6 * (1 + abs(ref))
     else:
         assert "no HIP GPU device" in res["error"]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "src/api/igaos_cli.cpp")),
+                    reason="reference sources not present")
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
+def test_reference_cli_with_the_shim(tmp_path, objectives):
+    # the reference's own CLI (src/api/igaos_cli.cpp and the engines it
+    # links, compiled from their sources with IGAOS_HAS_PDHG) with the shim
+    # in place of pdhg.cu: `igaos solve --engine pdhg` runs this library, and
+    # its JSON carries the same fields as igaos_amd/igaos (which adds y)
+    import json
+    lib = os.path.join(ROOT, "igaos_amd")
+    exe = str(tmp_path / "igaos_ref")
+    inc = []
+    for d in ("include", "src/io", "src/simplex", "src/milp", "src/qp", "src/pdhg", "src/api"):
+        inc += ["-I", os.path.join(REF, d)]
+    cmd = (["g++", "-std=c++20", "-O1", "-w", "-DIGAOS_HAS_PDHG"] + inc +
+           ["-I", os.path.join(ROOT, "include")] +
+           [os.path.join(REF, s) for s in ("src/api/igaos_cli.cpp", "src/simplex/simplex.cpp",
+                                           "src/milp/milp.cpp", "src/qp/qp.cpp")] +
+           [os.path.join(lib, "integration", "pdhg_hip.cpp"),
+            "-L", lib, "-ligaos_pdhg", f"-Wl,-rpath,{lib}", "-o", exe])
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    args = ["solve", netlib_path("afiro"), "--engine", "pdhg", "--tol", "1e-6"]
+    ref_out = subprocess.run([exe] + args, capture_output=True, text=True, timeout=120)
+    ours = subprocess.run([os.path.join(lib, "igaos")] + args, capture_output=True,
+                          text=True, timeout=120)
+    a, b = json.loads(ref_out.stdout), json.loads(ours.stdout)
+    assert set(b) - set(a) == {"y"} and set(a) <= set(b)
+    assert a["engine"] == b["engine"] == "pdhg" and a["status"] == b["status"]
+    import igaos_amd
+    if igaos_amd.has_pdhg:
+        ref = objectives["netlib/afiro"]["highs_ref"]
+        assert a["status"] == "near-optimal"
+        assert abs(a["objective"] - ref) <= 1e-6 * (1 + abs(ref))
+        assert a["objective"] == b["objective"] and a["x"] == b["x"]
+    else:
+        assert a["status"] == "error" and "CUDA" in a["message"]
