@@ -50,6 +50,9 @@ Solution solve(const io::Model& md, const Options& o) {
     if (sol.status != Status::Error) {  // A x, as the reference simplex fills it
         sol.row_activity.assign(md.m, 0.0);
         igaos_model_row_activity(h, sol.x.data(), sol.row_activity.data());
+    } else {  // no point to report, as in a build without the engine
+        sol.x.clear();
+        sol.y.clear();
     }
     igaos_model_free(h);
     sol.objective = info.objective;
