@@ -4,8 +4,11 @@ No compute is launched here: these tests check that libigaos_pdhg.so
 loads and exports every entry point include/igaos_pdhg.h declares, that
 the model reader keeps the reference's MPS semantics (the edge cases of
 tests/engine_smoke.cpp in the reference), that the Python and CLI
-surfaces keep the reference's names, defaults and error behaviour, and
-the row-block layout invariants the SpMV kernels rely on.
+surfaces keep the reference's names, defaults and error behaviour, that
+malformed caller patterns are refused before device work, the layout
+invariants the SpMV kernels rely on, and the host-side pieces of the check
+(KKT evaluation, dual repair, row activity) against the oracle and closed
+forms.
 """
 import math
 import os
