@@ -529,3 +529,28 @@ def test_host_kkt_evaluate_matches_oracle(name):
         assert np.array_equal(out, [t.pinf, t.dinf, t.gap, t.op, t.od, t.prow])
         if k == 1:
             assert out[2] == INF and out[4] == -INF
+
+
+def test_autotuner_classes_and_cached_schedule(tmp_path):
+    # host logic of the autotuner: quantile size classes cover every
+    # instance, each instance maps to its class's lanes per row, and a
+    # cached schedule is returned without probing the device
+    import json
+    from dataclasses import asdict
+    from igaos_amd import autotune
+    lps = [lpgen.gen_batch_instance(0, i, m_hi=80) for i in range(40)]
+    nnz = np.array([lp.nnz for lp in lps])
+    edges = autotune._classes(nnz, 4)
+    assert edges[0] == nnz.min() and edges[-1] == nnz.max() + 1
+    assert np.all(np.diff(edges) > 0)
+    k = len(edges) - 1
+    s = autotune.Schedule(edges=[float(e) for e in edges], tpr=[0, 1, 2, 4][:k], waves=4,
+                          probe_ms={}, mode="lockstep",
+                          mode_ms={"persistent": 2.0, "lockstep": 1.0})
+    cls = np.searchsorted(edges, nnz, side="right") - 1
+    assert np.all((cls >= 0) & (cls < k))
+    assert np.array_equal(s.tpr_for(nnz), np.array(s.tpr, np.int32)[cls])
+    key = ",".join(str(int(e)) for e in edges) + ";modes"
+    cache = tmp_path / "schedule.json"
+    cache.write_text(json.dumps({key: asdict(s)}))
+    assert autotune.tune(lps, cache=str(cache), modes=True) == s
