@@ -554,3 +554,15 @@ def test_autotuner_classes_and_cached_schedule(tmp_path):
     cache = tmp_path / "schedule.json"
     cache.write_text(json.dumps({key: asdict(s)}))
     assert autotune.tune(lps, cache=str(cache), modes=True) == s
+
+
+def test_batch_solution_sizes_are_checked_on_the_host():
+    # set_solution refuses arrays whose length is not the batch's sum of n
+    # (or m) before the C-ABI reads them
+    from igaos_amd.batch import BatchSolver
+    b = BatchSolver.__new__(BatchSolver)
+    b._h, b._coff, b._roff = None, np.array([0, 3, 5]), np.array([0, 2, 4])
+    with pytest.raises(ValueError, match="5 entries"):
+        b.set_solution(x=np.zeros(4))
+    with pytest.raises(ValueError, match="4 entries"):
+        b.set_solution(y=[np.zeros(2), np.zeros(1)])
