@@ -120,6 +120,12 @@ def test_reference_cli_with_the_shim(tmp_path, objectives):
                           text=True, timeout=120)
     a, b = json.loads(ref_out.stdout), json.loads(ours.stdout)
     assert set(b) - set(a) == {"y"} and set(a) <= set(b)
+    # `igaos info` prints the same text as the reference's CLI
+    for name in ("afiro", "sc50a", "israel", "scfxm1"):
+        info = [subprocess.run([e, "info", netlib_path(name)], capture_output=True,
+                               text=True, timeout=60).stdout
+                for e in (exe, os.path.join(lib, "igaos"))]
+        assert info[0] and info[0] == info[1], name
     assert a["engine"] == b["engine"] == "pdhg" and a["status"] == b["status"]
     import igaos_amd
     if igaos_amd.has_pdhg:
