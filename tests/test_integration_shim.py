@@ -135,6 +135,8 @@ def test_reference_cli_with_the_shim(tmp_path, objectives):
         assert a["objective"] == b["objective"] and a["x"] == b["x"]
     else:
         assert a["status"] == "error" and "CUDA" in a["message"]
+        # no point and the same fields, values and message as ours
+        assert a["x"] == [] and {k: v for k, v in b.items() if k != "y"} == a
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(REF, "python/igaos_bindings.cpp")),
