@@ -6,7 +6,9 @@ io::parse_mps.  Needs the reference sources (skipped where they are
 absent) and g++; without a GPU the engine reports status "error" with the
 reference's no-GPU wording ("CUDA"), as src/api/engine.hpp:37 does.
 The pybind11 binding of the batched engine (batch_bindings.cpp) is built
-the same way and loaded as an extension module."""
+the same way and loaded as an extension module, and the reference's own
+CLI and pybind11 module are compiled from their sources with the shim in
+place of pdhg.cu and compared with this repository's CLI and API."""
 import os
 import shutil
 import subprocess
