@@ -5,13 +5,17 @@ m log-uniform in [32, 4096], n = round(m * U[1.5, 3]), 4-8 nonzeros per
 column with values N(0,1), rhs from a feasible x0 ~ U[0,1], box bounds
 [0, 10], c ~ N(0,1) (igaos_amd.lpgen.gen_batch).  A step is one batched
 solve of every instance to relative KKT 1e-6 with FP32 iterations and FP64
-refinement (per-instance budget --time-limit seconds), in the execution
-mode the autotuner measures as faster on a probe of the batch (lockstep
-rounds or one persistent launch; --mode fixes it); ``value`` is instances
-certified near-optimal per wall second of the step, with the batch already
-resident in HBM.  Beside it: the plain FP64 single-LP
-engine run instance by instance on a fixed sample (every 512th instance),
-timed with its own HBM-resident setup excluded, extrapolated to LPs/s.
+refinement (per-instance budget --time-limit seconds, 60 by default), in
+the execution mode the autotuner measures as faster on a probe of the
+batch (lockstep rounds or one persistent launch; --mode fixes it);
+``value`` is instances certified near-optimal per wall second of the timed
+steps, with the batch already resident in HBM.  A step lasts tens of
+seconds, so warm-up and timed steps run only while the whole run stays
+within --budget-s seconds of wall clock (at least one step is timed):
+``steps`` / ``warmup`` are the counts actually run, next to the requested
+ones.  Beside it: the plain FP64 single-LP engine run instance by instance
+on a fixed sample (every 512th instance, within --fp64-budget-s), timed
+with its own HBM-resident setup excluded, extrapolated to LPs/s.
 
 ``roofline`` (bound: HBM) uses the algorithmic byte model per accepted
 iteration of an instance, B = 2 nnz (w_a + 4) + 4 (m + n + 2) +
@@ -31,7 +35,7 @@ FP32 solve (the LP needs well over 10^4 iterations to reach 1e-6).
 
 ``cpu_baseline`` is the oracle (a single-threaded C host replay of the
 reference engine) on a bounded sample of the same workload, named in
-``sample``.  Beside it, ``cpu_baseline_all_cores`` runs the same replay on
+``sample`` (the batch's three CPU legs share --cpu-budget-s).  Beside it, ``cpu_baseline_all_cores`` runs the same replay on
 up to 16 host threads (one instance per thread for the batch, the rows of
 each sparse product split among the threads for the Zipf LP), and
 ``cpu_baseline_reference_simplex`` times the reference's own CPU path for
@@ -45,6 +49,8 @@ import json
 import os
 import sys
 import time
+
+T_START = time.perf_counter()
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -62,16 +68,19 @@ def bytes_per_iteration(m, n, nnz, w_a, w):
     return 2 * nnz * (w_a + 4) + 4 * (m + n + 2) + w * (7 * n + 8 * m)
 
 
-def load_traffic(name, alg_bytes=None):
+def load_traffic(name, alg_bytes=None, kernel=None):
     """HBM bytes per launch from the committed rocprofv3 counter summary:
     measured directly for a fixed launch, or, where the summary holds the
     ratio of measured to algorithmic bytes on full launches, that ratio
-    times this launch's algorithmic bytes."""
+    times this launch's algorithmic bytes.  ``kernel`` picks a kernel's
+    entry of a summary that holds several."""
     path = os.path.join(ROOT, "profiles", f"{name}_pmc.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
+    if kernel is not None and isinstance(d.get(kernel), dict):
+        d = d[kernel]
     if "hbm_bytes_per_launch" in d:
         return d["hbm_bytes_per_launch"]
     if alg_bytes is not None and "traffic_over_algorithmic" in d:
@@ -81,6 +90,9 @@ def load_traffic(name, alg_bytes=None):
 
 # ---------------------------------------------------------------- batch --
 def bench_batch(args):
+    """The 8,192-LP batch.  A step is one full batched solve; steps are
+    timed while the wall-clock budget allows (see ``--budget-s``), so the
+    line reports how many it timed next to how many were asked for."""
     import numpy as np
     import igaos_amd
     from igaos_amd import lpgen
@@ -119,118 +131,109 @@ def bench_batch(args):
         log(f"[bench]   {done}/{total} instances done at "
             f"{time.perf_counter() - t0[0]:.1f} s")
 
-    for w in range(args.warmup):
+    def one_solve():
         t0[0] = time.perf_counter()
         s = b.run(tolerance=args.tolerance, time_limit=args.time_limit,
                   want_xy=False, progress=prog, poll_s=10.0)
-        log(f"[bench] warmup {w}: {s.solved}/{count} solved")
+        return s, time.perf_counter() - t0[0]
+
+    # the time the legs after the timed steps may take (each is bounded)
+    reserve = args.fp64_budget_s + (0.0 if args.no_cpu_baseline else args.cpu_budget_s) + 15.0
+    # a step cannot outlast the per-instance budget by much: every instance
+    # starts with the launch (lockstep) and stops at its time limit
+    est = args.time_limit + 5.0
+
+    def room():
+        return args.budget_s - (time.perf_counter() - T_START) - reserve
+
+    # timed steps come first: a warm-up solve runs only where the budget
+    # holds every requested timed step as well (the autotuner's probe runs
+    # have already run every kernel at the batch's shapes)
+    fit = int(room() // est)
+    warm_done = 0
+    for w in range(min(args.warmup, max(0, fit - args.steps))):
+        s, el = one_solve()
+        warm_done += 1
+        log(f"[bench] warmup {w}: {s.solved}/{count} solved in {el:.1f} s")
     solved = 0
     elapsed = 0.0
     kernel_ms = 0.0
     sol = None
+    step_s = []
+    lk = None
     for k in range(args.steps):
-        t0[0] = time.perf_counter()
-        sol = b.run(tolerance=args.tolerance, time_limit=args.time_limit,
-                    want_xy=False, progress=prog, poll_s=10.0)
-        elapsed += time.perf_counter() - t0[0]
+        if k > 0 and room() < est:
+            break
+        sol, el = one_solve()
+        step_s.append(el)
+        elapsed += el
         kernel_ms += sol.kernel_ms
         solved += sol.solved
-        log(f"[bench] step {k}: {sol.solved}/{count} solved, launch "
+        est = max(step_s) + 2.0
+        log(f"[bench] step {k}: {sol.solved}/{count} solved in {el:.1f} s, launch "
             f"{sol.kernel_ms:.0f} ms")
-    lk = b.lockstep_timing() if mode == "lockstep" else None
+        if mode == "lockstep" and lk is None:
+            lk = b.lockstep_timing()
+    steps = len(step_s)
     b.close()
     value = solved / elapsed
     its = np.array(sol.iterations)
     att = np.array(sol.attempts)
     ok32 = np.array([st == "near-optimal" for st in sol.status])
+    tms = np.array(sol.solve_ms)
     alg = float((att * bytes_per_iteration(ms, ns, nnz, 4, 4)).sum())
-    launch_ms = kernel_ms / args.steps
+    launch_ms = kernel_ms / steps
     achieved_run = alg / (launch_ms * 1e-3) / 1e9
+    dom = None
     if lk is not None and lk["samples"] > 0:
-        # dominant kernel: the dual attempt kernel of the sampled rounds
-        achieved = lk["dual_bytes"] / (lk["dual_ms"] * 1e-3) / 1e9
+        # dominant kernel: the attempt kernel with the larger time per round
+        dom = "primal" if lk["primal_ms"] >= lk["dual_ms"] else "dual"
+        achieved = lk[f"{dom}_bytes"] / (lk[f"{dom}_ms"] * 1e-3) / 1e9
     else:
         achieved = achieved_run
 
-    # plain FP64 single-LP engine, instance by instance, on a fixed sample
+    # plain FP64 single-LP engine, instance by instance, on a fixed sample,
+    # within its own budget (each instance gets what is left, at most the
+    # per-instance limit)
     sample = list(range(0, count, max(1, count // args.fp64_sample)))
     t64 = 0.0
     solved64 = 0
+    run64 = []
     rel32 = []  # batched FP32+refine objective vs the plain FP64 engine's
+    tb = time.perf_counter()
     for i in sample:
+        left = args.fp64_budget_s - (time.perf_counter() - tb)
+        if left < 1.0:
+            break
         e = igaos_amd.Engine(lps[i], precision=64)
-        s = e.run(time_limit=args.time_limit, tolerance=args.tolerance)
+        s = e.run(time_limit=min(args.time_limit, left), tolerance=args.tolerance)
         e.close()
+        run64.append(i)
         t64 += s.solve_time_ms * 1e-3
         solved64 += s.status == "near-optimal"
         if s.status == "near-optimal" and sol.status[i] == "near-optimal":
             rel32.append(abs(sol.objective[i] - s.objective) / (1 + abs(s.objective)))
     fp64_rate = solved64 / t64 if t64 > 0 else 0.0
-    log(f"[bench] plain FP64 per instance: {solved64}/{len(sample)} solved "
+    log(f"[bench] plain FP64 per instance: {solved64}/{len(run64)} solved "
         f"in {t64:.1f} s -> {fp64_rate:.2f} LPs/s")
 
-    cpu = None
+    cpu = cpu_par = cpu_ref = None
     if not args.no_cpu_baseline:
-        import oracle
-        cs = sample[: args.cpu_sample]
-        tc = 0.0
-        sc = 0
-        wall = time.perf_counter()
-        for i in cs:
-            r = oracle.solve(lps[i], tolerance=args.tolerance,
-                             time_limit=args.cpu_time_limit)
-            tc += r.solve_time_ms * 1e-3
-            sc += r.status == "near-optimal"
-        wall = time.perf_counter() - wall
-        cpu = {"value": sc / tc if tc > 0 else 0.0, "unit": "LPs/s",
-               "cores": 1, "kind": "port",
-               "sample": (f"oracle host replay (1 thread) on instances "
-                          f"{cs} of the batch, {sc}/{len(cs)} certified in "
-                          f"{tc:.1f} s of solve time ({wall:.1f} s wall incl. "
-                          f"setup), budget {args.cpu_time_limit} s each")}
-    cpu_par = None
-    if not args.no_cpu_baseline:
-        # the same host replay on all the host cores this job may use, one
-        # instance per thread (ctypes releases the GIL during each solve)
-        import oracle
-        from concurrent.futures import ThreadPoolExecutor
-        cores = max(1, min(16, os.cpu_count() or 1))
-        step = max(1, count // (4 * cores))
-        cs = list(range(0, count, step))[:4 * cores]
-        wall = time.perf_counter()
-        with ThreadPoolExecutor(cores) as ex:
-            res = list(ex.map(lambda i: oracle.solve(lps[i], tolerance=args.tolerance,
-                                                     time_limit=args.cpu_time_limit), cs))
-        wall = time.perf_counter() - wall
-        sc = sum(r.status == "near-optimal" for r in res)
-        cpu_par = {"value": sc / wall, "unit": "LPs/s", "cores": cores, "kind": "port",
-                   "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
-                   "sample": (f"oracle host replay, {cores} threads, one instance each: "
-                              f"{len(cs)} instances (every {step}th), {sc} certified in "
-                              f"{wall:.1f} s wall, budget {args.cpu_time_limit} s each")}
-    cpu_ref = None
-    if not args.no_cpu_baseline:
-        import oracle
-        if oracle.have_ref_simplex():
-            # the reference's own CPU path for these LPs: its revised
-            # simplex (built from its sources), same instances, 5 s each
-            cs = sample[: args.cpu_sample]
-            wall = time.perf_counter()
-            sc = 0
-            for i in cs:
-                st = oracle.ref_simplex(lps[i], time_limit=5.0)[0]
-                sc += st == "optimal"
-            wall = time.perf_counter() - wall
-            cpu_ref = {"value": sc / wall, "unit": "LPs/s", "cores": 1,
-                       "kind": "reference",
-                       "sample": (f"reference revised simplex (src/simplex, 1 thread) "
-                                  f"on instances {cs}, {sc}/{len(cs)} optimal in "
-                                  f"{wall:.1f} s wall, budget 5 s each")}
+        cpu, cpu_par, cpu_ref = batch_cpu_legs(args, lps, sample)
 
+    unsolved = [i for i in range(count) if not ok32[i]]
     return {
         "metric": METRIC, "value": value, "unit": "LPs/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": elapsed * 1000.0 / args.steps,
+        "steps": steps, "warmup": warm_done,
+        "steps_requested": args.steps, "warmup_requested": args.warmup,
+        "ms_per_step": elapsed * 1000.0 / steps,
+        "step_seconds": [round(x, 2) for x in step_s],
+        "budget_s": args.budget_s,
+        "budget_note": (f"a step is one full solve of the batch (per-instance limit "
+                        f"{args.time_limit:g} s, fixed); steps are timed while the "
+                        f"run stays within {args.budget_s:g} s of wall clock, so "
+                        f"'steps' is the number timed; the autotuner's probe runs "
+                        f"(every kernel at the batch's shapes) warm the GPU up"),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64refine", "data": "synthetic",
         "config": {"workload": (f"batch of {count} random LPs, m log-uniform "
@@ -248,8 +251,14 @@ def bench_batch(args):
                        "waves_per_simd": sched.waves,
                        "mode_probe_ms": sched.mode_ms,
                        "tuning_s": round(tune_s, 1)}},
-        "solved": solved, "instances": count * args.steps,
-        "iterations_per_s": float(its.sum()) / (elapsed / args.steps),
+        "solved": solved, "instances": count * steps,
+        "last_step": {"certified": int(ok32.sum()), "instances": count,
+                      "makespan_s": round(step_s[-1], 2),
+                      "certified_by_s": {str(q): int((tms[ok32] <= q * 1e3).sum())
+                                         for q in (10, 20, 30, 45, 60)},
+                      "uncertified": unsolved[:64],
+                      "uncertified_iterations": [int(its[i]) for i in unsolved[:64]]},
+        "iterations_per_s": float(its.sum()) / (elapsed / steps),
         "end_to_end": {
             "value": solved / (t_create + elapsed), "unit": "LPs/s",
             "setup_s": round(t_create, 2),
@@ -260,8 +269,9 @@ def bench_batch(args):
         "iterations_p99": float(np.percentile(its, 99)),
         "launch_ms": launch_ms,
         "plain_fp64_per_instance": {
-            "value": fp64_rate, "unit": "LPs/s", "sample": sample,
-            "solved": solved64, "solve_seconds": round(t64, 3)},
+            "value": fp64_rate, "unit": "LPs/s", "sample": run64,
+            "solved": solved64, "solve_seconds": round(t64, 3),
+            "budget_s": args.fp64_budget_s},
         "fp32_refined_error": {
             "objective_vs_fp64_engine_rel_max": max(rel32) if rel32 else None,
             "compared": len(rel32),
@@ -273,21 +283,92 @@ def bench_batch(args):
         "batched_vs_plain_fp64": (value / fp64_rate) if fp64_rate > 0 else None,
         "roofline": {"bound": "hbm",
                      "kernel": ("k_batch<float> (one persistent launch)"
-                                if mode == "persistent" else
-                                "k_lk_dual<float> (HIP events around it in sampled "
-                                "lockstep rounds)"),
+                                if dom is None else
+                                f"k_lk_{dom}<float> (HIP events around it in sampled "
+                                f"lockstep rounds of the first timed step)"),
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": (None if lk is None else
-                                 load_traffic("lockstep_batch8192", lk["dual_bytes"])),
-                     "bytes_per_launch": alg if lk is None else lk["dual_bytes"],
-                     "ms_per_launch": round(launch_ms if lk is None else lk["dual_ms"], 4),
-                     "lockstep_samples": None if lk is None else lk,
+                     "traffic": (None if dom is None else
+                                 load_traffic("lockstep_batch8192", lk[f"{dom}_bytes"],
+                                              kernel=dom)),
+                     "bytes_per_launch": alg if dom is None else lk[f"{dom}_bytes"],
+                     "ms_per_launch": round(launch_ms if dom is None else lk[f"{dom}_ms"], 4),
+                     "lockstep_samples": lk,
                      "whole_run_gbs": round(achieved_run, 1)},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_par,
         "cpu_baseline_reference_simplex": cpu_ref,
     }
+
+
+def batch_cpu_legs(args, lps, sample):
+    """CPU baselines of the batch, each on a bounded sample: the oracle on
+    one thread, the oracle on up to 16 threads (one instance each), and the
+    reference's revised simplex (oracle/_ref) when it was built."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    count = len(lps)
+    # 1 thread: instances of the sample in order while the leg's share of
+    # the budget lasts
+    share = args.cpu_budget_s / 3.0
+    tc = 0.0
+    sc = 0
+    done = []
+    wall = time.perf_counter()
+    for i in sample:
+        left = share - (time.perf_counter() - wall)
+        if left < 1.0:
+            break
+        r = oracle.solve(lps[i], tolerance=args.tolerance,
+                         time_limit=min(args.cpu_time_limit, left))
+        tc += r.solve_time_ms * 1e-3
+        sc += r.status == "near-optimal"
+        done.append(i)
+    wall = time.perf_counter() - wall
+    cpu = {"value": sc / tc if tc > 0 else 0.0, "unit": "LPs/s",
+           "cores": 1, "kind": "port",
+           "sample": (f"oracle host replay (1 thread) on instances {done} of the "
+                      f"batch, {sc}/{len(done)} certified in {tc:.1f} s of solve time "
+                      f"({wall:.1f} s wall incl. setup), at most "
+                      f"{args.cpu_time_limit:g} s each")}
+    log(f"[bench] oracle 1 thread: {cpu['value']:.3f} LPs/s")
+    # all cores: one instance per thread, one instance each
+    cores = max(1, min(16, os.cpu_count() or 1))
+    step = max(1, count // cores)
+    cs = list(range(0, count, step))[:cores]
+    tl = min(args.cpu_time_limit, share)
+    wall = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        res = list(ex.map(lambda i: oracle.solve(lps[i], tolerance=args.tolerance,
+                                                 time_limit=tl), cs))
+    wall = time.perf_counter() - wall
+    sc = sum(r.status == "near-optimal" for r in res)
+    cpu_par = {"value": sc / wall, "unit": "LPs/s", "cores": cores, "kind": "port",
+               "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+               "sample": (f"oracle host replay, {cores} threads, one instance each: "
+                          f"instances {cs}, {sc} certified in {wall:.1f} s wall, "
+                          f"at most {tl:g} s each")}
+    log(f"[bench] oracle {cores} threads: {cpu_par['value']:.3f} LPs/s")
+    cpu_ref = None
+    if oracle.have_ref_simplex():
+        # the reference's own CPU path for these LPs: its revised simplex
+        # (built from its sources), 5 s each, while the share lasts
+        wall = time.perf_counter()
+        sc = 0
+        done = []
+        for i in sample:
+            if share - (time.perf_counter() - wall) < 5.0:
+                break
+            st = oracle.ref_simplex(lps[i], time_limit=5.0)[0]
+            sc += st == "optimal"
+            done.append(i)
+        wall = time.perf_counter() - wall
+        cpu_ref = {"value": sc / wall if wall > 0 else 0.0, "unit": "LPs/s",
+                   "cores": 1, "kind": "reference",
+                   "sample": (f"reference revised simplex (src/simplex, 1 thread) "
+                              f"on instances {done}, {sc}/{len(done)} optimal in "
+                              f"{wall:.1f} s wall, budget 5 s each")}
+    return cpu, cpu_par, cpu_ref
 
 
 # ----------------------------------------------------------------- zipf --
@@ -560,8 +641,15 @@ def main():
     ap.add_argument("--time-limit", type=float, default=60.0)
     ap.add_argument("--attempts", type=int, default=20)
     ap.add_argument("--fp64-sample", type=int, default=16)
-    ap.add_argument("--cpu-sample", type=int, default=6)
+    ap.add_argument("--fp64-budget-s", type=float, default=30.0,
+                    help="batch workload: wall-clock budget of the plain FP64 sample")
     ap.add_argument("--cpu-time-limit", type=float, default=10.0)
+    ap.add_argument("--cpu-budget-s", type=float, default=45.0,
+                    help="batch workload: wall-clock budget of the three CPU legs")
+    ap.add_argument("--budget-s", type=float, default=450.0,
+                    help="batch workload: wall-clock seconds from start within "
+                         "which warm-up and timed steps are run (at least one "
+                         "timed step always runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (a known-size "

@@ -29,7 +29,7 @@ class Options(C.Structure):
 class EngineConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("graph_slots", C.c_int),
                 ("max_repair_m", C.c_int), ("device", C.c_int),
-                ("column_slices", C.c_int)]
+                ("column_slices", C.c_int), ("exact", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -43,7 +43,7 @@ class SolveInfo(C.Structure):
                 ("lam2", C.c_double), ("eta0", C.c_double),
                 ("omega0", C.c_double), ("last_is_avg", C.c_int),
                 ("used_incumbent", C.c_int), ("repaired", C.c_int),
-                ("kernel_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("rollbacks", C.c_long)]
 
 
 # Every symbol include/igaos_pdhg.h declares: (name, restype, argtypes).
@@ -89,6 +89,8 @@ SIGNATURES = {
                                      _vp]),
     "igaos_ruiz_f64": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp,
                                  _vp, _vp, _vp, _vp]),
+    "igaos_ls_dots_work": (C.c_long, [C.c_int, C.c_int]),
+    "igaos_ls_dots_f64": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

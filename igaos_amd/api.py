@@ -213,10 +213,10 @@ class Engine:
     """
 
     def __init__(self, lp, precision=64, graph_slots=None, max_repair_m=None,
-                 column_slices=None):
+                 column_slices=None, exact=False):
         self.model = lp if isinstance(lp, Model) else model_from_lp(lp)
         self.precision = precision
-        cfg = _config(precision, graph_slots, max_repair_m, column_slices)
+        cfg = _config(precision, graph_slots, max_repair_m, column_slices, exact)
         h = _core.lib().igaos_pdhg_create(self.model._h, C.byref(cfg))
         if not h:
             raise RuntimeError(_core.last_error())
@@ -285,10 +285,11 @@ def _options(time_limit, tolerance, max_iterations, presolve, seed,
 
 
 def _config(precision=64, graph_slots=None, max_repair_m=None,
-            column_slices=None) -> EngineConfig:
+            column_slices=None, exact=False) -> EngineConfig:
     c = EngineConfig()
     _core.lib().igaos_engine_config_default(C.byref(c))
     c.precision = int(precision)
+    c.exact = int(bool(exact))
     if graph_slots is not None:
         c.graph_slots = int(graph_slots)
     if max_repair_m is not None:
@@ -319,16 +320,19 @@ def solve(path, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
 
 def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
              precision=64, verbosity=0, graph_slots=None,
-             max_repair_m=None, column_slices=None) -> Solution:
+             max_repair_m=None, column_slices=None, exact=False) -> Solution:
     """Solve an in-memory LP (``lpgen.LP`` or ``Model``) with the engine.
 
     ``precision=64`` is the plain FP64 engine; ``precision=32`` runs FP32
-    iterations on an FP64-refined correction problem.
+    iterations on an FP64-refined correction problem.  ``exact=True``
+    (FP64, iteration-bounded runs) fixes every summation order to the
+    reference's sequential one and disables fused multiply-adds, so the run
+    is bit-reproducible against a sequential FP64 host replay.
     """
     model = lp if isinstance(lp, Model) else model_from_lp(lp)
     return _run(model, _options(time_limit, tolerance, max_iterations, True,
                                 0, verbosity),
-                _config(precision, graph_slots, max_repair_m, column_slices))
+                _config(precision, graph_slots, max_repair_m, column_slices, exact))
 
 
 has_pdhg = _has_pdhg()

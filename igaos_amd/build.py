@@ -20,8 +20,10 @@ LIB = os.path.join(HERE, "libigaos_pdhg.so")
 CLI = os.path.join(HERE, "igaos")
 
 LIB_SOURCES = ["mps.cpp", "repair.cpp", "engine.cpp", "capi.cpp",
-               "pdhg_kernels.hip", "batch.hip", "batch_host.cpp",
+               "pdhg_kernels.hip", "exact.hip", "batch.hip", "batch_host.cpp",
                "adjoint.hip", "unroll.hip"]
+# per-source extra flags: the exact mode's kernels must not fuse a*b + c
+EXTRA = {"exact.hip": ["-ffp-contract=off"]}
 CLI_SOURCES = ["cli.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -47,7 +49,7 @@ def _compile(src, verbose):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src.replace(".", "_") + ".o")
     lang = ["-x", "hip"] if src.endswith(".hip") else []
-    _run([HIPCC] + FLAGS + lang + ["-c", path, "-o", obj], verbose)
+    _run([HIPCC] + FLAGS + EXTRA.get(src, []) + lang + ["-c", path, "-o", obj], verbose)
     return obj
 
 

@@ -13,6 +13,7 @@
 #include "igaos/status.h"
 #include "model.hpp"
 #include "pdhg.hpp"
+#include "exact.hpp"
 #include "pdhg_kernels.hpp"
 
 struct igaos_model {
@@ -29,6 +30,12 @@ int fail(const std::string& what) {
 }
 
 }  // namespace
+
+namespace igaos {
+// Sets the text igaos_last_error() returns on this thread (for entry
+// points defined in other translation units); returns -1.
+int set_last_error(const std::string& what) { return fail(what); }
+}  // namespace igaos
 
 extern "C" {
 
@@ -224,6 +231,7 @@ void igaos_engine_config_default(igaos_engine_config* cfg) {
     cfg->max_repair_m = e.max_repair_m;
     cfg->device = e.device;
     cfg->column_slices = e.column_slices;
+    cfg->exact = e.exact;
 }
 
 namespace {
@@ -251,6 +259,9 @@ const char* to_config(const igaos_engine_config* cfg, igaos::pdhg::EngineConfig&
     ec.max_repair_m = cfg->max_repair_m;
     ec.device = cfg->device;
     ec.column_slices = cfg->column_slices;
+    if (cfg->exact != 0 && cfg->exact != 1) return "exact must be 0 or 1";
+    if (cfg->exact && cfg->precision != 64) return "exact mode needs precision 64";
+    ec.exact = cfg->exact;
     return nullptr;
 }
 
@@ -288,6 +299,7 @@ void export_solution(const igaos::Solution& s, const igaos::pdhg::SolveInfo& si,
         info->used_incumbent = si.used_incumbent;
         info->repaired = si.repaired;
         info->kernel_ms = si.kernel_ms;
+        info->rollbacks = si.rollbacks;
     }
     if (s.status == igaos::Status::Error) g_err = s.message;
 }
@@ -477,6 +489,21 @@ int igaos_ruiz_f64(int m, int n, const int* ap, const int* ai,
     if (m < 0 || n < 0) return fail("negative size");
     igaos::dev::ruiz(m, n, ap, ai, ax, cp, ci, acx, dr, dc,
                      static_cast<hipStream_t>(stream));
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail(hipGetErrorString(e));
+}
+
+long igaos_ls_dots_work(int n, int m) {
+    if (n < 0 || m < 0) return fail("negative size");
+    return 3L * igaos::dev::dots_blocks(n, m);
+}
+
+int igaos_ls_dots_f64(int n, int m, const double* dx, const double* dy,
+                      const double* adx, double* work, double* out, void* stream) {
+    if (n < 0 || m < 0) return fail("negative size");
+    if ((n > 0 && !dx) || (m > 0 && (!dy || !adx)) || !work || !out)
+        return fail("null device array");
+    igaos::dev::ls_dots(n, m, dx, dy, adx, work, out, static_cast<hipStream_t>(stream));
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : fail(hipGetErrorString(e));
 }

@@ -40,6 +40,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "exact.hpp"
 #include "pdhg.hpp"
 #include "pdhg_kernels.hpp"
 
@@ -103,7 +104,7 @@ std::unique_ptr<io::Model> locality_copy(const io::Model& md,
                                          std::vector<int>& perm) {
     const int n = md.n;
     const long nnz = md.nnz();
-    if (n < (1 << 16) || md.m <= cfg.max_repair_m || nnz == 0) return nullptr;
+    if (cfg.exact || n < (1 << 16) || md.m <= cfg.max_repair_m || nnz == 0) return nullptr;
     std::vector<int> ord(n);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
@@ -213,6 +214,13 @@ class Engine final : public EngineBase {
     } slA_, slT_;
     dev::IterBufs<V> ib_;
     dev::KktData kd_;
+    // exact FP64 mode (exact.hip): per-attempt vectors, dot partials, the
+    // k^-0.3 / k^-0.6 table (filled up to the largest MAXIT run so far)
+    // and the KKT products and summands
+    DBuf<double> xdx_, xdy_, xtm_, xpart_, powtab_, kata_[2], ktm_[2], kterm_;
+    long long pow_filled_ = 0;
+    dev::ExactBufs eb_;
+    static constexpr long long EXACT_POW_CAP = 1LL << 21;
     dev::RowBlocks rA_, rT_;
     double lam2_ = 1.0, eta0_ = 1.0, omega0_ = 1.0;
 
@@ -226,6 +234,21 @@ class Engine final : public EngineBase {
     double dsum(const double* a, const double* b, long n);
     void check_metrics(int cur, Metrics& mc, Metrics& ma);
     void recompute_refs();
+    // y = A~ x (rows of A) in the engine's summation order
+    void spmv_rows(const double* x, double* y) {
+        if (cfg_.exact) dev::spmv_seq(m_, ap_.p, ai_.p, sax_.p, x, y, s_);
+        else dev::spmv(rA_, ap_.p, ai_.p, sax_.p, x, y, s_);
+    }
+    void spmv_cols(const double* yv, double* out) {
+        if (cfg_.exact) dev::spmv_seq(n_, cp_.p, ci_.p, sacx_.p, yv, out, s_);
+        else dev::spmv(rT_, cp_.p, ci_.p, sacx_.p, yv, out, s_);
+    }
+    void attempt(hipStream_t s) {
+        if constexpr (F32) dev::launch_attempt(ib_, s);
+        else if (cfg_.exact) dev::launch_attempt_exact(ib_, eb_, s);
+        else dev::launch_attempt(ib_, s);
+    }
+    void fill_powtab(long long maxit);
     void sync() { HIPCK(hipStreamSynchronize(s_)); }
     LsState read_state() {
         LsState h;
@@ -261,7 +284,8 @@ void Engine<V>::full(double* dst, const double* ref, const V* d, long k) {
 
 template <typename V>
 double Engine<V>::dsum(const double* a, const double* b, long n) {
-    dev::sumsq_diff(a, b, n, red_.p, scal_.p, s_);
+    if (cfg_.exact) dev::sumsq_seq(a, b, n, scal_.p, s_);
+    else dev::sumsq_diff(a, b, n, red_.p, scal_.p, s_);
     double h = 0.0;
     HIPCK(hipMemcpyAsync(&h, scal_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
     sync();
@@ -277,13 +301,13 @@ double Engine<V>::power_iteration() {
     w.alloc(n_);
     dev::fill(v.p, n_, 1.0 / std::sqrt(static_cast<double>(n_)), s_);
     for (int it = 0; it < 30; ++it) {
-        dev::spmv(rA_, ap_.p, ai_.p, sax_.p, v.p, tmpm_.p, s_);
-        dev::spmv(rT_, cp_.p, ci_.p, sacx_.p, tmpm_.p, w.p, s_);
+        spmv_rows(v.p, tmpm_.p);
+        spmv_cols(tmpm_.p, w.p);
         const double nw = std::sqrt(dsum(w.p, nullptr, n_));
         if (nw < 1e-300) break;
         dev::scale_into(v.p, w.p, n_, 1.0 / nw, s_);
     }
-    dev::spmv(rA_, ap_.p, ai_.p, sax_.p, v.p, tmpm_.p, s_);
+    spmv_rows(v.p, tmpm_.p);
     const double na = std::sqrt(dsum(tmpm_.p, nullptr, m_));
     double lam2 = na * na;
     if (!(lam2 > 0) || !std::isfinite(lam2)) lam2 = 1.0;
@@ -338,7 +362,8 @@ void Engine<V>::setup() {
     eta0_ = 1.2 / std::sqrt(lam2_);
     {  // pdhg.cu:647-663
         const double cn = dsum(sc_.p, nullptr, n_);
-        dev::sumsq_q(srmin_.p, srmax_.p, m_, red_.p, scal_.p, s_);
+        if (cfg_.exact) dev::sumsq_q_seq(srmin_.p, srmax_.p, m_, scal_.p, s_);
+        else dev::sumsq_q(srmin_.p, srmax_.p, m_, red_.p, scal_.p, s_);
         double qn = 0.0;
         HIPCK(hipMemcpyAsync(&qn, scal_.p, sizeof(double), hipMemcpyDeviceToHost, s_));
         sync();
@@ -415,10 +440,29 @@ void Engine<V>::setup() {
     kd_.partC = partC_.p; kd_.partR = partR_.p; kd_.partR2 = partR2_.p;
     kd_.rowres = rowres_.p;
 
+    if (cfg_.exact) {
+        xdx_.alloc(n_);
+        xdy_.alloc(m_);
+        xtm_.alloc(m_);
+        xpart_.alloc(3 * dev::DOTS_MAX_BLOCKS);
+        powtab_.alloc(2 * EXACT_POW_CAP);
+        for (int q = 0; q < 2; ++q) {
+            kata_[q].alloc(n_);
+            ktm_[q].alloc(m_);
+        }
+        kterm_.alloc(2 * (2 * static_cast<size_t>(n_) + m_));
+        eb_.dx = xdx_.p;
+        eb_.dy = xdy_.p;
+        eb_.tm = xtm_.p;
+        eb_.part = xpart_.p;
+        eb_.powtab = powtab_.p;
+        eb_.pow_cap = EXACT_POW_CAP;
+    }
+
     const int G = cfg_.graph_slots;
     if (G > 0) {  // G attempt slots, replayed until the state freezes
         HIPCK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
-        for (int g = 0; g < G; ++g) dev::launch_attempt(ib_, s_);
+        for (int g = 0; g < G; ++g) attempt(s_);
         HIPCK(hipStreamEndCapture(s_, &graph_));
         HIPCK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
     }
@@ -439,6 +483,7 @@ void Engine<V>::setup() {
 // slices: 435 us, 6: 463 us).
 template <typename V>
 int Engine<V>::slices_for(int cols, int rows, long nnz) const {
+    if (cfg_.exact) return 0;
     const int k = cfg_.column_slices;
     if (k >= 0) return std::min(k, dev::MAX_SLICES);
     const double bytes = static_cast<double>(cols) * sizeof(V);
@@ -521,6 +566,18 @@ void Engine<V>::check_metrics(int cur, Metrics& mc, Metrics& ma) {
             it.yd[q] = ky_[q].p;
         }
         dev::launch_kkt(kd_, it, met_.p, s_);
+    } else if (cfg_.exact) {
+        dev::ExactKkt k;
+        k.x[0] = xcur(cur);
+        k.y[0] = ycur(cur);
+        k.x[1] = xavg_.p;
+        k.y[1] = yavg_.p;
+        for (int q = 0; q < 2; ++q) {
+            k.ata[q] = kata_[q].p;
+            k.tm[q] = ktm_[q].p;
+        }
+        k.term = kterm_.p;
+        dev::launch_kkt_exact(kd_, k, met_.p, s_);
     } else {
         dev::KktIter<V> it;
         it.xd[0] = xcur(cur);
@@ -536,6 +593,27 @@ void Engine<V>::check_metrics(int cur, Metrics& mc, Metrics& ma) {
     ma = h[1];
 }
 
+// Exact mode: k^-0.3 and k^-0.6 for k = 1..maxit with the host's pow, as
+// the reference's host controller computes them (pdhg.cu:780-781).
+template <typename V>
+void Engine<V>::fill_powtab(long long maxit) {
+    if (maxit > EXACT_POW_CAP)
+        throw std::invalid_argument(
+            "exact mode runs at most 2097152 iterations per solve (set "
+            "time_limit_s >= 1e6 or <= 0 and max_iterations within that)");
+    if (maxit <= pow_filled_) return;
+    std::vector<double> h(2 * static_cast<size_t>(maxit - pow_filled_));
+    for (long long k = pow_filled_ + 1; k <= maxit; ++k) {
+        const double kk = static_cast<double>(k);
+        h[2 * (k - pow_filled_ - 1)] = std::pow(kk, -0.3);
+        h[2 * (k - pow_filled_ - 1) + 1] = std::pow(kk, -0.6);
+    }
+    HIPCK(hipMemcpyAsync(powtab_.p + 2 * pow_filled_, h.data(), sizeof(double) * h.size(),
+                         hipMemcpyHostToDevice, s_));
+    sync();
+    pow_filled_ = maxit;
+}
+
 template <typename V>
 Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
     Solution sol;
@@ -546,6 +624,7 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
                            ? 20000000L
                            : opt_.max_iterations;
     const double TOL = opt_.tolerance;
+    if (cfg_.exact) fill_powtab(MAXIT);
 
     LsState st{};
     st.eta = eta0_;
@@ -565,7 +644,7 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
             const long reps = (slots + G - 1) / G;
             for (long r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(exec_, s_));
         } else {
-            for (long r = 0; r < slots; ++r) dev::launch_attempt(ib_, s_);
+            for (long r = 0; r < slots; ++r) attempt(s_);
         }
     };
 
@@ -579,7 +658,7 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
     long last_rollback = 0;
     long next_repair = 5000, repair_spacing = 2000;
     bool repaired = false, time_limit = false;
-    long iters_run = 0, restarts = 0, checks = 0, refinements = 0;
+    long iters_run = 0, restarts = 0, checks = 0, refinements = 0, rollbacks = 0;
     std::vector<double> rep_x, rep_y;  // repaired pair (original units)
     float kernel_ms_total = 0.0f;
 
@@ -605,7 +684,9 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
         const long kdone = st.k - 1;
         iters_run = kdone;
         if (st.mix_pending) {
-            dev::launch_flush(ib_, st.cur, st.mix_w, s_);
+            if constexpr (F32) dev::launch_flush(ib_, st.cur, st.mix_w, s_);
+            else if (cfg_.exact) dev::launch_flush_exact(ib_, st.cur, st.mix_w, s_);
+            else dev::launch_flush(ib_, st.cur, st.mix_w, s_);
             st.mix_pending = 0;
         }
         if (kdone <= 0 || kdone % CHECK_EVERY != 0) break;  // MAXIT reached
@@ -716,7 +797,7 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
                 copy_v(ycur(cur), yb_.p, m_);
                 copy_v(xavg_.p, xb_.p, n_);
                 copy_v(yavg_.p, yb_.p, m_);
-                dev::spmv(rA_, ap_.p, ai_.p, sax_.p, xcur(cur), axcur_.p, s_);
+                spmv_rows(xcur(cur), axcur_.p);
             }
             st.eta = std::min(st.eta, 0.5 * eta0_);
             st.hn = 0;
@@ -724,6 +805,7 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
             rs.epoch_kkt = -1.0;
             rs.prev_cand = INF;
             last_rollback = kdone;
+            ++rollbacks;
             if (opt_.verbosity > 1) std::fprintf(stderr, "[pdhg] rollback @k=%ld to inc\n", kdone);
         }
         // restart + primal weight (pdhg.cu:989-1075)
@@ -751,7 +833,7 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
                 }
                 copy_v(xavg_.p, xcur(cur), n_);
                 copy_v(yavg_.p, ycur(cur), m_);
-                dev::spmv(rA_, ap_.p, ai_.p, sax_.p, xcur(cur), axcur_.p, s_);
+                spmv_rows(xcur(cur), axcur_.p);
             }
             st.hn = 0;
             rs.epoch_start = kdone;
@@ -865,12 +947,14 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
         info->used_incumbent = used_inc ? 1 : 0;
         info->repaired = repaired ? 1 : 0;
         info->kernel_ms = kernel_ms_total;
+        info->rollbacks = rollbacks;
     }
     return sol;
 }
 
 template <typename V>
 AttemptTiming Engine<V>::profile(int attempts) {
+    if (cfg_.exact) throw std::invalid_argument("attempt timing is not offered in exact mode");
     reset();
     LsState st{};
     st.eta = eta0_;
@@ -932,6 +1016,8 @@ std::unique_ptr<EngineBase> make_engine(const io::Model& model,
         throw std::invalid_argument("malformed model: CSR/CSC pointer sizes do not match m, n");
     // the primal step's launch covers the rows of A' (one per column)
     if (model.n == 0) throw std::invalid_argument("the LP has no columns (n = 0)");
+    if (cfg.exact && cfg.precision != 64)
+        throw std::invalid_argument("exact mode is an FP64 mode (precision 64)");
     if (cfg.precision == 32) return std::make_unique<Engine<float>>(model, cfg);
     return std::make_unique<Engine<double>>(model, cfg);
 }

@@ -18,6 +18,8 @@ struct EngineConfig {
     int max_repair_m = 4096; // host dense dual repair only up to this many rows
     int device = 0;
     int column_slices = -1;  // column slices of A / A' products: -1 auto, 0 off, 1..16 forced
+    int exact = 0;           // 1: exact FP64 mode (reference summation orders, no FMA;
+                             // precision 64, iteration-bounded runs only)
 };
 
 struct SolveInfo {
@@ -32,6 +34,7 @@ struct SolveInfo {
     int used_incumbent = 0;
     int repaired = 0;
     double kernel_ms = 0.0;    // device time of the attempt kernels (events)
+    long rollbacks = 0;        // incumbent rollbacks (pdhg.cu:965-988)
 };
 
 // The reference's entry point (src/pdhg/pdhg.hpp:9).  Besides the

@@ -36,17 +36,18 @@ struct LsState {
     double dots[3];      // dots of the last attempt (diagnostics)
 };
 
-__host__ __device__ inline void ls_decide(LsState& s, double d0, double d1,
-                                          double d2) {
+// p3 = k^-0.3 and p6 = k^-0.6 of the iteration s.k being decided.  The
+// rule is written without fused multiply-adds (the reference's controller
+// runs on the host, pdhg.cu:770-793).
+__host__ __device__ inline void ls_decide_pw(LsState& s, double d0, double d1,
+                                             double d2, double p3, double p6) {
+#pragma clang fp contract(off)
     const double INF = __builtin_huge_val();
     const double num = s.omega * d0 + d1 / s.omega;
     const double cross = d2;
     const double etabar = (num > 0.0 && cross > 0.0) ? num / (2.0 * cross) : INF;
-    const double kk = static_cast<double>(s.k);
     const double eta_new =
-        std::isfinite(etabar)
-            ? fmin((1.0 - pow(kk, -0.3)) * etabar, (1.0 + pow(kk, -0.6)) * s.eta)
-            : INF;
+        std::isfinite(etabar) ? fmin((1.0 - p3) * etabar, (1.0 + p6) * s.eta) : INF;
     s.dots[0] = d0;
     s.dots[1] = d1;
     s.dots[2] = d2;
@@ -69,6 +70,12 @@ __host__ __device__ inline void ls_decide(LsState& s, double d0, double d1,
         s.fresh = 0;
         s.mix_pending = 0;
     }
+}
+
+__host__ __device__ inline void ls_decide(LsState& s, double d0, double d1,
+                                          double d2) {
+    const double kk = static_cast<double>(s.k);
+    ls_decide_pw(s, d0, d1, d2, pow(kk, -0.3), pow(kk, -0.6));
 }
 
 // pdhg.cu:159-170

@@ -14,7 +14,7 @@
 extern "C" {
 #endif
 
-#define IGAOS_PDHG_ABI_VERSION 1
+#define IGAOS_PDHG_ABI_VERSION 2
 
 /* Status codes: the order of igaos::Status (include/igaos/status.h:5-14). */
 enum {
@@ -97,6 +97,16 @@ typedef struct {
     int device;
     int column_slices; /* column slices of the A / A' products (gather
                           locality on large LPs): -1 auto, 0 off, 1..16 */
+    int exact;         /* 1: exact FP64 mode -- every sparse row summed in
+                          storage order, the line-search dots in the
+                          reference's two-stage layout (pdhg.cu:63-125),
+                          norms and the KKT sums in index order, no fused
+                          multiply-add, k^-0.3 / k^-0.6 from the host's pow:
+                          a run is bit-reproducible against a sequential
+                          FP64 host replay of pdhg.cu.  Precision 64 only,
+                          iteration-bounded runs only (time_limit_s >= 1e6
+                          or <= 0, max_iterations <= 2^21); far slower than
+                          the default kernels.  0: default. */
 } igaos_engine_config;
 void igaos_engine_config_default(igaos_engine_config* cfg);
 
@@ -112,6 +122,7 @@ typedef struct {
     double eta, omega, lam2, eta0, omega0;
     int last_is_avg, used_incumbent, repaired;
     double kernel_ms;
+    long rollbacks; /* incumbent rollbacks (pdhg.cu:965-988) */
 } igaos_solve_info;
 
 /* Replaces igaos::pdhg::solve (src/pdhg/pdhg.hpp:9, pdhg.cu:457-1156).
@@ -366,6 +377,18 @@ int igaos_spmv_csr_f32(int nblk, const int* rowblocks, const int* ptr,
 int igaos_ruiz_f64(int m, int n, const int* ap, const int* ai,
                    const double* ax, const int* cp, const int* ci,
                    const double* acx, double* dr, double* dc, void* stream);
+/* The three line-search dot products |dx|^2, |dy|^2 and dy'adx of an
+ * attempt; replaces kern_dots + kern_dotfin (pdhg.cu:63-125, launched at
+ * 764-767) with the same two-stage fixed order: gD = min(ceil(max(n,m) /
+ * 256), 4096) workgroups of 256 threads accumulate grid-stride and reduce
+ * by a halving tree, then one workgroup sums the gD partials strided and
+ * reduces them by the same tree, so results are bit-reproducible.  dx (n),
+ * dy and adx (m) and out (3 doubles) are device arrays; work is device
+ * scratch of at least igaos_ls_dots_work(n, m) doubles (3 gD).  Enqueued on
+ * the stream; nothing is allocated. */
+long igaos_ls_dots_work(int n, int m);
+int igaos_ls_dots_f64(int n, int m, const double* dx, const double* dy,
+                      const double* adx, double* work, double* out, void* stream);
 
 #ifdef __cplusplus
 }

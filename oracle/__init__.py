@@ -56,7 +56,7 @@ class _Result(C.Structure):
                 ("omega", C.c_double), ("lam2", C.c_double),
                 ("eta0", C.c_double), ("omega0", C.c_double),
                 ("last_is_avg", C.c_int), ("used_incumbent", C.c_int),
-                ("repaired", C.c_int)]
+                ("repaired", C.c_int), ("rollbacks", C.c_long)]
 
 
 _lib = None
@@ -155,6 +155,7 @@ class Result:
     x: np.ndarray
     y: np.ndarray
     trace_eta: np.ndarray | None = None
+    rollbacks: int = 0
 
 
 def set_mutation(code: int):
@@ -184,7 +185,8 @@ def solve(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
                   r.restarts, r.eta, r.omega, r.lam2, r.eta0, r.omega0,
                   bool(r.last_is_avg), bool(r.used_incumbent),
                   bool(r.repaired), x, y,
-                  tr[:min(trace, r.iterations)].copy() if trace else None)
+                  tr[:min(trace, r.iterations)].copy() if trace else None,
+                  r.rollbacks)
 
 
 def spmv(ptr, idx, val, x):

@@ -763,6 +763,7 @@ int orc_pdhg_solve(const orc_model* lp, const orc_options* opt,
             orc_spmv(m, lp->ap, lp->ai, sax, x, tm);
             COPY(axcur, tm, m);
             last_rollback = k;
+            ++res->rollbacks;
         }
         /* restart + primal weight, pdhg.cu:989-1075 */
         double cand = dmin(ec, ea);

@@ -43,6 +43,7 @@ typedef struct {
     int last_is_avg;
     int used_incumbent;
     int repaired;
+    long rollbacks; /* incumbent rollbacks (pdhg.cu:965-988) */
 } orc_result;
 
 /* mutation switch for pin tests: 0 = faithful.  See pdhg_oracle.c. */

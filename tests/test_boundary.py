@@ -47,7 +47,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_and_status_names():
     L = _core.lib()
-    assert L.igaos_pdhg_abi_version() == 1
+    assert L.igaos_pdhg_abi_version() == 2
     names = [L.igaos_status_name(i).decode() for i in range(8)]
     assert names == ["optimal", "near-optimal", "feasible", "infeasible",
                      "unbounded", "iteration-limit", "time-limit", "error"]
@@ -285,6 +285,16 @@ def test_column_slice_invariants(S):
         counts = np.array([sptr[sc][-1] - sptr[sc][0] for sc in range(S)])
         colmax = np.bincount(idx, minlength=ncols).max()
         assert counts.max() <= ptr[-1] / S + colmax
+
+
+def test_exact_mode_config_is_checked_before_any_device_work():
+    # the exact mode is FP64 only; the C-ABI refuses the pair as an invalid
+    # argument (negative return, igaos_last_error), GPU or not
+    import igaos_amd
+    from igaos_amd import lpgen
+    lp = lpgen.gen_random_eq(seed=3, m=20, n=50)
+    with pytest.raises(ValueError, match="exact mode needs precision 64"):
+        igaos_amd.solve_lp(lp, precision=32, exact=True, time_limit=1e7, max_iterations=10)
 
 
 def test_bench_refuses_more_than_one_gpu():
