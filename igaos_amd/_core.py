@@ -91,6 +91,7 @@ SIGNATURES = {
                                  _vp, _vp, _vp, _vp]),
     "igaos_ls_dots_work": (C.c_long, [C.c_int, C.c_int]),
     "igaos_ls_dots_f64": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "igaos_ls_dots_f64_noalloc": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

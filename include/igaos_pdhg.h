@@ -389,6 +389,18 @@ int igaos_ruiz_f64(int m, int n, const int* ap, const int* ai,
 long igaos_ls_dots_work(int n, int m);
 int igaos_ls_dots_f64(int n, int m, const double* dx, const double* dy,
                       const double* adx, double* work, double* out, void* stream);
+/* No counterpart in the reference: igaos_ls_dots_f64 for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid hipMalloc) and have no workspace to hand over.  The
+ * partials go to a scratch array the library holds per GPU for the life of
+ * the process (96 KiB: 3 x 4096 doubles, which the layout's 4096-workgroup
+ * cap makes enough for every n, m); sizes that would need more are refused
+ * with igaos_last_error().  Same arguments as igaos_ls_dots_f64 without
+ * `work`; the stream must belong to the current device.  Calls that overlap
+ * on one GPU share the scratch and race; each call rewrites every partial
+ * it reads, so no clearing is needed between calls.  Allocates nothing. */
+int igaos_ls_dots_f64_noalloc(int n, int m, const double* dx, const double* dy,
+                              const double* adx, double* out, void* stream);
 
 #ifdef __cplusplus
 }
