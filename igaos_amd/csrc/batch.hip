@@ -685,15 +685,102 @@ constexpr bool GATHER_ORDERED = sizeof(I) == 2;
 
 // Values of the gather-ordered copies: per table entry (one row block),
 // dst[p] = src[p0 + slot[p]] over the block's nonzeros.
+// FP32: the packed records {index | slot << 16, value bits} instead.
 template <typename V>
 __global__ __launch_bounds__(NT) void k_lk_sortvals(const Arrays<V>* A, const LPDesc* lps,
                                                     const LkEnt* tbl, bool transposed) {
     const LkEnt e = tbl[blockIdx.x];
     const LP<V> L = view(A, lps[e.lp]);
     const V* src = transposed ? L.facx() : L.fax();
-    V* dst = transposed ? L.facxs() : L.faxs();
     const unsigned short* slot = transposed ? L.cslot() : L.aslot();
-    for (int p = e.p0 + threadIdx.x; p < e.p1; p += NT) dst[p] = src[e.p0 + slot[p]];
+    if constexpr (sizeof(V) == 4) {
+        const unsigned short* idx = transposed ? L.cis16() : L.ais16();
+        uint2* dst = transposed ? L.cpk() : L.apk();
+        for (int p = e.p0 + threadIdx.x; p < e.p1; p += NT) {
+            const unsigned int sq = slot[p];
+            dst[p] = make_uint2(static_cast<unsigned int>(idx[p]) | (sq << 16),
+                                __float_as_uint(src[e.p0 + sq]));
+        }
+    } else {
+        V* dst = transposed ? L.facxs() : L.faxs();
+        for (int p = e.p0 + threadIdx.x; p < e.p1; p += NT) dst[p] = src[e.p0 + slot[p]];
+    }
+}
+
+// Row sums of one row block (dev::rowblock_sums with a slot permutation)
+// from packed FP32 records: one 8-byte load per nonzero carries its
+// index, its slot in the block's row order and its value.  Same products
+// at the same LDS positions, so the sums are bit-identical to the three
+// separate streams.
+__device__ __forceinline__ void rowblock_sums_packed(const int* __restrict__ ptr,
+                                                     const uint2* __restrict__ rec,
+                                                     const float* __restrict__ x, int r0,
+                                                     int r1, int p0, int p1, float* s_rows,
+                                                     float* s_prod, float* red) {
+    using dev::BS;
+    using dev::NNZ_MAX;
+    const int tid = threadIdx.x;
+    const int nrows = r1 - r0;
+    const int nz = p1 - p0;
+    if (nrows == 1 && nz > NNZ_MAX) {  // a single long row keeps its order
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        int p = p0 + tid;
+        for (; p + 3 * BS < p1; p += 4 * BS) {
+            const uint2 a = rec[p], b = rec[p + BS], c = rec[p + 2 * BS], d = rec[p + 3 * BS];
+            s0 += __uint_as_float(a.y) * x[a.x & 0xffffu];
+            s1 += __uint_as_float(b.y) * x[b.x & 0xffffu];
+            s2 += __uint_as_float(c.y) * x[c.x & 0xffffu];
+            s3 += __uint_as_float(d.y) * x[d.x & 0xffffu];
+        }
+        for (; p < p1; p += BS) {
+            const uint2 a = rec[p];
+            s0 += __uint_as_float(a.y) * x[a.x & 0xffffu];
+        }
+        const float s = dev::block_sum((s0 + s1) + (s2 + s3), red);
+        if (tid == 0) s_rows[0] = s;
+        __syncthreads();
+        return;
+    }
+    int tpr = 1;
+    while (tpr < BS && tpr * 2 * nrows <= BS) tpr *= 2;
+    const int row = tid / tpr;
+    const int lane = tid & (tpr - 1);
+    int a = 0, b = 0;
+    if (row < nrows) {
+        a = ptr[r0 + row] - p0;
+        b = ptr[r0 + row + 1] - p0;
+    }
+    if (nz > 0) {
+        constexpr int U = NNZ_MAX / BS;
+        const int last = nz - 1;
+        uint2 r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) r[u] = rec[p0 + min(tid + u * BS, last)];
+        float xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[r[u].x & 0xffffu];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (tid + u * BS < nz) s_prod[r[u].x >> 16] = __uint_as_float(r[u].y) * xv[u];
+    }
+    __syncthreads();
+    float s = 0.0f;
+    for (int q = a + lane; q < b; q += tpr) s += s_prod[q];
+    const int wl = tpr < dev::WAVE ? tpr : dev::WAVE;
+    for (int o = wl / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (tpr <= dev::WAVE) {
+        if (row < nrows && lane == 0) s_rows[row] = s;
+    } else {
+        if ((tid & (dev::WAVE - 1)) == 0) red[tid / dev::WAVE] = s;
+        __syncthreads();
+        if (lane == 0 && row < nrows) {
+            const int wpr = tpr / dev::WAVE;
+            float t = red[row * wpr];
+            for (int k = 1; k < wpr; ++k) t += red[row * wpr + k];
+            s_rows[row] = t;
+        }
+    }
+    __syncthreads();
 }
 
 template <typename V>
@@ -745,7 +832,10 @@ __global__ __launch_bounds__(NT, LK_WAVES<V>) void k_lk_primal(const Arrays<V>* 
     V* rows = reinterpret_cast<V*>(S.rows);
     // A'y is recomputed on a retry too (same y, same sums) rather than
     // stored for it: retries are rare, the store would be paid every round
-    if constexpr (GATHER_ORDERED<I>)
+    if constexpr (GATHER_ORDERED<I> && sizeof(V) == 4)
+        rowblock_sums_packed(L.cp(), L.cpk(), y, r0, r1, p0, p1, rows,
+                             reinterpret_cast<V*>(S.prod), reinterpret_cast<V*>(S.red));
+    else if constexpr (GATHER_ORDERED<I>)
         dev::rowblock_sums<V>(L.cp(), L.cis16(), L.facxs(), dev::GatherPlain<V>{y}, r0, r1,
                               p0, p1, rows, reinterpret_cast<V*>(S.prod),
                               reinterpret_cast<V*>(S.red), L.cslot());
@@ -796,7 +886,10 @@ __global__ __launch_bounds__(NT, LK_WAVES<V>) void k_lk_dual(const Arrays<V>* A,
     V* __restrict__ yn = L.y(cur ^ 1);
     const int r0 = e.r0, p0 = e.p0, r1 = e.r1, p1 = e.p1;
     V* rows = reinterpret_cast<V*>(S.rows);
-    if constexpr (GATHER_ORDERED<I>)
+    if constexpr (GATHER_ORDERED<I> && sizeof(V) == 4)
+        rowblock_sums_packed(L.ap(), L.apk(), L.xbar(), r0, r1, p0, p1, rows,
+                             reinterpret_cast<V*>(S.prod), reinterpret_cast<V*>(S.red));
+    else if constexpr (GATHER_ORDERED<I>)
         dev::rowblock_sums<V>(L.ap(), L.ais16(), L.faxs(), dev::GatherPlain<V>{L.xbar()},
                               r0, r1, p0, p1, rows, reinterpret_cast<V*>(S.prod),
                               reinterpret_cast<V*>(S.red), L.aslot());

@@ -54,6 +54,10 @@ struct Arrays {
     // faxs / facxs hold the values in the same order (filled per run).
     const unsigned short *ais16, *cis16, *aslot, *cslot;
     V *faxs, *facxs;
+    // FP32: the same three streams packed into one 8-byte record per
+    // nonzero, {column or row | slot << 16, value bits} (filled per run),
+    // so the attempt kernels issue one load per nonzero instead of three
+    uint2 *apk, *cpk;
     const int *rbA, *rbT;             // per-LP row-block tables (pairs)
     const double *ax, *acx;           // original values
     const double *c, *cl, *cu, *rmin, *rmax;

@@ -70,6 +70,8 @@ struct LP {
     __device__ const unsigned short* cslot() const { return A->cslot + zo; }
     __device__ V* faxs() const { return A->faxs + zo; }
     __device__ V* facxs() const { return A->facxs + zo; }
+    __device__ uint2* apk() const { return A->apk + zo; }
+    __device__ uint2* cpk() const { return A->cpk + zo; }
     __device__ const int* rbA() const { return A->rbA + rbao; }
     __device__ const int* rbT() const { return A->rbT + rbto; }
     __device__ const double* ax() const { return A->ax + zo; }

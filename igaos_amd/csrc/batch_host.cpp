@@ -210,6 +210,7 @@ void fill_arrays(igaos_batch& B, igaos::batch::Arrays<V>& A, const int* ap,
     A.acx = M.upload(acx, z);
     A.ai16 = A.ci16 = A.ais16 = A.cis16 = A.aslot = A.cslot = nullptr;
     A.faxs = A.facxs = nullptr;
+    A.apk = A.cpk = nullptr;
     bool fits = true;
     for (const LPDesc& d : B.desc) fits = fits && d.m <= 65536 && d.n <= 65536;
     if (fits && z > 0) {  // 16-bit column indices for the lockstep attempt kernels
@@ -226,8 +227,13 @@ void fill_arrays(igaos_batch& B, igaos::batch::Arrays<V>& A, const int* ap,
         gather_order(B.desc, B.rbT, true, ci, z, h, hs);
         A.cis16 = M.upload(h.data(), z);
         A.cslot = M.upload(hs.data(), z);
-        A.faxs = M.alloc<V>(z);
-        A.facxs = M.alloc<V>(z);
+        if constexpr (sizeof(V) == 4) {
+            A.apk = M.alloc<uint2>(z);
+            A.cpk = M.alloc<uint2>(z);
+        } else {
+            A.faxs = M.alloc<V>(z);
+            A.facxs = M.alloc<V>(z);
+        }
     }
     A.c = M.upload(c, n);
     A.cl = M.upload(cl, n);
