@@ -213,10 +213,11 @@ class Engine:
     """
 
     def __init__(self, lp, precision=64, graph_slots=None, max_repair_m=None,
-                 column_slices=None, exact=False):
+                 column_slices=None, exact=False, xcd_pinned=None, scaling="ruiz"):
         self.model = lp if isinstance(lp, Model) else model_from_lp(lp)
         self.precision = precision
-        cfg = _config(precision, graph_slots, max_repair_m, column_slices, exact)
+        cfg = _config(precision, graph_slots, max_repair_m, column_slices, exact,
+                      xcd_pinned, scaling)
         h = _core.lib().igaos_pdhg_create(self.model._h, C.byref(cfg))
         if not h:
             raise RuntimeError(_core.last_error())
@@ -284,12 +285,22 @@ def _options(time_limit, tolerance, max_iterations, presolve, seed,
     return o
 
 
+# diagonal preconditioning: Ruiz (the reference's) or Ruiz + Pock-Chambolle
+SCALINGS = {"ruiz": 0, "ruiz+pock-chambolle": 1}
+
+
 def _config(precision=64, graph_slots=None, max_repair_m=None,
-            column_slices=None, exact=False) -> EngineConfig:
+            column_slices=None, exact=False, xcd_pinned=None,
+            scaling="ruiz") -> EngineConfig:
     c = EngineConfig()
     _core.lib().igaos_engine_config_default(C.byref(c))
     c.precision = int(precision)
+    if scaling not in SCALINGS:
+        raise ValueError(f"scaling must be one of {sorted(SCALINGS)}, got {scaling!r}")
+    c.scaling = SCALINGS[scaling]
     c.exact = int(bool(exact))
+    if xcd_pinned is not None:
+        c.xcd_pinned = int(bool(xcd_pinned))
     if graph_slots is not None:
         c.graph_slots = int(graph_slots)
     if max_repair_m is not None:
@@ -320,7 +331,8 @@ def solve(path, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
 
 def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
              precision=64, verbosity=0, graph_slots=None,
-             max_repair_m=None, column_slices=None, exact=False) -> Solution:
+             max_repair_m=None, column_slices=None, exact=False,
+             xcd_pinned=None, scaling="ruiz") -> Solution:
     """Solve an in-memory LP (``lpgen.LP`` or ``Model``) with the engine.
 
     ``precision=64`` is the plain FP64 engine; ``precision=32`` runs FP32
@@ -332,7 +344,8 @@ def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
     model = lp if isinstance(lp, Model) else model_from_lp(lp)
     return _run(model, _options(time_limit, tolerance, max_iterations, True,
                                 0, verbosity),
-                _config(precision, graph_slots, max_repair_m, column_slices, exact))
+                _config(precision, graph_slots, max_repair_m, column_slices, exact,
+                        xcd_pinned, scaling))
 
 
 has_pdhg = _has_pdhg()

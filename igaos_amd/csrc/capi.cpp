@@ -231,6 +231,8 @@ void igaos_engine_config_default(igaos_engine_config* cfg) {
     cfg->max_repair_m = e.max_repair_m;
     cfg->device = e.device;
     cfg->column_slices = e.column_slices;
+    cfg->xcd_pinned = e.xcd_pinned;
+    cfg->scaling = e.scaling;
     cfg->exact = e.exact;
 }
 
@@ -259,6 +261,10 @@ const char* to_config(const igaos_engine_config* cfg, igaos::pdhg::EngineConfig&
     ec.max_repair_m = cfg->max_repair_m;
     ec.device = cfg->device;
     ec.column_slices = cfg->column_slices;
+    if (cfg->xcd_pinned != 0 && cfg->xcd_pinned != 1) return "xcd_pinned must be 0 or 1";
+    ec.xcd_pinned = cfg->xcd_pinned;
+    if (cfg->scaling != 0 && cfg->scaling != 1) return "scaling must be 0 (Ruiz) or 1 (Ruiz + Pock-Chambolle)";
+    ec.scaling = cfg->scaling;
     if (cfg->exact != 0 && cfg->exact != 1) return "exact must be 0 or 1";
     if (cfg->exact && cfg->precision != 64) return "exact mode needs precision 64";
     ec.exact = cfg->exact;

@@ -209,7 +209,7 @@ class Engine final : public EngineBase {
     DBuf<Metrics> met_;
     // column-sliced copies of A (gathers x) and A' (gathers y), when used
     struct SliceBufs {
-        DBuf<int> ptr, idx, quad, bslice;
+        DBuf<int> ptr, idx, quad, bslice, xmap;
         DBuf<V> val, part;
     } slA_, slT_;
     dev::IterBufs<V> ib_;
@@ -342,6 +342,11 @@ void Engine<V>::setup() {
     dr_.alloc(m_);
     dc_.alloc(n_);
     dev::ruiz(m_, n_, ap_.p, ai_.p, ax_.p, cp_.p, ci_.p, acx_.p, dr_.p, dc_.p, s_);
+    tmpm_.alloc(m_);
+    tmpn_.alloc(n_);
+    if (cfg_.scaling == 1)
+        dev::pock_chambolle(m_, n_, ap_.p, ai_.p, ax_.p, cp_.p, ci_.p, acx_.p, dr_.p, dc_.p,
+                            tmpm_.p, tmpn_.p, s_);
     sax_.alloc(nnz);
     sacx_.alloc(nnz);
     sc_.alloc(n_);
@@ -353,8 +358,6 @@ void Engine<V>::setup() {
                     dc_.p, c_.p, cl_.p, cu_.p, rmin_.p, rmax_.p, sax_.p,
                     sacx_.p, sc_.p, scl_.p, scu_.p, srmin_.p, srmax_.p, s_);
 
-    tmpm_.alloc(m_);
-    tmpn_.alloc(n_);
     red_.alloc(1024);
     scal_.alloc(4);
 
@@ -517,6 +520,13 @@ void Engine<V>::make_slices(dev::Sliced<V>& sl, SliceBufs& B, int rows, int cols
     for (int sc = 0, k = 0; sc < S; ++sc) {
         while (k < sl.nblk && h.bslice[k] == sc) ++k;
         sl.sb[sc + 1] = k;
+    }
+    if (cfg_.xcd_pinned) {
+        const std::vector<int> xm = dev::build_xcd_map(h.bslice, S);
+        B.xmap.upload(xm.data(), xm.size(), s_);
+        sync();  // xm is freed on scope exit
+        sl.xmap = B.xmap.p;
+        sl.xgrid = static_cast<int>(xm.size());
     }
     sl.ptr = B.ptr.p;
     sl.idx = B.idx.p;

@@ -18,6 +18,8 @@ struct EngineConfig {
     int max_repair_m = 4096; // host dense dual repair only up to this many rows
     int device = 0;
     int column_slices = -1;  // column slices of A / A' products: -1 auto, 0 off, 1..16 forced
+    int xcd_pinned = 0;      // column slices: 1 = all in one launch, each XCD on its own slices
+    int scaling = 0;         // 0: Ruiz (the reference); 1: Ruiz then Pock-Chambolle (alpha 1)
     int exact = 0;           // 1: exact FP64 mode (reference summation orders, no FMA;
                              // precision 64, iteration-bounded runs only)
 };

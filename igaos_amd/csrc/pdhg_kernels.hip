@@ -417,9 +417,56 @@ HostSlices build_slices(int rows, int cols, const int* ptr, const int* idx, int 
     return h;
 }
 
+std::vector<int> build_xcd_map(const std::vector<int>& bslice, int S) {
+    std::vector<std::vector<int>> of_slice(S);
+    for (int k = 0; k < static_cast<int>(bslice.size()); ++k) of_slice[bslice[k]].push_back(k);
+    std::vector<std::vector<int>> lists(XCDS);
+    for (int sc = 0; sc < S; ++sc) {
+        std::vector<int> xs;  // XCDs serving slice sc
+        for (int x = 0; x < XCDS; ++x)
+            if ((S <= XCDS ? x % S : sc % XCDS) == (S <= XCDS ? sc : x)) xs.push_back(x);
+        const std::vector<int>& blk = of_slice[sc];
+        const size_t g = xs.size(), nb = blk.size();
+        for (size_t i = 0; i < g; ++i)
+            for (size_t k = nb * i / g; k < nb * (i + 1) / g; ++k) lists[xs[i]].push_back(blk[k]);
+    }
+    size_t len = 0;
+    for (const auto& l : lists) len = std::max(len, l.size());
+    std::vector<int> map(len * XCDS, -1);
+    for (int x = 0; x < XCDS; ++x)
+        for (size_t k = 0; k < lists[x].size(); ++k) map[k * XCDS + x] = lists[x][k];
+    return map;
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_primal_slices_x(IterBufs<V> b) {
+    __shared__ V s_prod[NNZ_MAX];
+    __shared__ V s_rows[ROWS_MAX];
+    __shared__ V red_v[BS / WAVE];
+    const LsState st = *b.st;
+    if (st.frozen || !st.fresh) return;  // a retry reuses the stored A'y
+    const int blk = b.sT.xmap[blockIdx.x];
+    if (blk < 0) return;
+    slice_rows(b.sT, blk, b.y[st.cur], s_prod, s_rows, red_v);
+}
+
+template <typename V>
+__global__ __launch_bounds__(BS) void k_dual_slices_x(IterBufs<V> b) {
+    __shared__ V s_prod[NNZ_MAX];
+    __shared__ V s_rows[ROWS_MAX];
+    __shared__ V red_v[BS / WAVE];
+    if (b.st->frozen) return;
+    const int blk = b.sA.xmap[blockIdx.x];
+    if (blk < 0) return;
+    slice_rows(b.sA, blk, b.xbar, s_prod, s_rows, red_v);
+}
+
 template <typename V>
 static void primal_impl(const IterBufs<V>& b, hipStream_t s) {
-    if (b.sT.S) {
+    if (b.sT.S && b.sT.xmap) {
+        if (b.sT.xgrid > 0) k_primal_slices_x<V><<<b.sT.xgrid, BS, 0, s>>>(b);
+        if (b.nT_part > 0) k_primal_epi<V><<<b.nT_part, BS, 0, s>>>(b);
+    } else if (b.sT.S) {
         for (int sc = 0; sc < b.sT.S; ++sc) {
             const int nb = b.sT.sb[sc + 1] - b.sT.sb[sc];
             if (nb > 0) k_primal_slices<V><<<nb, BS, 0, s>>>(b, b.sT.sb[sc]);
@@ -431,7 +478,10 @@ static void primal_impl(const IterBufs<V>& b, hipStream_t s) {
 }
 template <typename V>
 static void dual_impl(const IterBufs<V>& b, hipStream_t s) {
-    if (b.sA.S) {
+    if (b.sA.S && b.sA.xmap) {
+        if (b.sA.xgrid > 0) k_dual_slices_x<V><<<b.sA.xgrid, BS, 0, s>>>(b);
+        k_dual_epi<V><<<grid_of(b.nA_part), BS, 0, s>>>(b);
+    } else if (b.sA.S) {
         for (int sc = 0; sc < b.sA.S; ++sc) {
             const int nb = b.sA.sb[sc + 1] - b.sA.sb[sc];
             if (nb > 0) k_dual_slices<V><<<nb, BS, 0, s>>>(b, b.sA.sb[sc]);
@@ -722,6 +772,41 @@ void ruiz(int m, int n, const int* ap, const int* ai, const double* ax,
     }
     if (m > 0) k_fix_ones<<<elem_grid(m), BS, 0, s>>>(dr, m);
     if (n > 0) k_fix_ones<<<elem_grid(n), BS, 0, s>>>(dc, n);
+}
+
+// Pock-Chambolle (alpha = 1) on top of the Ruiz factors, opt-in: row and
+// column absolute sums of the currently scaled matrix, one thread per row /
+// column in storage order (the same sums as a sequential host loop), then
+// dr_i *= sqrt(r_i), dc_j /= sqrt(c_j).  The reference implements Ruiz only
+// (src/pdhg/pdhg.cu:469-492); its algorithm sheet pairs the two
+// (docs/research/pdlp-algorithm-sheet.md:62).
+__global__ __launch_bounds__(BS) void k_pc_sums(int rows, const int* ptr, const int* idx,
+                                                const double* val, const double* dr,
+                                                const double* dc, int by_col,
+                                                double* out) {
+    const long r = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (r >= rows) return;
+    double s = 0.0;
+    for (int p = ptr[r]; p < ptr[r + 1]; ++p) {
+        const double t = by_col ? fabs(val[p]) * dc[r] / dr[idx[p]]
+                                : fabs(val[p]) * dc[idx[p]] / dr[r];
+        s = s + t;
+    }
+    out[r] = s;
+}
+__global__ void k_pc_apply(int m, int n, const double* rs, const double* cs, double* dr,
+                           double* dc) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < m && rs[i] > 0 && __builtin_isfinite(rs[i])) dr[i] *= sqrt(rs[i]);
+    if (i < n && cs[i] > 0 && __builtin_isfinite(cs[i])) dc[i] /= sqrt(cs[i]);
+}
+void pock_chambolle(int m, int n, const int* ap, const int* ai, const double* ax,
+                    const int* cp, const int* ci, const double* acx, double* dr,
+                    double* dc, double* rs, double* cs, hipStream_t s) {
+    if (m > 0) k_pc_sums<<<elem_grid(m), BS, 0, s>>>(m, ap, ai, ax, dr, dc, 0, rs);
+    if (n > 0) k_pc_sums<<<elem_grid(n), BS, 0, s>>>(n, cp, ci, acx, dr, dc, 1, cs);
+    const long mx = m > n ? m : n;
+    if (mx > 0) k_pc_apply<<<elem_grid(mx), BS, 0, s>>>(m, n, rs, cs, dr, dc);
 }
 
 // CSR: out = val * dc[col] / dr[row];  CSC: out = val * dc[col] / dr[row]

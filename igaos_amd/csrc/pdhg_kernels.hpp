@@ -46,7 +46,20 @@ struct Sliced {
     const int4* quad = nullptr;
     const int* bslice = nullptr;
     V* part = nullptr;        // S * rows partial row sums
+    // XCD-pinned schedule: all slices run in one launch of xgrid
+    // workgroups; workgroup b (dispatched to XCD b % 8) takes block
+    // xmap[b] (-1: none), and every XCD only takes blocks of the slices
+    // assigned to it, so its L2 holds only those slices' range of the
+    // gathered vector
+    const int* xmap = nullptr;
+    int xgrid = 0;
 };
+constexpr int XCDS = 8;  // MI355X: 8 XCDs, workgroups dispatched round robin
+// Host: the XCD-pinned block map of a sliced matrix (see Sliced::xmap).
+// With S <= 8 slices, XCD x serves slice x % S (S = 1, 2, 4, 8 split the
+// XCDs evenly); with S > 8, XCD x serves slices x, x + 8, ...  A slice's
+// blocks are cut into contiguous runs, one per XCD serving it.
+std::vector<int> build_xcd_map(const std::vector<int>& bslice, int S);
 // dst[q] = V(src_val[src[q]]) for q < nnz (the sliced copy of the values).
 void gather_values(const double* src_val, const int* src, double* dst, long nnz,
                    hipStream_t s);
@@ -133,6 +146,11 @@ void launch_kkt(const KktData& d, const KktIter<float>& it,
 void ruiz(int m, int n, const int* ap, const int* ai, const double* ax,
           const int* cp, const int* ci, const double* acx, double* dr,
           double* dc, hipStream_t s);
+// Pock-Chambolle (alpha = 1) on top of the current factors (opt-in);
+// rs (m), cs (n) are scratch.
+void pock_chambolle(int m, int n, const int* ap, const int* ai, const double* ax,
+                    const int* cp, const int* ci, const double* acx, double* dr,
+                    double* dc, double* rs, double* cs, hipStream_t s);
 // Scaled data (pdhg.cu:499-513).
 void scale_data(int m, int n, const int* ap, const int* ai, const double* ax,
                 const int* cp, const int* ci, const double* acx,

@@ -97,6 +97,17 @@ typedef struct {
     int device;
     int column_slices; /* column slices of the A / A' products (gather
                           locality on large LPs): -1 auto, 0 off, 1..16 */
+    int xcd_pinned;    /* column slices (above): 1 = all slices in one
+                          launch, each of the 8 XCDs taking only blocks of
+                          its own slices (its L2 then holds only their range
+                          of the gathered vector); 0 (default) = one launch
+                          per slice over all XCDs, which measured faster on
+                          the 1M-column LP at every slice count but 16 */
+    int scaling;       /* 0 (default): Ruiz equilibration, 10 sweeps, as the
+                          reference (pdhg.cu:469-492); 1: Ruiz, then
+                          Pock-Chambolle diagonal preconditioning with
+                          alpha = 1 (the reference's algorithm sheet,
+                          docs/research/pdlp-algorithm-sheet.md:62) */
     int exact;         /* 1: exact FP64 mode -- every sparse row summed in
                           storage order, the line-search dots in the
                           reference's two-stage layout (pdhg.cu:63-125),

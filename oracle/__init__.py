@@ -38,7 +38,8 @@ class _Model(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [("time_limit_s", C.c_double), ("tolerance", C.c_double),
-                ("max_iterations", C.c_long), ("verbosity", C.c_int)]
+                ("max_iterations", C.c_long), ("verbosity", C.c_int),
+                ("scaling", C.c_int)]
 
 
 class Metrics(C.Structure):
@@ -85,6 +86,7 @@ def load():
         lib.orc_omega0.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
         lib.orc_spmv.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp]
         lib.orc_ruiz.argtypes = [C.POINTER(_Model), _dp, _dp]
+        lib.orc_pock_chambolle.argtypes = [C.POINTER(_Model), _dp, _dp]
         lib.orc_scale.argtypes = [C.POINTER(_Model), _dp, _dp] + [_dp] * 7
         lib.orc_dots.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         lib.orc_dual_repair.argtypes = [C.POINTER(_Model), _dp, _dp, _dp,
@@ -168,12 +170,13 @@ def set_threads(k: int):
 
 
 def solve(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
-          verbosity=0, trace=0) -> Result:
-    """Host FP64 replay of igaos::pdhg::solve (src/pdhg/pdhg.cu:457-1156)."""
+          verbosity=0, trace=0, scaling=0) -> Result:
+    """Host FP64 replay of igaos::pdhg::solve (src/pdhg/pdhg.cu:457-1156);
+    ``scaling=1`` adds Pock-Chambolle after Ruiz (orc_pock_chambolle)."""
     lib = load()
     h = _Held(lp)
     opt = _Options(float(time_limit), float(tolerance), int(max_iterations),
-                   int(verbosity))
+                   int(verbosity), int(scaling))
     x = np.zeros(lp.n)
     y = np.zeros(lp.m)
     r = _Result()
@@ -204,6 +207,16 @@ def ruiz(lp):
     dr = np.zeros(lp.m)
     dc = np.zeros(lp.n)
     lib.orc_ruiz(C.byref(h.model), _pd(dr), _pd(dc))
+    return dr, dc
+
+
+def pock_chambolle(lp, dr, dc):
+    """Pock-Chambolle (alpha = 1) factors on top of (dr, dc)."""
+    lib = load()
+    h = _Held(lp)
+    dr = np.ascontiguousarray(dr, dtype=np.float64).copy()
+    dc = np.ascontiguousarray(dc, dtype=np.float64).copy()
+    lib.orc_pock_chambolle(C.byref(h.model), _pd(dr), _pd(dc))
     return dr, dc
 
 

@@ -94,6 +94,38 @@ void orc_ruiz(const orc_model* lp, double* dr, double* dc) {
         if (!(dc[j] > 0) || !isfinite(dc[j])) dc[j] = 1.0;
 }
 
+/* Pock-Chambolle diagonal preconditioning with alpha = 1 (Pock and
+ * Chambolle, ICCV 2011, Lemma 2), applied after Ruiz as the reference's
+ * algorithm sheet prescribes (docs/research/pdlp-algorithm-sheet.md:62,141)
+ * but pdhg.cu does not implement (pdhg.cu:469-492 is Ruiz only): with the
+ * current factors, r_i = sum_j |a_ij| dc_j / dr_i and c_j = sum_i |a_ij|
+ * dc_j / dr_i, both from the same matrix, each summed in storage order;
+ * then dr_i *= sqrt(r_i), dc_j /= sqrt(c_j) (a zero or non-finite sum
+ * leaves the factor).  The scaled matrix then has spectral norm <= 1. */
+void orc_pock_chambolle(const orc_model* lp, double* dr, double* dc) {
+    const int m = lp->m, n = lp->n;
+    double* r = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    double* c = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < m; ++i) {
+        double s = 0.0;
+        for (int p = lp->ap[i]; p < lp->ap[i + 1]; ++p)
+            s += fabs(lp->ax[p]) * dc[lp->ai[p]] / dr[i];
+        r[i] = s;
+    }
+    for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int p = lp->cp[j]; p < lp->cp[j + 1]; ++p)
+            s += fabs(lp->acx[p]) * dc[j] / dr[lp->ci[p]];
+        c[j] = s;
+    }
+    for (int i = 0; i < m; ++i)
+        if (r[i] > 0 && isfinite(r[i])) dr[i] *= g_mut == 7 ? r[i] : sqrt(r[i]);
+    for (int j = 0; j < n; ++j)
+        if (c[j] > 0 && isfinite(c[j])) dc[j] /= sqrt(c[j]);
+    free(r);
+    free(c);
+}
+
 /* Scaled data, pdhg.cu:499-513 (gamma = 1, pdhg.cu:498). */
 void orc_scale(const orc_model* lp, const double* dr, const double* dc,
                double* sax, double* sacx, double* sc, double* scl,
@@ -564,6 +596,7 @@ int orc_pdhg_solve(const orc_model* lp, const orc_options* opt,
 
     double *dr = ALLOC(double, m), *dc = ALLOC(double, n);
     orc_ruiz(lp, dr, dc);
+    if (opt->scaling == 1) orc_pock_chambolle(lp, dr, dc);
     double *sax = ALLOC(double, nnz), *sacx = ALLOC(double, nnz);
     double *sc = ALLOC(double, n), *scl = ALLOC(double, n),
            *scu = ALLOC(double, n), *srmin = ALLOC(double, m),

@@ -25,6 +25,7 @@ typedef struct {
     double tolerance;
     long max_iterations;
     int verbosity;
+    int scaling; /* 0: Ruiz (the reference); 1: Ruiz, then Pock-Chambolle */
 } orc_options;
 
 /* pdhg.cu:159-170 (Metrics) */
@@ -53,6 +54,7 @@ void orc_set_threads(int k);
 void orc_spmv(int nrows, const int* ptr, const int* idx, const double* val,
               const double* x, double* y);
 void orc_ruiz(const orc_model* lp, double* dr, double* dc);
+void orc_pock_chambolle(const orc_model* lp, double* dr, double* dc);
 void orc_scale(const orc_model* lp, const double* dr, const double* dc,
                double* sax, double* sacx, double* sc, double* scl,
                double* scu, double* srmin, double* srmax);

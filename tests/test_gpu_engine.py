@@ -310,6 +310,20 @@ def test_column_sliced_products_keep_parity(slices):
         assert abs(got.objective - ref.objective) <= 1e-6 * (1 + abs(ref.objective))
 
 
+@pytest.mark.parametrize("slices", [3, 8, 16])
+def test_xcd_pinned_slices_sum_like_per_slice_launches(slices):
+    # one launch with every XCD on its own slices (igaos_engine_config.
+    # xcd_pinned) runs the same blocks with the same partial sums as one
+    # launch per slice: the runs are identical bit for bit
+    lp = lpgen.gen_zipf_badly_scaled(seed=4, m=6_000, n=20_000, kmax=600)
+    kw = dict(time_limit=1e7, tolerance=0.0, max_iterations=128, column_slices=slices)
+    for prec in (64, 32):
+        a = igaos_amd.solve_lp(lp, precision=prec, xcd_pinned=False, **kw)
+        b = igaos_amd.solve_lp(lp, precision=prec, xcd_pinned=True, **kw)
+        assert a.info["attempts"] == b.info["attempts"]
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y), prec
+
+
 def test_reference_recorded_pdhg_battery():
     # the reference's own PDHG runs at its default tolerance 1e-4 (recorded,
     # not re-run): same status, and an objective within the tolerance band

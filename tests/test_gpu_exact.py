@@ -95,6 +95,29 @@ def test_exact_long_runs_with_check_time_paths(name, tol, maxit, event):
     _assert_identical(ref, got)
 
 
+@pytest.mark.parametrize("name", ["afiro", "sc105", "share2b"])
+def test_exact_pock_chambolle_scaling(name):
+    # the opt-in Ruiz + Pock-Chambolle preconditioning (device factors from
+    # sequential row / column sums) against the oracle's, bit for bit
+    lp = _netlib(name)
+    ref = oracle.solve(lp, time_limit=1e7, tolerance=1e-6, max_iterations=20000, scaling=1)
+    got = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=1e-6, max_iterations=20000,
+                             exact=True, scaling="ruiz+pock-chambolle")
+    _assert_identical(ref, got)
+
+
+@pytest.mark.parametrize("name", ["afiro", "sc50a", "adlittle", "sc205"])
+def test_pock_chambolle_solves_certify_to_highs(name, objectives):
+    # the default (fast) kernels with the opt-in scaling: certified at 1e-6
+    # and within 1e-6 of the reference's recorded HiGHS optimum
+    ref = objectives[f"netlib/{name}"]["highs_ref"]
+    for prec in (64, 32):
+        s = igaos_amd.solve_lp(_netlib(name), time_limit=120, tolerance=1e-6, precision=prec,
+                               scaling="ruiz+pock-chambolle")
+        assert s.status == "near-optimal", (prec, s.message)
+        assert abs(s.objective - ref) <= 2e-6 * (1 + abs(ref)), (prec, s.objective, ref)
+
+
 def test_exact_mode_refuses_time_limited_runs():
     # a wall-clock budget makes MAXIT 2e7 (pdhg.cu:669-671), beyond the
     # exponent table: the engine says so instead of running

@@ -112,6 +112,42 @@ def test_ruiz_mutation_is_caught():
     assert not np.allclose(d * dc / dr, d ** (4.0 ** -10), rtol=1e-6)
 
 
+def _pc_scaled(lp, dr, dc):
+    return (sp.diags(1 / dr) @ lp.to_scipy() @ sp.diags(dc)).toarray()
+
+
+def test_pock_chambolle_rank_one_has_unit_norm():
+    # Pock & Chambolle (ICCV 2011, Lemma 2), alpha = 1: with row sums r and
+    # column sums c of |A|, A_ij / sqrt(r_i c_j) has spectral norm <= 1; on
+    # a positive rank-one A = u v' it is s t' with |s| = |t| = 1, norm 1
+    rng = np.random.default_rng(4)
+    u, v = rng.uniform(0.1, 10, 7), rng.uniform(0.01, 100, 11)
+    A = np.outer(u, v)
+    rows, cols = np.nonzero(A)
+    lp = lpgen.LP.from_triplets(7, 11, rows, cols, A[rows, cols], np.ones(11), np.zeros(11),
+                                np.ones(11), np.ones(7), np.ones(7))
+    dr, dc = oracle.pock_chambolle(lp, np.ones(7), np.ones(11))
+    assert abs(np.linalg.norm(_pc_scaled(lp, dr, dc), 2) - 1.0) < 1e-13
+    oracle.set_mutation(7)  # dr *= r instead of sqrt(r)
+    dr, dc = oracle.pock_chambolle(lp, np.ones(7), np.ones(11))
+    assert abs(np.linalg.norm(_pc_scaled(lp, dr, dc), 2) - 1.0) > 1e-3
+
+
+def test_pock_chambolle_diagonal_and_norm_bound():
+    # diagonal: every scaled entry becomes +-1 exactly up to rounding
+    d = np.array([4.0, -1e-3, 25.0, 1e6])
+    lp = lpgen.LP.from_triplets(4, 4, range(4), range(4), d, np.ones(4), np.zeros(4),
+                                np.ones(4), np.ones(4), np.ones(4))
+    dr, dc = oracle.pock_chambolle(lp, np.ones(4), np.ones(4))
+    np.testing.assert_allclose(np.diag(_pc_scaled(lp, dr, dc)), np.sign(d), rtol=1e-15)
+    # after Ruiz (as the engine applies it), on badly scaled sparse LPs
+    for seed in range(3):
+        lp = lpgen.gen_zipf_badly_scaled(seed=seed, m=120, n=300)
+        dr0, dc0 = oracle.ruiz(lp)
+        dr, dc = oracle.pock_chambolle(lp, dr0, dc0)
+        assert np.linalg.norm(_pc_scaled(lp, dr, dc), 2) <= 1.0 + 1e-12
+
+
 def test_scale_is_diag_similarity():
     lp = lpgen.gen_random_eq(seed=5, m=40, n=90)
     dr, dc = oracle.ruiz(lp)
@@ -204,6 +240,21 @@ def test_line_search_properties():
     # k = 1: (1 - 1^-0.3) = 0 so an accepted step keeps eta
     acc, e, _ = oracle.ls_decide(1, 0.5, omega, dots)
     assert acc and e == 0.5
+
+
+def test_line_search_exponents_closed_form():
+    # PDLP's adaptive step (Applegate et al. 2021, Algorithm 2):
+    # eta' = min((1 - k^-0.3) etabar, (1 + k^-0.6) eta).  At k = 1024 the
+    # factors are exactly 1 - 2^-3 and 1 + 2^-6, at k = 2^20 exactly
+    # 1 - 2^-6 and 1 + 2^-12; dots (1, 1, 1) with omega 1 give etabar = 1.
+    dots = [1.0, 1.0, 1.0]
+    for k, a, b in ((1024, 2.0 ** -3, 2.0 ** -6), (1 << 20, 2.0 ** -6, 2.0 ** -12)):
+        acc, e, _ = oracle.ls_decide(k, 0.5, 1.0, dots)   # accepted, growth binds
+        assert acc and e == 0.5 * (1 + b)
+        acc, e, _ = oracle.ls_decide(k, 0.99, 1.0, dots)  # accepted, etabar binds
+        assert acc and e == min(1 - a, 0.99 * (1 + b))
+        acc, e, _ = oracle.ls_decide(k, 2.0, 1.0, dots)   # rejected
+        assert not acc and e == 1 - a
 
 
 # ------------------------------------------------------------- evaluate --
