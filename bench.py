@@ -26,6 +26,9 @@ mode: summed over every instance's attempts and divided by the launch's
 duration.  ``traffic`` comes from the committed rocprofv3 --pmc summary
 when present.
 
+``--workload c3`` solves the block-structured multi-period LP (96 periods,
+m = 480,000, n = 768,000) once per step in FP32 + FP64 refinement and
+times its attempt kernels.
 ``--workload c1`` solves the reference's CPU-runnable LP (m = 1,000,
 n = 3,000) with the single-LP engine and reports iterations/s.
 ``--workload zipf`` instead measures the single-LP attempt kernels on the
@@ -544,6 +547,97 @@ def bench_zipf(args):
     }
 
 
+def bench_c3(args):
+    """The block-structured multi-period blending LP (96 periods of 5,000
+    rows x 8,000 columns, 6 nonzeros per column, 1,000 inventory columns
+    per period linked to the next period, bounds [0, 10]): one FP32 + FP64
+    refinement solve to 1e-6 (a step), its iterations/s, and the attempt
+    kernels' roofline from timed attempts in FP32 and FP64."""
+    import numpy as np
+    import igaos_amd
+    from igaos_amd import lpgen
+
+    t = time.perf_counter()
+    lp = lpgen.gen_multiperiod(seed=0)
+    log(f"[bench] multi-period LP m={lp.m} n={lp.n} nnz={lp.nnz} generated in "
+        f"{time.perf_counter() - t:.1f} s")
+    kern = {}
+    for prec in (64, 32):
+        e = igaos_amd.Engine(lp, precision=prec)
+        tm = e.profile_attempts(args.attempts)
+        e.close()
+        w = 8 if prec == 64 else 4
+        bp = lp.nnz * (w + 4) + 4 * (lp.n + 1) + 7 * lp.n * w
+        bd = lp.nnz * (w + 4) + 4 * (lp.m + 1) + 8 * lp.m * w
+        kern[prec] = dict(ms_primal=tm["ms_primal"], ms_dual=tm["ms_dual"],
+                          gbs_primal=bp / tm["ms_primal"] / 1e6,
+                          gbs_dual=bd / tm["ms_dual"] / 1e6,
+                          bytes_primal=bp, bytes_dual=bd)
+        log(f"[bench] FP{prec}: primal {tm['ms_primal']:.3f} ms ({kern[prec]['gbs_primal']:.0f} "
+            f"GB/s), dual {tm['ms_dual']:.3f} ms ({kern[prec]['gbs_dual']:.0f} GB/s)")
+    e = igaos_amd.Engine(lp, precision=32)
+    setup_ms = e.setup_ms
+    solves = []
+    for k in range(args.warmup + args.steps):
+        s = e.run(time_limit=args.time_limit, tolerance=args.tolerance)
+        log(f"[bench] FP32+refine solve {k}: {s.status} {s.iterations} it "
+            f"{s.solve_time_ms:.0f} ms obj {s.objective:.10g}")
+        if k >= args.warmup:
+            solves.append(s)
+    e.close()
+    ms_tot = sum(s.solve_time_ms for s in solves)
+    solved = sum(s.status == "near-optimal" for s in solves)
+    its = sum(s.iterations for s in solves)
+    k32 = kern[32]
+    dom = "primal" if k32["ms_primal"] >= k32["ms_dual"] else "dual"
+    achieved = k32[f"bytes_{dom}"] / k32[f"ms_{dom}"] / 1e6
+    cpu = cpu_par = None
+    if not args.no_cpu_baseline:
+        import oracle
+        t0 = time.perf_counter()
+        r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=64)
+        wall = time.perf_counter() - t0
+        cpu = {"value": r.iterations / (r.solve_time_ms * 1e-3), "unit": "iterations/s",
+               "cores": 1, "kind": "port",
+               "sample": (f"oracle host replay (1 thread): setup + 64 iterations of the "
+                          f"same LP, {wall:.1f} s wall")}
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        try:
+            t0 = time.perf_counter()
+            r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=64)
+            wall = time.perf_counter() - t0
+        finally:
+            oracle.set_threads(1)
+        cpu_par = {"value": r.iterations / (r.solve_time_ms * 1e-3), "unit": "iterations/s",
+                   "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+                   "sample": (f"oracle host replay, sparse products row-split over "
+                              f"{threads} threads: setup + 64 iterations, {wall:.1f} s wall")}
+    last = solves[-1]
+    return {
+        "metric": METRIC, "value": solved / (ms_tot * 1e-3), "unit": "LPs/s", "n_gpus": 1,
+        "steps": len(solves), "warmup": args.warmup, "ms_per_step": ms_tot / len(solves),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64refine", "data": "synthetic",
+        "config": {"workload": (f"multi-period blending LP, 96 periods x (5000 x 8000), "
+                                f"m={lp.m} n={lp.n} nnz={lp.nnz}, seed 0"),
+                   "tolerance": args.tolerance, "time_limit_s": args.time_limit},
+        "solve": {"status": last.status, "iterations": last.iterations,
+                  "iterations_per_s": its / (ms_tot * 1e-3), "objective": last.objective,
+                  "pinf": last.pinf, "dinf": last.dinf, "rel_gap": last.rel_gap,
+                  "setup_ms": setup_ms,
+                  "note": "solve_time_ms excludes setup, as the reference reports it"},
+        "attempt_kernels": kern,
+        "roofline": {"bound": "hbm", "kernel": f"FP32 {dom} step",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "bytes_per_launch": k32[f"bytes_{dom}"],
+                     "ms_per_launch": round(k32[f"ms_{dom}"], 4)},
+        "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_par,
+    }
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -628,7 +722,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--workload", default="batch", choices=["batch", "zipf", "diff", "c1"])
+    ap.add_argument("--workload", default="batch",
+                    choices=["batch", "zipf", "diff", "c1", "c3"])
     ap.add_argument("--count", type=int, default=8192)
     ap.add_argument("--mode", default="auto", choices=["auto", "lockstep", "persistent"],
                     help="batched execution mode (igaos_batch_set_mode); auto: "
@@ -672,7 +767,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     line = {"batch": bench_batch, "zipf": bench_zipf, "diff": bench_diff,
-            "c1": bench_c1}[args.workload](args)
+            "c1": bench_c1, "c3": bench_c3}[args.workload](args)
     print(json.dumps(line), flush=True)
 
 

@@ -19,6 +19,7 @@ from .api import (  # noqa: F401
     read_mps,
     solve,
     solve_lp,
+    model_from_csr,
     model_from_lp,
     Engine,
 )

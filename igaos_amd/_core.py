@@ -60,10 +60,16 @@ SIGNATURES = {
     "igaos_model_from_csr_csc": (_vp, [C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                        _dp, _ip, _ip, _dp, _ip, _ip, _dp,
                                        C.c_double]),
+    "igaos_model_from_csr": (_vp, [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip,
+                                   _dp, C.c_double]),
     "igaos_model_free": (None, [_vp]),
     "igaos_model_row_activity": (C.c_int, [_vp, _dp, _dp]),
     "igaos_dual_repair": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, _dp]),
     "igaos_kkt_evaluate": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
+    "igaos_pdhg_polish": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp,
+                                    _vp]),
+    "igaos_crossover_basis": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_int, _ip,
+                                        C.POINTER(C.c_ubyte)]),
     "igaos_model_dims": (None, [_vp, _ip, _ip, _ip, _dp, _ip, _ip, _ip]),
     "igaos_model_get": (None, [_vp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp,
                                _ip, _ip, _dp]),
@@ -90,6 +96,8 @@ SIGNATURES = {
                                      _vp]),
     "igaos_ruiz_f64": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp,
                                  _vp, _vp, _vp, _vp]),
+    "igaos_csr_to_csc_f64": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp,
+                                       _vp, _vp]),
     "igaos_ls_dots_work": (C.c_long, [C.c_int, C.c_int]),
     "igaos_ls_dots_f64": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "igaos_ls_dots_f64_noalloc": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),

@@ -177,6 +177,59 @@ def model_from_lp(lp) -> Model:
     return Model(h)
 
 
+def model_from_csr(m, n, c, cl, cu, rmin, rmax, ap, ai, ax, obj_const=0.0) -> Model:
+    """A Model from CSR arrays alone; the CSC is built on the GPU
+    (igaos_model_from_csr), each column's entries in row order."""
+    d = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    i = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+    arrs = [d(c), d(cl), d(cu), d(rmin), d(rmax), i(ap), i(ai), d(ax)]
+    ptrs = [a.ctypes.data_as(C.POINTER(C.c_int)) if a.dtype == np.int32
+            else a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]
+    h = _core.lib().igaos_model_from_csr(int(m), int(n), *ptrs, float(obj_const))
+    if not h:
+        raise ValueError(_core.last_error())
+    return Model(h)
+
+
+def polish(model: Model, x, y, tolerance=1e-12, max_iterations=None):
+    """Active-set polishing of one LP's solution (igaos_pdhg_polish):
+    returns (x, y, objective, info); info["accepted"] says whether the
+    polished vertex replaced the input pair."""
+    from ._batch_sigs import AdjointInfo
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    if x.shape != (model.n,) or y.shape != (model.m,):
+        raise ValueError(f"x must have {model.n} entries and y {model.m}")
+    if max_iterations is None:
+        max_iterations = int(min(200000, 20 * (model.m + model.n)))
+    xo, yo, ob = np.zeros(model.n), np.zeros(model.m), C.c_double()
+    info = AdjointInfo()
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    if _core.lib().igaos_pdhg_polish(model._h, dp(x), dp(y), float(tolerance),
+                                     int(max_iterations), dp(xo), dp(yo), C.byref(ob),
+                                     C.byref(info)) != 0:
+        raise RuntimeError(_core.last_error())
+    return xo, yo, ob.value, {k: getattr(info, k) for k, _ in AdjointInfo._fields_}
+
+
+def crossover_basis(model: Model, x, y, tol=1e-6, fill=True):
+    """Starting basis for the reference's revised simplex from a
+    near-optimal (x, y) (igaos_crossover_basis): returns (basis, nb_state,
+    count) in the simplex's WarmStart layout."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    basis = np.zeros(model.m, np.int32)
+    st = np.zeros(model.n + model.m, np.uint8)
+    k = _core.lib().igaos_crossover_basis(
+        model._h, x.ctypes.data_as(C.POINTER(C.c_double)),
+        y.ctypes.data_as(C.POINTER(C.c_double)), float(tol), int(bool(fill)),
+        basis.ctypes.data_as(C.POINTER(C.c_int)),
+        st.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    if k < 0:
+        raise ValueError(_core.last_error())
+    return basis, st, k
+
+
 def _run(model: Model, opt: Options, cfg: EngineConfig | None,
          engine=None) -> Solution:
     L = _core.lib()

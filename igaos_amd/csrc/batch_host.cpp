@@ -534,8 +534,9 @@ igaos_batch* igaos_batch_create(int count, const int* m, const int* n,
         auto work = [&]() {
             for (int i = next++; i < count; i = next++) {
                 try {
-                    bad[i] = igaos::io::check_csr_csc(m[i], n[i], ap + rpo[i], ai + zo[i],
-                                                      cp + cpo[i], ci + zo[i]);
+                    bad[i] = igaos::io::check_transpose(m[i], n[i], ap + rpo[i], ai + zo[i],
+                                                        ax + zo[i], cp + cpo[i], ci + zo[i],
+                                                        acx + zo[i]);
                 } catch (const std::exception& e) {
                     bad[i] = e.what();
                 }

@@ -14,6 +14,7 @@
 #include "model.hpp"
 #include "pdhg.hpp"
 #include "exact.hpp"
+#include "transpose.hpp"
 #include "pdhg_kernels.hpp"
 
 struct igaos_model {
@@ -127,7 +128,7 @@ igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
         fail("invalid sizes or pointers");
         return nullptr;
     }
-    const std::string bad = igaos::io::check_csr_csc(m, n, ap, ai, cp, ci);
+    std::string bad = igaos::io::check_csr_csc(m, n, ap, ai, cp, ci);
     if (!bad.empty()) {
         fail(bad);
         return nullptr;
@@ -136,6 +137,11 @@ igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
     if ((nnz > 0 && (!ax || !acx)) || (n > 0 && (!c || !cl || !cu)) ||
         (m > 0 && (!rmin || !rmax))) {
         fail("null data array");
+        return nullptr;
+    }
+    bad = igaos::io::check_transpose(m, n, ap, ai, ax, cp, ci, acx);
+    if (!bad.empty()) {
+        fail(bad);
         return nullptr;
     }
     try {
@@ -156,6 +162,83 @@ igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
         md.acx.assign(acx, acx + nnz);
         md.integ.assign(n, 0);
         md.obj_const = obj_const;
+        return h.release();
+    } catch (const std::exception& e) {
+        fail(e.what());
+        return nullptr;
+    }
+}
+
+igaos_model* igaos_model_from_csr(int m, int n, const double* c, const double* cl,
+                                  const double* cu, const double* rmin,
+                                  const double* rmax, const int* ap, const int* ai,
+                                  const double* ax, double obj_const) {
+    const std::string bad = igaos::io::check_compressed(m, n, ap, ai);
+    if (!bad.empty()) {
+        fail("CSR: " + bad);
+        return nullptr;
+    }
+    const int nnz = ap[m];
+    if ((nnz > 0 && !ax) || (n > 0 && (!c || !cl || !cu)) || (m > 0 && (!rmin || !rmax))) {
+        fail("null data array");
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        fail("igaos_model_from_csr builds the CSC on a HIP device; none is visible");
+        return nullptr;
+    }
+    struct Buf {  // device arrays, freed on every exit
+        void* p = nullptr;
+        ~Buf() {
+            if (p) (void)hipFree(p);
+        }
+    } d_ap, d_ai, d_ax, d_cp, d_ci, d_acx;
+    try {
+        auto h = std::make_unique<igaos_model>();
+        auto& md = h->m;
+        md.m = m;
+        md.n = n;
+        md.c.assign(c, c + n);
+        md.cl.assign(cl, cl + n);
+        md.cu.assign(cu, cu + n);
+        md.rmin.assign(rmin, rmin + m);
+        md.rmax.assign(rmax, rmax + m);
+        md.ap.assign(ap, ap + m + 1);
+        md.ai.assign(ai, ai + nnz);
+        md.ax.assign(ax, ax + nnz);
+        md.cp.resize(static_cast<size_t>(n) + 1);
+        md.ci.resize(nnz);
+        md.acx.resize(nnz);
+        md.integ.assign(n, 0);
+        md.obj_const = obj_const;
+        auto chk = [](hipError_t e) {
+            if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+        };
+        const size_t zi = sizeof(int) * (nnz > 0 ? nnz : 1);
+        const size_t zd = sizeof(double) * (nnz > 0 ? nnz : 1);
+        chk(hipMalloc(&d_ap.p, sizeof(int) * (static_cast<size_t>(m) + 1)));
+        chk(hipMalloc(&d_ai.p, zi));
+        chk(hipMalloc(&d_ax.p, zd));
+        chk(hipMalloc(&d_cp.p, sizeof(int) * (static_cast<size_t>(n) + 1)));
+        chk(hipMalloc(&d_ci.p, zi));
+        chk(hipMalloc(&d_acx.p, zd));
+        chk(hipMemcpy(d_ap.p, ap, sizeof(int) * (static_cast<size_t>(m) + 1), hipMemcpyHostToDevice));
+        if (nnz > 0) {
+            chk(hipMemcpy(d_ai.p, ai, zi, hipMemcpyHostToDevice));
+            chk(hipMemcpy(d_ax.p, ax, zd, hipMemcpyHostToDevice));
+        }
+        igaos::dev::csr_to_csc(m, n, nnz, static_cast<int*>(d_ap.p), static_cast<int*>(d_ai.p),
+                               static_cast<double*>(d_ax.p), static_cast<int*>(d_cp.p),
+                               static_cast<int*>(d_ci.p), static_cast<double*>(d_acx.p),
+                               nullptr);
+        chk(hipMemcpy(md.cp.data(), d_cp.p, sizeof(int) * (static_cast<size_t>(n) + 1),
+                      hipMemcpyDeviceToHost));
+        if (nnz > 0) {
+            chk(hipMemcpy(md.ci.data(), d_ci.p, zi, hipMemcpyDeviceToHost));
+            chk(hipMemcpy(md.acx.data(), d_acx.p, zd, hipMemcpyDeviceToHost));
+        }
         return h.release();
     } catch (const std::exception& e) {
         fail(e.what());
@@ -407,6 +490,46 @@ int igaos_pdhg_profile_attempts(igaos_engine* engine, int attempts,
     return 0;
 }
 
+int igaos_pdhg_polish(const igaos_model* model, const double* x, const double* y,
+                      double tolerance, int max_iterations, double* x_out, double* y_out,
+                      double* objective, igaos_adjoint_info* info) {
+    if (!model) return fail("null model");
+    const auto& md = model->m;
+    if ((md.n > 0 && !x) || (md.m > 0 && !y)) return fail("null array");
+    int m = md.m, n = md.n, nnz = md.nnz();
+    const double oc = md.obj_const;
+    struct Batch {  // destroyed on every exit
+        igaos_batch* b = nullptr;
+        ~Batch() {
+            if (b) igaos_batch_destroy(b);
+        }
+    } B;
+    B.b = igaos_batch_create(1, &m, &n, &nnz, md.c.data(), md.cl.data(), md.cu.data(),
+                             md.rmin.data(), md.rmax.data(), md.ap.data(), md.ai.data(),
+                             md.ax.data(), md.cp.data(), md.ci.data(), md.acx.data(), &oc, 64);
+    if (!B.b) return fail(igaos_batch_last_error());
+    // a zero-iteration run computes the scaling the active sets are read in
+    igaos_options o;
+    igaos_options_default(&o);
+    o.time_limit_s = 1e7;
+    o.max_iterations = 0;
+    o.tolerance = 0.0;
+    if (igaos_batch_run(B.b, &o, nullptr, nullptr, nullptr) != 0 ||
+        igaos_batch_set_solution(B.b, x, y) != 0 ||
+        igaos_batch_polish(B.b, tolerance, max_iterations, x_out, y_out, objective, info) != 0)
+        return fail(igaos_batch_last_error());
+    return 0;
+}
+
+int igaos_crossover_basis(const igaos_model* model, const double* x, const double* y,
+                          double tol, int fill, int* basis, unsigned char* nb_state) {
+    if (!model || !basis || !nb_state) return fail("null model or output");
+    const auto& md = model->m;
+    if ((md.n > 0 && !x) || (md.m > 0 && !y)) return fail("null array");
+    if (!(tol >= 0)) return fail("tolerance must be >= 0");
+    return igaos::pdhg::crossover_basis(md, x, y, tol, basis, nb_state, fill != 0);
+}
+
 int igaos_kkt_evaluate(const igaos_model* model, const double* x, const double* y,
                        const double* Ax, const double* d, double* out) {
     if (!model || !out) return fail("null model or output");
@@ -497,6 +620,19 @@ int igaos_ruiz_f64(int m, int n, const int* ap, const int* ai,
                      static_cast<hipStream_t>(stream));
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : fail(hipGetErrorString(e));
+}
+
+int igaos_csr_to_csc_f64(int m, int n, int nnz, const int* ap, const int* ai,
+                         const double* ax, int* cp, int* ci, double* acx, void* stream) {
+    if (m < 0 || n < 0 || nnz < 0) return fail("negative size");
+    if (!ap || !cp || (nnz > 0 && (!ai || !ax || !ci || !acx))) return fail("null device array");
+    try {
+        igaos::dev::csr_to_csc(m, n, nnz, ap, ai, ax, cp, ci, acx,
+                               static_cast<hipStream_t>(stream));
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+    return 0;
 }
 
 long igaos_ls_dots_work(int n, int m) {

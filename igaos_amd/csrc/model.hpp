@@ -65,5 +65,10 @@ std::string check_compressed(int rows, int cols, const int* ptr, const int* idx)
 // in every column.
 std::string check_csr_csc(int m, int n, const int* ap, const int* ai, const int* cp,
                           const int* ci);
+// check_csr_csc, and the CSC holds exactly the CSR's entries: per column
+// the same (row, value) pairs, in any order (O(nnz); a column listed in
+// row order is compared directly).
+std::string check_transpose(int m, int n, const int* ap, const int* ai, const double* ax,
+                            const int* cp, const int* ci, const double* acx);
 
 }  // namespace igaos::io

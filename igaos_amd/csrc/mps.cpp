@@ -9,6 +9,7 @@
 // file is read whole and tokenised in place (string views into the buffer,
 // name tables keyed by those views, numbers converted where they lie)
 // instead of going through an istringstream and string copies per line.
+#include <algorithm>
 #include <charconv>
 #include <cerrno>
 #include <cmath>
@@ -390,6 +391,50 @@ std::string check_csr_csc(int m, int n, const int* ap, const int* ai, const int*
     for (int i = 0; i < m; ++i)
         if (per_row[i] != ap[i + 1] - ap[i])
             return "CSR and CSC differ in the entries of row " + std::to_string(i);
+    return {};
+}
+
+std::string check_transpose(int m, int n, const int* ap, const int* ai, const double* ax,
+                            const int* cp, const int* ci, const double* acx) {
+    std::string e = check_csr_csc(m, n, ap, ai, cp, ci);
+    if (!e.empty()) return e;
+    const int nz = ap[m];
+    // the CSR's transpose with each column's entries in row order
+    std::vector<int> fill(cp, cp + n), tr(nz);
+    std::vector<double> tv(nz);
+    for (int i = 0; i < m; ++i)
+        for (int p = ap[i]; p < ap[i + 1]; ++p) {
+            const int q = fill[ai[p]]++;
+            tr[q] = i;
+            tv[q] = ax[p];
+        }
+    auto same = [](double a, double b) { return a == b || (a != a && b != b); };
+    std::vector<std::pair<int, double>> u, w;
+    for (int j = 0; j < n; ++j) {
+        bool sorted = true;
+        for (int q = cp[j] + 1; q < cp[j + 1] && sorted; ++q) sorted = ci[q - 1] <= ci[q];
+        bool ok = true;
+        if (sorted) {
+            // equal rows (duplicates) may come in either order: compare
+            // runs of one row as sorted value lists below if they differ
+            for (int q = cp[j]; q < cp[j + 1] && ok; ++q)
+                ok = ci[q] == tr[q] && same(acx[q], tv[q]);
+        }
+        if (!sorted || !ok) {
+            u.clear();
+            w.clear();
+            for (int q = cp[j]; q < cp[j + 1]; ++q) {
+                u.emplace_back(ci[q], acx[q]);
+                w.emplace_back(tr[q], tv[q]);
+            }
+            std::sort(u.begin(), u.end());
+            std::sort(w.begin(), w.end());
+            ok = true;
+            for (size_t k = 0; k < u.size() && ok; ++k)
+                ok = u[k].first == w[k].first && same(u[k].second, w[k].second);
+        }
+        if (!ok) return "CSC is not the transpose of CSR in column " + std::to_string(j);
+    }
     return {};
 }
 

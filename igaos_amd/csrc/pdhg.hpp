@@ -85,4 +85,8 @@ HostMetrics host_evaluate(const io::Model& lp, const double* x,
 bool host_dual_repair(const io::Model& lp, const double* x, const double* Ax,
                       const double* y_it, double col_tol, double* y);
 
+// PDHG -> simplex crossover basis (crossover.cpp); see igaos_crossover_basis.
+int crossover_basis(const io::Model& md, const double* x, const double* y, double tol,
+                    int* basis, unsigned char* nb_state, bool fill);
+
 }  // namespace igaos::pdhg

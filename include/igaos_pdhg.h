@@ -52,7 +52,8 @@ igaos_model* igaos_model_from_triplets(int m, int n, int nnz,
                                        double obj_const);
 /* Builds a model from ready CSR + CSC arrays (copied).  Returns NULL when
  * a pointer array does not start at 0 or decreases, an index is out of
- * range, or CSR and CSC disagree in an entry count of a row or column. */
+ * range, or the CSC is not the CSR's transpose: every column must hold the
+ * same (row, value) pairs in both, in any order (checked in O(nnz)). */
 igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
                                       const double* cl, const double* cu,
                                       const double* rmin, const double* rmax,
@@ -60,6 +61,16 @@ igaos_model* igaos_model_from_csr_csc(int m, int n, const double* c,
                                       const double* ax, const int* cp,
                                       const int* ci, const double* acx,
                                       double obj_const);
+/* Builds a model from CSR arrays alone (copied): the CSC is built on the
+ * GPU (igaos_csr_to_csc_f64) instead of by the host's bucket sort
+ * (counting_sort, src/io/model.hpp:39-67), with each column's entries in
+ * row order -- counting_sort's order whenever triplets arrive row by row.
+ * Needs a visible HIP device.  Returns NULL on a malformed CSR or a HIP
+ * error (igaos_last_error). */
+igaos_model* igaos_model_from_csr(int m, int n, const double* c, const double* cl,
+                                  const double* cu, const double* rmin,
+                                  const double* rmax, const int* ap, const int* ai,
+                                  const double* ax, double obj_const);
 void igaos_model_free(igaos_model* model);
 /* Host: row activity A x (m entries) of a point x (n entries), each row
  * summed in FP64 from 0 in CSR order -- how the reference simplex fills
@@ -356,6 +367,34 @@ int igaos_kkt_evaluate(const igaos_model* model, const double* x, const double* 
 int igaos_dual_repair(const igaos_model* model, const double* x, const double* Ax,
                       const double* y_it, double col_tol, double* y);
 
+/* Active-set polishing of a single LP's solution (no reference
+ * counterpart; the batched engine's igaos_batch_polish on a batch of one):
+ * from x (n), y (m) in original units it reads the active sets off
+ * complementarity, puts x_N on its bounds and y off the active rows to
+ * zero, and solves A_RB x_B = b_R - A_RN x_N and A_RB' y_R = -c_B by CGLS in
+ * FP64 on the GPU.  x_out, y_out and objective receive the polished pair if
+ * it is primal feasible, has d_B = 0 and correct reduced-cost and dual
+ * signs to 1e-9 relative (info->accepted = 1), else the input pair.  A
+ * 1e-6 PDHG solution then becomes the vertex it approximates, exact to
+ * FP64 rounding.  Any output may be NULL.  0 on success. */
+int igaos_pdhg_polish(const igaos_model* model, const double* x, const double* y,
+                      double tolerance, int max_iterations, double* x_out, double* y_out,
+                      double* objective, igaos_adjoint_info* info);
+
+/* Host: PDHG -> simplex crossover.  From a near-optimal pair x (n), y (m;
+ * this engine's sign, d = c + A'y) of the model, a starting basis for the
+ * reference's revised simplex in its WarmStart layout
+ * (src/simplex/simplex.hpp:10-15; variables: columns [0, n), row slacks
+ * n + i valued (A x)_i in [rmin_i, rmax_i]): basis (m) receives the basic
+ * variable of each slot or -1 (the simplex puts an artificial there),
+ * nb_state (n + m) each variable's state (0 at lower, 1 at upper, 2 basic,
+ * 3 free nonbasic).  A variable within tol of a bound (relative to 1 +
+ * |bound|) is nonbasic there unless its reduced cost is zero to tol; the
+ * most interior of the rest fill the basis.  Returns the number of basic
+ * variables placed (at most m), or -1 (igaos_last_error). */
+int igaos_crossover_basis(const igaos_model* model, const double* x, const double* y,
+                          double tol, int fill, int* basis, unsigned char* nb_state);
+
 /* Host: row-block layout of a CSR row pointer (host) for the SpMV
  * kernels: pairs (first row, first nonzero) of each block followed by the
  * pair (nrows, nnz), i.e. 2 * (blocks + 1) ints.  A block holds at most
@@ -384,6 +423,13 @@ int igaos_spmv_csr_f64(int nblk, const int* rowblocks, const int* ptr,
 int igaos_spmv_csr_f32(int nblk, const int* rowblocks, const int* ptr,
                        const int* idx, const float* val, const float* x,
                        float* y, void* stream);
+/* CSR -> CSC of device arrays (replaces the transpose half of
+ * counting_sort, src/io/model.hpp:39-67): cp (n + 1), ci, acx (nnz)
+ * receive the transpose, each column's entries in row order, by a stable
+ * radix sort of the column indices.  Allocates its own scratch and
+ * synchronises the stream before returning.  0 on success. */
+int igaos_csr_to_csc_f64(int m, int n, int nnz, const int* ap, const int* ai,
+                         const double* ax, int* cp, int* ci, double* acx, void* stream);
 /* Ruiz equilibration, 10 sweeps (pdhg.cu:469-492), device arrays. */
 int igaos_ruiz_f64(int m, int n, const int* ap, const int* ai,
                    const double* ax, const int* cp, const int* ci,

@@ -296,11 +296,12 @@ def have_ref_simplex() -> bool:
     return os.path.exists(REF_LIB)
 
 
-def ref_simplex(lp, time_limit=3600.0):
+def ref_simplex(lp, time_limit=3600.0, warm=None):
     """Solve with the reference revised simplex (src/simplex/simplex.cpp).
 
     Returns (status, objective, x, y, iterations, solve_ms).  The simplex
-    reports y with d = c - A'y, i.e. y_simplex = -y_pdhg.
+    reports y with d = c - A'y, i.e. y_simplex = -y_pdhg.  ``warm`` =
+    (basis (m ints), states (n + m bytes)) is passed as its WarmStart.
     """
     global _ref
     if _ref is None:
@@ -309,6 +310,9 @@ def ref_simplex(lp, time_limit=3600.0):
         _ref.ref_simplex_solve.argtypes = (
             [C.c_int, C.c_int] + [_dp] * 5 + [_ip, _ip, _dp, _ip, _ip, _dp] +
             [C.c_double, C.c_double, _dp, _dp, _dp, C.POINTER(C.c_long), _dp])
+        _ref.ref_simplex_solve_warm.restype = C.c_int
+        _ref.ref_simplex_solve_warm.argtypes = (
+            _ref.ref_simplex_solve.argtypes + [_ip, C.POINTER(C.c_ubyte)])
     h = _Held(lp)
     a = h.arrs
     obj = C.c_double(0)
@@ -316,9 +320,15 @@ def ref_simplex(lp, time_limit=3600.0):
     ms = C.c_double(0)
     x = np.zeros(lp.n)
     y = np.zeros(lp.m)
-    st = _ref.ref_simplex_solve(
-        lp.m, lp.n, _pd(a["rmin"]), _pd(a["rmax"]), _pd(a["cl"]), _pd(a["cu"]),
-        _pd(a["c"]), _pi(a["ap"]), _pi(a["ai"]), _pd(a["ax"]), _pi(a["cp"]),
-        _pi(a["ci"]), _pd(a["acx"]), float(lp.obj_const), float(time_limit),
-        C.byref(obj), _pd(x), _pd(y), C.byref(its), C.byref(ms))
+    args = (lp.m, lp.n, _pd(a["rmin"]), _pd(a["rmax"]), _pd(a["cl"]), _pd(a["cu"]),
+            _pd(a["c"]), _pi(a["ap"]), _pi(a["ai"]), _pd(a["ax"]), _pi(a["cp"]),
+            _pi(a["ci"]), _pd(a["acx"]), float(lp.obj_const), float(time_limit),
+            C.byref(obj), _pd(x), _pd(y), C.byref(its), C.byref(ms))
+    if warm is None:
+        st = _ref.ref_simplex_solve(*args)
+    else:
+        wb = np.ascontiguousarray(warm[0], dtype=np.int32)
+        ws = np.ascontiguousarray(warm[1], dtype=np.uint8)
+        st = _ref.ref_simplex_solve_warm(*args, _pi(wb),
+                                         ws.ctypes.data_as(C.POINTER(C.c_ubyte)))
     return STATUS_NAMES[st], obj.value, x, y, its.value, ms.value

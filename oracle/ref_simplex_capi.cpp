@@ -9,6 +9,20 @@
 #include "model.hpp"
 #include "simplex.hpp"
 
+// warm_basis (m) / warm_state (n + m), when not null, are passed to the
+// simplex as its WarmStart (src/simplex/simplex.hpp:10-15).
+extern "C" int ref_simplex_solve_warm(int m, int n, const double* rmin,
+                                      const double* rmax, const double* cl,
+                                      const double* cu, const double* c,
+                                      const int* ap, const int* ai,
+                                      const double* ax, const int* cp,
+                                      const int* ci, const double* acx,
+                                      double obj_const, double time_limit_s,
+                                      double* objective, double* x_out,
+                                      double* y_out, long* iterations,
+                                      double* solve_ms, const int* warm_basis,
+                                      const unsigned char* warm_state);
+
 extern "C" int ref_simplex_solve(int m, int n, const double* rmin,
                                  const double* rmax, const double* cl,
                                  const double* cu, const double* c,
@@ -19,6 +33,22 @@ extern "C" int ref_simplex_solve(int m, int n, const double* rmin,
                                  double* objective, double* x_out,
                                  double* y_out, long* iterations,
                                  double* solve_ms) {
+    return ref_simplex_solve_warm(m, n, rmin, rmax, cl, cu, c, ap, ai, ax, cp, ci, acx,
+                                  obj_const, time_limit_s, objective, x_out, y_out,
+                                  iterations, solve_ms, nullptr, nullptr);
+}
+
+extern "C" int ref_simplex_solve_warm(int m, int n, const double* rmin,
+                                      const double* rmax, const double* cl,
+                                      const double* cu, const double* c,
+                                      const int* ap, const int* ai,
+                                      const double* ax, const int* cp,
+                                      const int* ci, const double* acx,
+                                      double obj_const, double time_limit_s,
+                                      double* objective, double* x_out,
+                                      double* y_out, long* iterations,
+                                      double* solve_ms, const int* warm_basis,
+                                      const unsigned char* warm_state) {
     igaos::io::Model md;
     md.m = m;
     md.n = n;
@@ -39,7 +69,12 @@ extern "C" int ref_simplex_solve(int m, int n, const double* rmin,
     igaos::Options opt;
     opt.time_limit_s = time_limit_s;
     opt.tolerance = 1e-9;
-    igaos::Solution s = igaos::simplex::solve(md, opt);
+    igaos::simplex::WarmStart ws;
+    if (warm_basis && warm_state) {
+        ws.basis.assign(warm_basis, warm_basis + m);
+        ws.nb_state.assign(warm_state, warm_state + n + m);
+    }
+    igaos::Solution s = igaos::simplex::solve(md, opt, warm_basis ? &ws : nullptr);
     *objective = s.objective;
     *iterations = s.iterations;
     *solve_ms = s.solve_time_ms;

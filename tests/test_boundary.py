@@ -376,7 +376,7 @@ def test_malformed_patterns_are_rejected_before_device_work():
     ci = lp.ci.copy()
     ci[0] = (ci[0] + 1) % lp.m         # CSC entry moved to another row
     cases = [(bad(ai=ai), "outside"), (bad(ap=ap), "decreases"),
-             (bad(ci=ci), "CSR and CSC differ")]
+             (bad(ci=ci), "CSR and CSC differ|not the transpose")]
     for b, what in cases:
         with pytest.raises(ValueError, match=what):
             igaos_amd.api.model_from_lp(b)
@@ -385,6 +385,47 @@ def test_malformed_patterns_are_rejected_before_device_work():
     # the well-formed instance passes the checks (and then needs a device)
     m = igaos_amd.api.model_from_lp(lp)
     assert m.n == lp.n
+
+
+def test_csc_must_be_the_transpose_of_csr():
+    # same entry counts per row and per column, but not the same matrix:
+    # an A' that is not the transpose of A would make the engine iterate on
+    # a different operator, so both model entry points refuse it; a CSC
+    # listing each column's rows in another order is the same matrix
+    import dataclasses
+    from igaos_amd.batch import BatchSolver
+    lp = lpgen.gen_random_eq(seed=4, m=30, n=70)
+    j1, j2 = 3, 11
+    p1, p2 = lp.cp[j1], lp.cp[j2]
+    assert lp.ci[p1] != lp.ci[p2]
+    ci = lp.ci.copy()
+    ci[p1], ci[p2] = lp.ci[p2], lp.ci[p1]   # row counts and column counts kept
+    acx = lp.acx.copy()
+    acx[p1] = acx[p1] + 1.0                   # same pattern, another value
+    for b in (dataclasses.replace(lp, ci=ci), dataclasses.replace(lp, acx=acx)):
+        with pytest.raises(ValueError, match="not the transpose"):
+            igaos_amd.api.model_from_lp(b)
+        with pytest.raises(RuntimeError, match="instance 1: .*not the transpose"):
+            BatchSolver([lp, b])
+    rev_ci, rev_acx = lp.ci.copy(), lp.acx.copy()
+    for j in range(lp.n):
+        a, b = lp.cp[j], lp.cp[j + 1]
+        rev_ci[a:b] = lp.ci[a:b][::-1]
+        rev_acx[a:b] = lp.acx[a:b][::-1]
+    m = igaos_amd.api.model_from_lp(dataclasses.replace(lp, ci=rev_ci, acx=rev_acx))
+    assert m.nnz() == lp.nnz
+
+
+def test_model_from_csr_needs_a_device_and_a_well_formed_csr():
+    lp = lpgen.gen_random_eq(seed=4, m=30, n=70)
+    ai = lp.ai.copy()
+    ai[0] = lp.n
+    args = (lp.m, lp.n, lp.c, lp.cl, lp.cu, lp.rmin, lp.rmax, lp.ap)
+    with pytest.raises(ValueError, match="outside"):
+        igaos_amd.model_from_csr(*args, ai, lp.ax)
+    if _core.device_count() == 0:
+        with pytest.raises(ValueError, match="none is visible"):
+            igaos_amd.model_from_csr(*args, lp.ai, lp.ax)
 
 
 def test_layout_builders_refuse_malformed_patterns():
@@ -576,3 +617,38 @@ def test_batch_solution_sizes_are_checked_on_the_host():
         b.set_solution(x=np.zeros(4))
     with pytest.raises(ValueError, match="4 entries"):
         b.set_solution(y=[np.zeros(2), np.zeros(1)])
+
+
+@pytest.mark.parametrize("case", ["gen_random_eq(seed=1,m=200,n=600)", "netlib/sc50a",
+                                  "netlib/blend", "netlib/grow7"])
+def test_crossover_basis_warm_starts_the_reference_simplex(case):
+    # PDHG -> simplex crossover (igaos_crossover_basis): from a 1e-6 PDHG
+    # point (the oracle's, standing in for the engine's on the CPU) a basis
+    # in the reference simplex's WarmStart layout; the reference's own
+    # revised simplex (oracle/_ref, built from its sources) started there
+    # reaches the cold solve's optimum in no more pivots, and in a few on
+    # the random LP the cold solve needs 1,388 for
+    import oracle
+    from tests.conftest import netlib_path
+    if not oracle.have_ref_simplex():
+        pytest.skip("reference simplex not built (oracle/_ref)")
+    if case.startswith("netlib/"):
+        lp = igaos_amd.read_mps(netlib_path(case[7:])).to_lp()
+    else:
+        lp = lpgen.gen_random_eq(seed=1, m=200, n=600)
+    r = oracle.solve(lp, tolerance=1e-6, time_limit=60)
+    assert r.status == "near-optimal"
+    md = igaos_amd.model_from_lp(lp)
+    basis, st, k = igaos_amd.api.crossover_basis(md, r.x, r.y, 1e-6)
+    assert k == lp.m
+    placed = basis[basis >= 0]
+    assert len(set(placed.tolist())) == len(placed) and placed.max() < lp.n + lp.m
+    assert np.all(st[placed] == 2) and np.sum(st == 2) == len(placed)
+    assert set(np.unique(st).tolist()) <= {0, 1, 2, 3}
+    cold = oracle.ref_simplex(lp, time_limit=60)
+    warm = oracle.ref_simplex(lp, time_limit=60, warm=(basis, st))
+    assert cold[0] == warm[0] == "optimal"
+    assert abs(warm[1] - cold[1]) <= 1e-9 * (1 + abs(cold[1]))
+    assert warm[4] <= cold[4]
+    if case.startswith("gen_random_eq"):
+        assert warm[4] <= 0.1 * cold[4], (warm[4], cold[4])

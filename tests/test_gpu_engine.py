@@ -339,3 +339,57 @@ def test_reference_recorded_pdhg_battery():
         assert s.status == rec["status"], (name, s.status)
         ref = rec["objective"]
         assert abs(s.objective - ref) <= 3e-4 * (1 + abs(ref)), (name, s.objective, ref)
+
+
+@pytest.mark.parametrize("key", ["gen_random_eq(seed=1,m=200,n=600)",
+                                 "gen_batch_instance(seed=0,index=1)",
+                                 "gen_batch_instance(seed=0,index=5)",
+                                 "netlib/afiro", "netlib/sc50a"])
+def test_single_lp_polish_returns_the_vertex(key, objectives):
+    # active-set polishing of the engine's 1e-6 solution (igaos_pdhg_polish).
+    # When the polished pair is kept it is primal and dual feasible to 1e-9
+    # and its objective is the HiGHS optimum to 1e-9; an unkept pair leaves
+    # the input unchanged.  The active set read off complementarity is square
+    # at a nondegenerate vertex (the random LPs: kept); degenerate Netlib
+    # vertices may be refused, as afiro is.
+    if key.startswith("netlib/"):
+        lp = igaos_amd.read_mps(netlib_path(key[7:])).to_lp()
+        ref = objectives[key]["highs_ref"]
+    else:
+        import re
+        kw = dict((k, int(v)) for k, v in re.findall(r"(\w+)=(\d+)", key))
+        lp = getattr(lpgen, key.split("(")[0])(**kw)
+        ref = objectives[key]["scipy_highs"]
+    md = igaos_amd.model_from_lp(lp)
+    for prec in (64, 32):
+        s = igaos_amd.solve_lp(md, time_limit=120, tolerance=1e-6, precision=prec)
+        assert s.status == "near-optimal"
+        x, y, obj, info = igaos_amd.api.polish(md, s.x, s.y)
+        if not key.startswith("netlib/"):
+            assert info["accepted"] == 1, info
+        if not info["accepted"]:
+            assert np.array_equal(x, s.x) and np.array_equal(y, s.y)
+            continue
+        assert abs(obj - ref) <= 1e-9 * (1 + abs(ref)) + 5e-10 * abs(ref), (obj, ref)
+        A = lp.to_scipy()
+        mt = oracle.evaluate(lp.rmin, lp.rmax, lp.cl, lp.cu, lp.c, x, y, A @ x,
+                             lp.c + A.T @ y)
+        assert mt.pinf <= 1e-9 and mt.dinf <= 1e-9 and mt.gap <= 1e-9, (mt.pinf, mt.dinf, mt.gap)
+
+
+def test_crossover_from_the_engine_warm_starts_the_reference_simplex():
+    # the engine's FP32 + refinement solution -> igaos_crossover_basis ->
+    # the reference's revised simplex (built from its sources) finishes at
+    # the cold optimum in a few pivots instead of 1,388
+    if not oracle.have_ref_simplex():
+        pytest.skip("reference simplex not built (oracle/_ref)")
+    lp = lpgen.gen_random_eq(seed=1, m=200, n=600)
+    md = igaos_amd.model_from_lp(lp)
+    s = igaos_amd.solve_lp(md, time_limit=120, tolerance=1e-6, precision=32)
+    assert s.status == "near-optimal"
+    basis, st, k = igaos_amd.api.crossover_basis(md, s.x, s.y, 1e-6)
+    cold = oracle.ref_simplex(lp, time_limit=60)
+    warm = oracle.ref_simplex(lp, time_limit=60, warm=(basis, st))
+    assert cold[0] == warm[0] == "optimal"
+    assert abs(warm[1] - cold[1]) <= 1e-9 * (1 + abs(cold[1]))
+    assert warm[4] <= 0.1 * cold[4], (warm[4], cold[4])
