@@ -66,6 +66,7 @@ SIGNATURES = {
     "igaos_model_row_activity": (C.c_int, [_vp, _dp, _dp]),
     "igaos_dual_repair": (C.c_int, [_vp, _dp, _dp, _dp, C.c_double, _dp]),
     "igaos_kkt_evaluate": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
+    "igaos_dual_repair_device": (C.c_int, [_vp, _dp, _dp, C.c_double, _dp, _dp]),
     "igaos_pdhg_polish": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_int, _dp, _dp, _dp,
                                     _vp]),
     "igaos_crossover_basis": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_int, _ip,

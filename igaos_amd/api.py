@@ -212,6 +212,22 @@ def polish(model: Model, x, y, tolerance=1e-12, max_iterations=None):
     return xo, yo, ob.value, {k: getattr(info, k) for k, _ in AdjointInfo._fields_}
 
 
+def dual_repair_device(model: Model, x, y_it, col_tol=1e-6):
+    """The engine's device dual repair on a point (igaos_dual_repair_device):
+    returns (found, y, info)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y_it = np.ascontiguousarray(y_it, dtype=np.float64)
+    y = np.zeros(model.m)
+    out = np.zeros(6)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    rc = _core.lib().igaos_dual_repair_device(model._h, dp(x), dp(y_it), float(col_tol),
+                                              dp(y), dp(out))
+    if rc < 0:
+        raise RuntimeError(_core.last_error())
+    keys = ("gap", "dinf", "dual_bound", "iterations", "cg_iterations", "columns_in_J")
+    return rc == 1, y, dict(zip(keys, out.tolist()))
+
+
 def crossover_basis(model: Model, x, y, tol=1e-6, fill=True):
     """Starting basis for the reference's revised simplex from a
     near-optimal (x, y) (igaos_crossover_basis): returns (basis, nb_state,

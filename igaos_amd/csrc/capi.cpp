@@ -14,6 +14,7 @@
 #include "model.hpp"
 #include "pdhg.hpp"
 #include "exact.hpp"
+#include "repair_dev.hpp"
 #include "transpose.hpp"
 #include "pdhg_kernels.hpp"
 
@@ -553,6 +554,33 @@ int igaos_dual_repair(const igaos_model* model, const double* x, const double* A
         return fail("null array");
     try {
         return igaos::pdhg::host_dual_repair(md, x, Ax, y_it, col_tol, y) ? 1 : 0;
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+}
+
+int igaos_dual_repair_device(const igaos_model* model, const double* x, const double* y_it,
+                             double col_tol, double* y, double* out) {
+    if (!model) return fail("null model");
+    const auto& md = model->m;
+    if ((md.n > 0 && !x) || (md.m > 0 && (!y_it || !y))) return fail("null array");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return fail("the device repair needs a HIP device; none is visible");
+    }
+    try {
+        igaos::dev::RepairInfo info;
+        const bool have = igaos::dev::dual_repair_device(md, x, y_it, col_tol, y, &info);
+        if (out) {
+            out[0] = info.gap;
+            out[1] = info.dinf;
+            out[2] = info.od;
+            out[3] = static_cast<double>(info.iterations);
+            out[4] = static_cast<double>(info.cg_iterations);
+            out[5] = static_cast<double>(info.columns_in_J);
+        }
+        return have ? 1 : 0;
     } catch (const std::exception& e) {
         return fail(e.what());
     }

@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "exact.hpp"
+#include "repair_dev.hpp"
 #include "pdhg.hpp"
 #include "pdhg_kernels.hpp"
 
@@ -96,15 +97,15 @@ double now_s() {
 // of x.  The engine then works on a column-permuted copy: x, c and the
 // bounds are permuted, every row keeps its nonzeros in their original
 // order (so A x sums in the same order), and the solution is permuted
-// back on output.  Applied when n >= 65536, the top 2% of columns hold at
-// least a quarter of the nonzeros, and the host dual repair (which works
-// on the model) is off for this size (m > max_repair_m).
+// back on output.  Applied when n >= 65536 and the top 2% of columns hold
+// at least a quarter of the nonzeros (never in exact mode); both dual
+// repairs work on the permuted copy like every other step.
 std::unique_ptr<io::Model> locality_copy(const io::Model& md,
                                          const EngineConfig& cfg,
                                          std::vector<int>& perm) {
     const int n = md.n;
     const long nnz = md.nnz();
-    if (cfg.exact || n < (1 << 16) || md.m <= cfg.max_repair_m || nnz == 0) return nullptr;
+    if (cfg.exact || n < (1 << 16) || nnz == 0) return nullptr;
     std::vector<int> ord(n);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
@@ -218,6 +219,10 @@ class Engine final : public EngineBase {
     // k^-0.3 / k^-0.6 table (filled up to the largest MAXIT run so far)
     // and the KKT products and summands
     DBuf<double> xdx_, xdy_, xtm_, xpart_, powtab_, kata_[2], ktm_[2], kterm_;
+    // device dual repair (m > max_repair_m): its scratch and the point
+    dev::RepairDev repair_;
+    DBuf<double> rpx_, rpy_, rpyit_;
+    long repair_cg_ = 0, repair_its_ = 0;
     long long pow_filled_ = 0;
     dev::ExactBufs eb_;
     static constexpr long long EXACT_POW_CAP = 1LL << 21;
@@ -249,6 +254,8 @@ class Engine final : public EngineBase {
         else dev::launch_attempt(ib_, s);
     }
     void fill_powtab(long long maxit);
+    bool device_repair(const Metrics& t, double tol, Metrics& mr, std::vector<double>& rx,
+                       std::vector<double>& ry);
     void sync() { HIPCK(hipStreamSynchronize(s_)); }
     LsState read_state() {
         LsState h;
@@ -603,6 +610,47 @@ void Engine<V>::check_metrics(int cur, Metrics& mc, Metrics& ma) {
     ma = h[1];
 }
 
+// Device dual repair of the scaled point in tmpn_ / tmpm_ (its metrics t):
+// both column tolerance tiers of pdhg.cu:897-925, accepted when the
+// repaired pair's gap is certified (x, hence pinf and prow, unchanged).
+// On success rx, ry receive the pair in original units.
+template <typename V>
+bool Engine<V>::device_repair(const Metrics& t, double tol, Metrics& mr,
+                              std::vector<double>& rx, std::vector<double>& ry) {
+    if (!rpx_.p) {
+        rpx_.alloc(n_);
+        rpy_.alloc(m_);
+        rpyit_.alloc(m_);
+        repair_.alloc(m_, n_);
+    }
+    dev::unscale_out(m_, n_, dc_.p, dr_.p, nullptr, tmpn_.p, nullptr, nullptr, tmpm_.p, nullptr,
+                     rpx_.p, rpyit_.p, s_);
+    dev::RepairData D;
+    D.ap = ap_.p; D.ai = ai_.p; D.ax = ax_.p;
+    D.cp = cp_.p; D.ci = ci_.p; D.acx = acx_.p;
+    D.rbA = rA_; D.rbT = rT_;
+    D.c = c_.p; D.cl = cl_.p; D.cu = cu_.p; D.rmin = rmin_.p; D.rmax = rmax_.p;
+    static const double ctol[2] = {1e-6, 1e-1};
+    for (int tier = 0; tier < 2; ++tier) {
+        dev::RepairInfo info;
+        const bool have = repair_.run(D, rpx_.p, rpyit_.p, ctol[tier], t.op, kd_.cmax, s_,
+                                      rpy_.p, &info);
+        repair_cg_ += info.cg_iterations;
+        repair_its_ += info.iterations;
+        if (!have || !(t.pinf <= tol && t.prow <= tol && std::isfinite(info.gap) &&
+                       info.gap <= tol))
+            continue;
+        mr = Metrics{t.pinf, info.dinf, info.gap, t.op, info.od, t.prow};
+        rx.resize(n_);
+        ry.resize(m_);
+        HIPCK(hipMemcpyAsync(rx.data(), rpx_.p, sizeof(double) * n_, hipMemcpyDeviceToHost, s_));
+        HIPCK(hipMemcpyAsync(ry.data(), rpy_.p, sizeof(double) * m_, hipMemcpyDeviceToHost, s_));
+        sync();
+        return true;
+    }
+    return false;
+}
+
 // Exact mode: k^-0.3 and k^-0.6 for k = 1..maxit with the host's pow, as
 // the reference's host controller computes them (pdhg.cu:780-781).
 template <typename V>
@@ -719,13 +767,12 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
         if (kdone >= 5000 && kdone >= next_repair) {  // pdhg.cu:854-931
             next_repair = kdone + repair_spacing;
             repair_spacing = std::min(repair_spacing * 3, 30000L);
-            for (int cc = 1; cc <= 2 && m_ <= cfg_.max_repair_m; ++cc) {
+            for (int cc = 1; cc <= 2; ++cc) {
                 const Metrics& t = cc == 1 ? ma : inc;
                 if (cc == 2 && !have_inc) break;
                 if (!(t.pinf <= TOL && t.prow <= TOL && t.dinf <= 10.0 * TOL &&
                       !(std::isfinite(t.gap) && t.gap <= TOL)))
                     continue;
-                std::vector<double> sx(n_), sy(m_), rx(n_), ry_it(m_), ry(m_);
                 if (cc == 1) {
                     full(tmpn_.p, F32 ? xref_.p : nullptr, xavg_.p, n_);
                     full(tmpm_.p, F32 ? yref_.p : nullptr, yavg_.p, m_);
@@ -733,6 +780,19 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
                     copy_d(tmpn_.p, xb_.p, n_);
                     full(tmpm_.p, F32 ? yref_.p : nullptr, ycur(cur), m_);
                 }
+                if (m_ > cfg_.max_repair_m) {
+                    // device repair (any m): the same schedule and tiers
+                    Metrics mr{};
+                    if (!device_repair(t, TOL, mr, rep_x, rep_y)) continue;
+                    if (opt_.verbosity > 1)
+                        std::fprintf(stderr, "[pdhg] device dual repair c=%d @k=%ld gap=%.3e\n",
+                                     cc, kdone, mr.gap);
+                    last = mr;
+                    last_is_avg = (cc == 1);
+                    repaired = true;
+                    break;
+                }
+                std::vector<double> sx(n_), sy(m_), rx(n_), ry_it(m_), ry(m_);
                 HIPCK(hipMemcpyAsync(sx.data(), tmpn_.p, sizeof(double) * n_, hipMemcpyDeviceToHost, s_));
                 HIPCK(hipMemcpyAsync(sy.data(), tmpm_.p, sizeof(double) * m_, hipMemcpyDeviceToHost, s_));
                 std::vector<double> hdc(n_), hdr(m_);

@@ -15,7 +15,8 @@ namespace igaos::pdhg {
 struct EngineConfig {
     int precision = 64;      // 64: plain FP64 engine; 32: FP32 inner + FP64 refinement
     int graph_slots = 16;    // attempts captured per hipGraph replay (0 = no graph)
-    int max_repair_m = 4096; // host dense dual repair only up to this many rows
+    int max_repair_m = 4096; // dual repair: host dense (pdhg.cu's) up to this many rows,
+                             // device CG repair above
     int device = 0;
     int column_slices = -1;  // column slices of A / A' products: -1 auto, 0 off, 1..16 forced
     int xcd_pinned = 0;      // column slices: 1 = all in one launch, each XCD on its own slices

@@ -104,7 +104,11 @@ void igaos_options_default(igaos_options* opt);
 typedef struct {
     int precision;    /* 64: plain FP64; 32: FP32 inner + FP64 refinement */
     int graph_slots;  /* attempts per hipGraph replay; 0 = direct launches */
-    int max_repair_m; /* dense dual repair only for m <= this */
+    int max_repair_m; /* dual repair (pdhg.cu:288-448, schedule 854-931):
+                         the reference's dense host repair for m <= this,
+                         the device repair (conjugate gradients on
+                         A_J A_J' + ridge, simultaneous halfspace steps)
+                         above it */
     int device;
     int column_slices; /* column slices of the A / A' products (gather
                           locality on large LPs): -1 auto, 0 off, 1..16 */
@@ -394,6 +398,21 @@ int igaos_pdhg_polish(const igaos_model* model, const double* x, const double* y
  * variables placed (at most m), or -1 (igaos_last_error). */
 int igaos_crossover_basis(const igaos_model* model, const double* x, const double* y,
                           double tol, int fill, int* basis, unsigned char* nb_state);
+
+/* The device dual repair the engine runs for m > max_repair_m (no
+ * counterpart: the reference's repair, pdhg.cu:288-448, is dense and
+ * stops at m = 46,340): the same POCS alternation with x fixed -- the
+ * projection onto d_J = 0 by conjugate gradients on A_J A_J' + ridge (at
+ * most 200 iterations to 1e-10 relative residual each), the halfspace
+ * steps of the sign-constrained columns taken together (each row moves by
+ * the mean of its violated columns' steps), the row cones -- from the two
+ * anchors, at most 300 iterations each, keeping the y with the best finite
+ * gap.  x (n), y_it (m) host, original units; y (m, host) receives the
+ * dual; out (6 doubles, may be NULL) gap, dinf, dual bound, POCS and CG
+ * iterations, |J|.  Needs a visible HIP device.  Returns 1 when a finite
+ * gap was found, 0 otherwise, -1 on error. */
+int igaos_dual_repair_device(const igaos_model* model, const double* x, const double* y_it,
+                             double col_tol, double* y, double* out);
 
 /* Host: row-block layout of a CSR row pointer (host) for the SpMV
  * kernels: pairs (first row, first nonzero) of each block followed by the
