@@ -28,6 +28,7 @@ class BatchSolution:
     attempts: np.ndarray
     restarts: np.ndarray
     solve_ms: np.ndarray
+    prow: np.ndarray = None
     x: list = field(default_factory=list)
     y: list = field(default_factory=list)
     kernel_ms: float = 0.0
@@ -148,12 +149,45 @@ class BatchSolver:
             objective=f("objective"), pinf=f("pinf"), dinf=f("dinf"),
             rel_gap=f("rel_gap"), iterations=f("iterations"),
             attempts=f("attempts"), restarts=f("restarts"),
-            solve_ms=f("solve_ms"),
+            solve_ms=f("solve_ms"), prow=f("prow"),
             kernel_ms=L.igaos_batch_kernel_ms(self._h))
         if want_xy:
             sol.x = [x[self._coff[i]:self._coff[i + 1]] for i in range(self.count)]
             sol.y = [y[self._roff[i]:self._roff[i + 1]] for i in range(self.count)]
         return sol
+
+    def repair(self, sol: BatchSolution, tolerance=1e-6) -> list:
+        """Certification fallback at exit for the batched engine, which
+        does not run the in-loop dual repair: every instance that stopped
+        uncertified with its primal residuals at tolerance (pinf, prow <=
+        tol, dinf <= 10 tol, the trigger of src/pdhg/pdhg.cu:882-887) but
+        its gap not certified gets the engine's device dual repair
+        (igaos_dual_repair_device, both column tiers); where the repaired
+        gap is certified the instance becomes near-optimal with the
+        repaired y (x is unchanged).  ``sol`` needs x and y (run with
+        want_xy=True) and is updated in place; returns the repaired
+        instances."""
+        from .api import dual_repair_device, model_from_lp
+        if not sol.x or not sol.y:
+            raise ValueError("repair needs the run's x and y (want_xy=True)")
+        done = []
+        for i, st in enumerate(sol.status):
+            if st == "near-optimal":
+                continue
+            if not (sol.pinf[i] <= tolerance and sol.prow[i] <= tolerance and
+                    sol.dinf[i] <= 10 * tolerance):
+                continue
+            md = model_from_lp(self._lps[i])
+            for col_tol in (1e-6, 1e-1):
+                ok, y, info = dual_repair_device(md, sol.x[i], sol.y[i], col_tol)
+                if ok and info["gap"] <= tolerance:
+                    sol.y[i] = y
+                    sol.status[i] = "near-optimal"
+                    sol.rel_gap[i] = info["gap"]
+                    sol.dinf[i] = info["dinf"]
+                    done.append(i)
+                    break
+        return done
 
     def update(self, c=None, rmin=None, rmax=None, ax=None, cl=None, cu=None):
         """Replace numeric data in place (concatenated over instances; ax in
@@ -319,3 +353,61 @@ def solve_batch(lps, precision=32, mode="persistent", **kw) -> BatchSolution:
         return b.run(**kw)
     finally:
         b.close()
+
+
+def solve_node_relaxations(lp, nodes, precision=32, mode="lockstep", repair=True, **kw):
+    """LP relaxations of branch-and-bound nodes in one batched solve.
+
+    The reference's MILP solves each node's LP on its own with the simplex
+    (src/milp/milp.cpp:399-421); a node's LP is the root LP with the
+    node's column bounds (``lp.cl = nd.cl; lp.cu = nd.cu``).  Here
+    ``nodes`` is a list of (cl, cu) arrays (n each; +-inf for open bounds)
+    and all of them are solved together by the batched engine (one
+    instance per node, the root's pattern and values shared by copy).  A
+    node whose bounds cross (cl_j > cu_j) is infeasible without a solve and
+    reported as such.  Returns the BatchSolution over the nodes (solved
+    nodes only carry x, y; infeasible ones have status "infeasible").
+    """
+    import dataclasses
+    keep, out_status = [], []
+    insts = []
+    for k, (cl, cu) in enumerate(nodes):
+        cl = np.ascontiguousarray(cl, dtype=np.float64)
+        cu = np.ascontiguousarray(cu, dtype=np.float64)
+        if cl.shape != (lp.n,) or cu.shape != (lp.n,):
+            raise ValueError(f"node {k}: bounds must have {lp.n} entries")
+        if np.any(cl > cu):
+            out_status.append("infeasible")
+            continue
+        keep.append(k)
+        out_status.append(None)
+        insts.append(dataclasses.replace(lp, cl=cl, cu=cu))
+    kw.setdefault("want_xy", True)
+    sol = None
+    if insts:
+        b = BatchSolver(insts, precision=precision, mode=mode)
+        try:
+            sol = b.run(**kw)
+            if repair and kw["want_xy"]:
+                b.repair(sol, kw.get("tolerance", 1e-6))
+        finally:
+            b.close()
+    count = len(nodes)
+    res = BatchSolution(status=[None] * count, objective=np.full(count, np.nan),
+                        pinf=np.full(count, np.nan), dinf=np.full(count, np.nan),
+                        rel_gap=np.full(count, np.nan), iterations=np.zeros(count, np.int64),
+                        attempts=np.zeros(count, np.int64), restarts=np.zeros(count, np.int64),
+                        solve_ms=np.zeros(count), prow=np.full(count, np.nan),
+                        x=[None] * count, y=[None] * count,
+                        kernel_ms=sol.kernel_ms if sol is not None else 0.0)
+    for k, st in enumerate(out_status):
+        if st is not None:
+            res.status[k] = st
+    for j, k in enumerate(keep):
+        res.status[k] = sol.status[j]
+        for key in ("objective", "pinf", "dinf", "rel_gap", "iterations", "attempts",
+                    "restarts", "solve_ms", "prow"):
+            getattr(res, key)[k] = getattr(sol, key)[j]
+        if sol.x:
+            res.x[k], res.y[k] = sol.x[j], sol.y[j]
+    return res

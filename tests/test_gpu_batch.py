@@ -240,3 +240,41 @@ def test_netlib_sweep_as_one_lockstep_batch(precision, min_certified):
             assert abs(ob - ref) <= 2e-6 * (1 + abs(ref)), (r["instance"], ob, ref)
         else:
             assert st == "time-limit", (r["instance"], st)
+
+
+def test_branch_and_bound_node_relaxations_in_one_batch():
+    # the LP relaxations of 16 branch-and-bound nodes (the root with one
+    # column's bound tightened to floor / ceil of its root value, as
+    # src/milp/milp.cpp:399-421 forms a node's LP) plus a node whose bounds
+    # cross, solved in one batched call: every feasible node certifies at
+    # 1e-6 with HiGHS's objective, and HiGHS-infeasible nodes do not certify
+    from igaos_amd.batch import solve_node_relaxations
+    from tests.golden.make_fixtures import scipy_highs
+    lp = lpgen.gen_random_eq(seed=2, m=60, n=150)
+    root = igaos_amd.solve_lp(lp, time_limit=60, tolerance=1e-6)
+    assert root.status == "near-optimal"
+    frac = np.abs(root.x - np.round(root.x))
+    cols = np.argsort(-frac)[:8]
+    nodes = []
+    for j in cols:
+        for up in (False, True):
+            cl, cu = lp.cl.copy(), lp.cu.copy()
+            if up:
+                cl[j] = np.ceil(root.x[j])
+            else:
+                cu[j] = np.floor(root.x[j])
+            nodes.append((cl, cu))
+    bad_cl = lp.cl.copy()
+    bad_cl[0] = lp.cu[0] + 1.0
+    nodes.append((bad_cl, lp.cu.copy()))
+    res = solve_node_relaxations(lp, nodes, tolerance=1e-6, time_limit=20)
+    assert res.status[-1] == "infeasible"
+    import dataclasses
+    for k, (cl, cu) in enumerate(nodes[:-1]):
+        ref = scipy_highs(dataclasses.replace(lp, cl=cl, cu=cu))
+        if ref is None:
+            assert res.status[k] != "near-optimal", k
+            continue
+        assert res.status[k] == "near-optimal", (k, res.status[k])
+        assert abs(res.objective[k] - ref) <= 2e-6 * (1 + abs(ref)), (k, res.objective[k], ref)
+        assert res.objective[k] >= root.objective - 1e-6 * (1 + abs(root.objective))
