@@ -830,6 +830,21 @@ __global__ __launch_bounds__(NT, LK_WAVES<V>) void k_lk_primal(const Arrays<V>* 
     const V* __restrict__ y = L.y(cur);
     const int r0 = e.r0, p0 = e.p0, r1 = e.r1, p1 = e.p1;
     V* rows = reinterpret_cast<V*>(S.rows);
+    // a block holds at most NT rows: thread t steps row r0 + t.  Its
+    // operands are loaded before the product so that they arrive while
+    // the matrix streams (they depend on the table entry only)
+    static_assert(dev::ROWS_MAX <= NT, "one epilogue row per thread");
+    const int t = threadIdx.x;
+    const bool own = t < r1 - r0;
+    const int j = r0 + (own ? t : 0);
+    V xo = V(0), xa = V(0), g0 = V(0), lo = V(0), hi = V(0);
+    if (own) {
+        xo = x[j];
+        if (mix) xa = L.xavg()[j];
+        g0 = L.gref()[j];
+        lo = L.lo()[j];
+        hi = L.hi()[j];
+    }
     // A'y is recomputed on a retry too (same y, same sums) rather than
     // stored for it: retries are rare, the store would be paid every round
     if constexpr (GATHER_ORDERED<I> && sizeof(V) == 4)
@@ -844,20 +859,15 @@ __global__ __launch_bounds__(NT, LK_WAVES<V>) void k_lk_primal(const Arrays<V>* 
                               p1, rows, reinterpret_cast<V*>(S.prod),
                               reinterpret_cast<V*>(S.red));
     double acc = 0.0;
-    for (int t = threadIdx.x; t < r1 - r0; t += NT) {
-        const int j = r0 + t;
+    if (own) {
         const V a = rows[t];
-        const V xo = x[j];
-        if (mix) {
-            const V xa = L.xavg()[j];
-            L.xavg()[j] = xa + w * (xo - xa);
-        }
-        const V g = L.gref()[j] + a;
-        const V v = fmin(fmax(xo - tau * g, L.lo()[j]), L.hi()[j]);
+        if (mix) L.xavg()[j] = xa + w * (xo - xa);
+        const V g = g0 + a;
+        const V v = fmin(fmax(xo - tau * g, lo), hi);
         xn[j] = v;
         L.xbar()[j] = V(2) * v - xo;
         const V d = v - xo;
-        acc += static_cast<double>(d) * static_cast<double>(d);
+        acc = static_cast<double>(d) * static_cast<double>(d);
     }
     acc = bsum(acc, S.red);
     if (threadIdx.x == 0) part[te] = acc;
@@ -886,6 +896,23 @@ __global__ __launch_bounds__(NT, LK_WAVES<V>) void k_lk_dual(const Arrays<V>* A,
     V* __restrict__ yn = L.y(cur ^ 1);
     const int r0 = e.r0, p0 = e.p0, r1 = e.r1, p1 = e.p1;
     V* rows = reinterpret_cast<V*>(S.rows);
+    // the row's operands, loaded before the product (see k_lk_primal)
+    const V* yoff = L.yoff();
+    const int t = threadIdx.x;
+    const bool own = t < r1 - r0;
+    const int i = r0 + (own ? t : 0);
+    V yo = V(0), axc = V(0), ya = V(0), adx = V(0), yof = V(0), rlo = V(0), rhi = V(0);
+    if (own) {
+        yo = y[i];
+        axc = L.axcur()[i];
+        if (mix) {
+            ya = L.yavg()[i];
+            adx = L.adx()[i];
+        }
+        if (yoff) yof = yoff[i];
+        rlo = L.rlo()[i];
+        rhi = L.rhi()[i];
+    }
     if constexpr (GATHER_ORDERED<I> && sizeof(V) == 4)
         rowblock_sums_packed(L.ap(), L.apk(), L.xbar(), r0, r1, p0, p1, rows,
                              reinterpret_cast<V*>(S.prod), reinterpret_cast<V*>(S.red));
@@ -897,29 +924,24 @@ __global__ __launch_bounds__(NT, LK_WAVES<V>) void k_lk_dual(const Arrays<V>* A,
         dev::rowblock_sums<V>(L.ap(), L.ai(), L.fax(), dev::GatherPlain<V>{L.xbar()}, r0,
                               r1, p0, p1, rows, reinterpret_cast<V*>(S.prod),
                               reinterpret_cast<V*>(S.red));
-    const V* yoff = L.yoff();
     double a1 = 0.0, a2 = 0.0;
-    for (int t = threadIdx.x; t < r1 - r0; t += NT) {
-        const int i = r0 + t;
+    if (own) {
         const V tm = rows[t];
-        const V yo = y[i];
-        V axc = L.axcur()[i];
         if (mix) {
-            const V ya = L.yavg()[i];
             L.yavg()[i] = ya + w * (yo - ya);
-            axc = axc + L.adx()[i];
+            axc = axc + adx;
             L.axcur()[i] = axc;
         }
         const V tt = yo + sig * tm;
-        V u = yoff ? tt + yoff[i] : tt;
+        V u = yoff ? tt + yof : tt;
         u = u / sig;
-        const V yv = tt - sig * fmin(fmax(u, L.rlo()[i]), L.rhi()[i]);
+        const V yv = tt - sig * fmin(fmax(u, rlo), rhi);
         yn[i] = yv;
         const V dy = yv - yo;
         const V ad = V(0.5) * (tm - axc);
         L.adx()[i] = ad;
-        a1 += static_cast<double>(dy) * static_cast<double>(dy);
-        a2 += static_cast<double>(dy) * static_cast<double>(ad);
+        a1 = static_cast<double>(dy) * static_cast<double>(dy);
+        a2 = static_cast<double>(dy) * static_cast<double>(ad);
     }
     a1 = bsum(a1, S.red);
     a2 = bsum(a2, S.red);
