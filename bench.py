@@ -35,6 +35,8 @@ n = 3,000) with the single-LP engine and reports iterations/s.
 badly scaled LP (n = 1,000,000, m = 400,000, clip(Zipf(1.5), 2, 2000)
 nonzeros per column, 68.5M nonzeros) in FP64 and FP32, plus a bounded
 FP32 solve (the LP needs well over 10^4 iterations to reach 1e-6).
+``--workload zipf-solve`` solves that LP once, in ``--precision`` 64 or 32
+(FP32 + FP64 refinement), within ``--time-limit``.
 
 ``cpu_baseline`` is the oracle (a single-threaded C host replay of the
 reference engine) on a bounded sample of the same workload, named in
@@ -547,6 +549,60 @@ def bench_zipf(args):
     }
 
 
+def bench_zipf_solve(args):
+    """One solve of the 1M-column LP to the tolerance in one precision
+    (``--precision``, FP64 or FP32 + FP64 refinement) within
+    ``--time-limit``: status, iterations/s, the KKT measures and the
+    objective (the FP32 result's error is read against an FP64 run of the
+    same LP), and the roofline of the precision's dominant attempt step."""
+    import igaos_amd
+    from igaos_amd import lpgen
+
+    prec = args.precision
+    t = time.perf_counter()
+    lp = lpgen.gen_zipf_badly_scaled(seed=0)
+    log(f"[bench] zipf LP nnz={lp.nnz} generated in {time.perf_counter() - t:.1f} s")
+    e = igaos_amd.Engine(lp, precision=prec)
+    tm = e.profile_attempts(args.attempts)
+    sl = e.column_slices()
+    w = 8 if prec == 64 else 4
+    bp = lp.nnz * (w + 4) + 4 * (lp.n + 1) + 7 * lp.n * w
+    bd = lp.nnz * (w + 4) + 4 * (lp.m + 1) + 8 * lp.m * w
+    dom = "dual" if tm["ms_dual"] >= tm["ms_primal"] else "primal"
+    ms_dom = tm[f"ms_{dom}"]
+    b_dom = bp if dom == "primal" else bd
+    torch_sync()
+    t0 = time.perf_counter()
+    s = e.run(time_limit=args.time_limit, tolerance=args.tolerance, verbosity=1)
+    el = time.perf_counter() - t0
+    e.close()
+    log(f"[bench] FP{prec} solve: {s.status} {s.iterations} it {el:.1f} s "
+        f"obj {s.objective:.12g}")
+    achieved = b_dom / ms_dom / 1e6
+    solved = 1 if s.status == "near-optimal" else 0
+    return {
+        "metric": METRIC, "value": solved / el, "unit": "LPs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": el * 1000.0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == 64 else "f32+f64refine", "data": "synthetic",
+        "config": {"workload": "badly scaled Zipf LP, m=400000 n=1000000 "
+                               "nnz=68528436 seed 0, one solve",
+                   "precision": prec, "tolerance": args.tolerance,
+                   "time_limit_s": args.time_limit},
+        "solve": {"status": s.status, "iterations": s.iterations,
+                  "iterations_per_s": s.iterations / el, "objective": s.objective,
+                  "pinf": s.pinf, "dinf": s.dinf, "rel_gap": s.rel_gap,
+                  "solve_time_ms": s.solve_time_ms},
+        "roofline": {"bound": "hbm",
+                     "kernel": f"FP{prec} {dom} step: {step_kernels(dom, sl[dom], prec)}",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": None, "bytes_per_launch": b_dom,
+                     "ms_per_launch": round(ms_dom, 4)},
+        "cpu_baseline": None,
+    }
+
+
 def bench_c3(args):
     """The block-structured multi-period blending LP (96 periods of 5,000
     rows x 8,000 columns, 6 nonzeros per column, 1,000 inventory columns
@@ -723,7 +779,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--workload", default="batch",
-                    choices=["batch", "zipf", "diff", "c1", "c3"])
+                    choices=["batch", "zipf", "zipf-solve", "diff", "c1", "c3"])
+    ap.add_argument("--precision", type=int, default=32, choices=[32, 64],
+                    help="zipf-solve workload: FP64, or FP32 + FP64 refinement")
     ap.add_argument("--count", type=int, default=8192)
     ap.add_argument("--mode", default="auto", choices=["auto", "lockstep", "persistent"],
                     help="batched execution mode (igaos_batch_set_mode); auto: "
@@ -766,8 +824,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    line = {"batch": bench_batch, "zipf": bench_zipf, "diff": bench_diff,
-            "c1": bench_c1, "c3": bench_c3}[args.workload](args)
+    line = {"batch": bench_batch, "zipf": bench_zipf, "zipf-solve": bench_zipf_solve,
+            "diff": bench_diff, "c1": bench_c1, "c3": bench_c3}[args.workload](args)
     print(json.dumps(line), flush=True)
 
 
