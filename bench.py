@@ -562,7 +562,7 @@ def bench_zipf_solve(args):
     t = time.perf_counter()
     lp = lpgen.gen_zipf_badly_scaled(seed=0)
     log(f"[bench] zipf LP nnz={lp.nnz} generated in {time.perf_counter() - t:.1f} s")
-    e = igaos_amd.Engine(lp, precision=prec)
+    e = igaos_amd.Engine(lp, precision=prec, scaling=args.scaling)
     tm = e.profile_attempts(args.attempts)
     sl = e.column_slices()
     w = 8 if prec == 64 else 4
@@ -587,7 +587,7 @@ def bench_zipf_solve(args):
         "dtype": "f64" if prec == 64 else "f32+f64refine", "data": "synthetic",
         "config": {"workload": "badly scaled Zipf LP, m=400000 n=1000000 "
                                "nnz=68528436 seed 0, one solve",
-                   "precision": prec, "tolerance": args.tolerance,
+                   "precision": prec, "scaling": args.scaling, "tolerance": args.tolerance,
                    "time_limit_s": args.time_limit},
         "solve": {"status": s.status, "iterations": s.iterations,
                   "iterations_per_s": s.iterations / el, "objective": s.objective,
@@ -782,6 +782,8 @@ def main():
                     choices=["batch", "zipf", "zipf-solve", "diff", "c1", "c3"])
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64],
                     help="zipf-solve workload: FP64, or FP32 + FP64 refinement")
+    ap.add_argument("--scaling", default="ruiz", choices=["ruiz", "ruiz+pock-chambolle"],
+                    help="zipf-solve workload: the equilibration (Ruiz is the reference's)")
     ap.add_argument("--count", type=int, default=8192)
     ap.add_argument("--mode", default="auto", choices=["auto", "lockstep", "persistent"],
                     help="batched execution mode (igaos_batch_set_mode); auto: "

@@ -30,7 +30,8 @@ class EngineConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("graph_slots", C.c_int),
                 ("max_repair_m", C.c_int), ("device", C.c_int),
                 ("column_slices", C.c_int), ("xcd_pinned", C.c_int),
-                ("scaling", C.c_int), ("exact", C.c_int)]
+                ("scaling", C.c_int), ("exact", C.c_int),
+                ("small_runner", C.c_int)]
 
 
 class SolveInfo(C.Structure):

@@ -282,11 +282,12 @@ class Engine:
     """
 
     def __init__(self, lp, precision=64, graph_slots=None, max_repair_m=None,
-                 column_slices=None, exact=False, xcd_pinned=None, scaling="ruiz"):
+                 column_slices=None, exact=False, xcd_pinned=None, scaling="ruiz",
+                 small_runner=None):
         self.model = lp if isinstance(lp, Model) else model_from_lp(lp)
         self.precision = precision
         cfg = _config(precision, graph_slots, max_repair_m, column_slices, exact,
-                      xcd_pinned, scaling)
+                      xcd_pinned, scaling, small_runner)
         h = _core.lib().igaos_pdhg_create(self.model._h, C.byref(cfg))
         if not h:
             raise RuntimeError(_core.last_error())
@@ -360,7 +361,7 @@ SCALINGS = {"ruiz": 0, "ruiz+pock-chambolle": 1}
 
 def _config(precision=64, graph_slots=None, max_repair_m=None,
             column_slices=None, exact=False, xcd_pinned=None,
-            scaling="ruiz") -> EngineConfig:
+            scaling="ruiz", small_runner=None) -> EngineConfig:
     c = EngineConfig()
     _core.lib().igaos_engine_config_default(C.byref(c))
     c.precision = int(precision)
@@ -376,6 +377,8 @@ def _config(precision=64, graph_slots=None, max_repair_m=None,
         c.max_repair_m = int(max_repair_m)
     if column_slices is not None:
         c.column_slices = int(column_slices)
+    if small_runner is not None:
+        c.small_runner = int(small_runner)
     return c
 
 
@@ -401,7 +404,7 @@ def solve(path, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
 def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
              precision=64, verbosity=0, graph_slots=None,
              max_repair_m=None, column_slices=None, exact=False,
-             xcd_pinned=None, scaling="ruiz") -> Solution:
+             xcd_pinned=None, scaling="ruiz", small_runner=None) -> Solution:
     """Solve an in-memory LP (``lpgen.LP`` or ``Model``) with the engine.
 
     ``precision=64`` is the plain FP64 engine; ``precision=32`` runs FP32
@@ -409,12 +412,16 @@ def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
     (FP64, iteration-bounded runs) fixes every summation order to the
     reference's sequential one and disables fused multiply-adds, so the run
     is bit-reproducible against a sequential FP64 host replay.
+    ``small_runner`` (-1: exact mode only, 0: never, 1: also the default
+    mode): FP64 LPs with max(n, m) <= 16384 and nnz <= 15360 run a check
+    period of attempts per launch in one workgroup, with the exact mode's
+    arithmetic.
     """
     model = lp if isinstance(lp, Model) else model_from_lp(lp)
     return _run(model, _options(time_limit, tolerance, max_iterations, True,
                                 0, verbosity),
                 _config(precision, graph_slots, max_repair_m, column_slices, exact,
-                        xcd_pinned, scaling))
+                        xcd_pinned, scaling, small_runner))
 
 
 has_pdhg = _has_pdhg()

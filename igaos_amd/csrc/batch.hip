@@ -41,6 +41,21 @@ __device__ __forceinline__ double full(const double* ref, const V* d, int j) {
     return ref ? ref[j] + static_cast<double>(d[j]) : static_cast<double>(d[j]);
 }
 
+// The FP64 primal iterate of column j.  In FP32 a correction exactly on
+// its clamp bound (lo, hi) stands for the column at its bound and gives
+// that bound exactly: x_ref + hi misses it by the FP32 rounding of
+// scu - x_ref, which the KKT test's bound tolerance (1e-9 (1 + |bound|),
+// pdhg.cu:206-215) does not absorb.
+template <typename V>
+__device__ __forceinline__ double xfull(const LP<V>& L, const V* d, int j) {
+    if constexpr (sizeof(V) == 4) {
+        const V v = d[j];
+        if (v == L.hi()[j]) return L.scu()[j];
+        if (v == L.lo()[j]) return L.scl()[j];
+    }
+    return full(L.xref(), d, j);
+}
+
 template <typename V>
 __device__ void refresh_refs(const LP<V>& L, Lds& S) {
     // FP32: aref = A~ xref, atref = A~' yref (FP64), then the shifted data
@@ -75,11 +90,10 @@ __device__ Metrics metrics(const LP<V>& L, const V* xd, const V* yd,
     const double* xg;
     const double* yg;
     if constexpr (sizeof(V) == 4) {
-        const double* xr = L.xref();
         const double* yr = L.yref();
         double* xf = L.xout();
         double* yf = L.yout();
-        for (int j = threadIdx.x; j < L.n; j += NT) xf[j] = full(xr, xd, j);
+        for (int j = threadIdx.x; j < L.n; j += NT) xf[j] = xfull(L, xd, j);
         for (int i = threadIdx.x; i < L.m; i += NT) yf[i] = full(yr, yd, i);
         __syncthreads();
         xg = xf;
@@ -394,7 +408,7 @@ __device__ bool check_lp(const LP<V>& L, const Params& P, Ctl& C, Lds& S,
     if (C.inc_src >= 0) {
         const V* sx = C.inc_src ? L.xavg() : L.x(cur);
         const V* sy = C.inc_src ? L.yavg() : L.y(cur);
-        for (int j = tid; j < n; j += NT) L.xb()[j] = full(L.xref(), sx, j);
+        for (int j = tid; j < n; j += NT) L.xb()[j] = xfull(L, sx, j);
         for (int i = tid; i < m; i += NT) L.yb()[i] = full(L.yref(), sy, i);
     }
     __syncthreads();
@@ -423,7 +437,7 @@ __device__ bool check_lp(const LP<V>& L, const Params& P, Ctl& C, Lds& S,
         if constexpr (F32) {
             const V* ax = C.anchor_avg ? L.xavg() : L.x(cur);
             const V* ay = C.anchor_avg ? L.yavg() : L.y(cur);
-            for (int j = tid; j < n; j += NT) L.xref()[j] += static_cast<double>(ax[j]);
+            for (int j = tid; j < n; j += NT) L.xref()[j] = xfull(L, ax, j);
             for (int i = tid; i < m; i += NT) L.yref()[i] += static_cast<double>(ay[i]);
             __syncthreads();
             for (int j = tid; j < n; j += NT) L.x(0)[j] = L.x(1)[j] = L.xavg()[j] = V(0);
@@ -509,7 +523,7 @@ __device__ void finish_lp(const LP<V>& L, const Params& P, const LPDesc& D,
     const int cur = C.st.cur;
     for (int j = tid; j < n; j += NT) {
         const double sx = src == 2 ? L.xb()[j]
-                          : full(L.xref(), src == 1 ? L.xavg() : L.x(cur), j);
+                          : xfull(L, src == 1 ? L.xavg() : L.x(cur), j);
         L.xout()[j] = sx * L.dc()[j];
     }
     for (int i = tid; i < m; i += NT) {

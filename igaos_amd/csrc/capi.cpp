@@ -318,6 +318,7 @@ void igaos_engine_config_default(igaos_engine_config* cfg) {
     cfg->xcd_pinned = e.xcd_pinned;
     cfg->scaling = e.scaling;
     cfg->exact = e.exact;
+    cfg->small_runner = e.small_runner;
 }
 
 namespace {
@@ -352,6 +353,8 @@ const char* to_config(const igaos_engine_config* cfg, igaos::pdhg::EngineConfig&
     if (cfg->exact != 0 && cfg->exact != 1) return "exact must be 0 or 1";
     if (cfg->exact && cfg->precision != 64) return "exact mode needs precision 64";
     ec.exact = cfg->exact;
+    if (cfg->small_runner < -1 || cfg->small_runner > 1) return "small_runner must be -1, 0 or 1";
+    ec.small_runner = cfg->small_runner;
     return nullptr;
 }
 

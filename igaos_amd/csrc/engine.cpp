@@ -225,7 +225,14 @@ class Engine final : public EngineBase {
     long repair_cg_ = 0, repair_its_ = 0;
     long long pow_filled_ = 0;
     dev::ExactBufs eb_;
+    // small FP64 LPs: one workgroup runs a check period's attempts per
+    // launch (exact.hip, k_x_run), with the exact mode's arithmetic
+    bool runner_ = false;
     static constexpr long long EXACT_POW_CAP = 1LL << 21;
+    // attempts per runner launch: a check period of 64 accepted
+    // iterations takes at most 64 x 21 attempts (pdhg.cu:785: the 21st
+    // retry is accepted)
+    static constexpr int RUNNER_CAP = 64 * 22;
     dev::RowBlocks rA_, rT_;
     double lam2_ = 1.0, eta0_ = 1.0, omega0_ = 1.0;
 
@@ -277,6 +284,27 @@ class Engine final : public EngineBase {
     void zero_v(V* p, long k) { dev::fill(p, k, V(0), s_); }
     // full FP64 scaled iterate (x or y) of a V vector, into dst
     void full(double* dst, const double* ref, const V* d, long k);
+    // the FP32 correction's primal clamp bounds, for the FP64 iterate
+    dev::BoundSnap snap_x() const {
+        dev::BoundSnap b;
+        if constexpr (F32) {
+            b.lo = lo_.p;
+            b.hi = hi_.p;
+            b.scl = scl_.p;
+            b.scu = scu_.p;
+        }
+        return b;
+    }
+    // full() of a primal iterate: corrections on their clamp bounds give
+    // the bounds exactly
+    void full_x(double* dst, const double* ref, const V* d) {
+        if constexpr (F32) {
+            const dev::BoundSnap b = snap_x();
+            dev::combine(ref, d, dst, n_, s_, &b);
+        } else {
+            full(dst, ref, d, n_);
+        }
+    }
 };
 
 template <typename V>
@@ -450,6 +478,22 @@ void Engine<V>::setup() {
     kd_.partC = partC_.p; kd_.partR = partR_.p; kd_.partR2 = partR2_.p;
     kd_.rowres = rowres_.p;
 
+    if constexpr (!F32) {
+        const int sr = cfg_.small_runner;
+        const bool fits = dev::run_exact_fits(n_, m_, nnz) && !ib_.sT.S && !ib_.sA.S;
+        // by default in exact mode only: on the 1,000 x 3,000 LP the runner
+        // takes about 25 us per attempt on its one CU, the three-kernel
+        // attempts of the default mode about 15 us
+        runner_ = fits && (sr == 1 || (sr == -1 && cfg_.exact));
+    }
+    if (runner_ && !cfg_.exact) {  // the runner's per-attempt vectors
+        xdx_.alloc(n_);
+        xdy_.alloc(m_);
+        xtm_.alloc(m_);
+        eb_.dx = xdx_.p;
+        eb_.dy = xdy_.p;
+        eb_.tm = xtm_.p;
+    }
     if (cfg_.exact) {
         xdx_.alloc(n_);
         xdy_.alloc(m_);
@@ -469,7 +513,7 @@ void Engine<V>::setup() {
         eb_.pow_cap = EXACT_POW_CAP;
     }
 
-    const int G = cfg_.graph_slots;
+    const int G = runner_ ? 0 : cfg_.graph_slots;
     if (G > 0) {  // G attempt slots, replayed until the state freezes
         HIPCK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         for (int g = 0; g < G; ++g) attempt(s_);
@@ -573,9 +617,10 @@ void Engine<V>::check_metrics(int cur, Metrics& mc, Metrics& ma) {
         // both FP64 iterates (reference + FP32 correction) formed once, so
         // the KKT SpMVs gather one double per nonzero instead of a double
         // and a float (same values)
-        dev::combine(xref_.p, xcur(cur), kx_[0].p, n_, s_);
+        const dev::BoundSnap b = snap_x();
+        dev::combine(xref_.p, xcur(cur), kx_[0].p, n_, s_, &b);
         dev::combine(yref_.p, ycur(cur), ky_[0].p, m_, s_);
-        dev::combine(xref_.p, xavg_.p, kx_[1].p, n_, s_);
+        dev::combine(xref_.p, xavg_.p, kx_[1].p, n_, s_, &b);
         dev::combine(yref_.p, yavg_.p, ky_[1].p, m_, s_);
         dev::KktIter<double> it;
         for (int q = 0; q < 2; ++q) {
@@ -698,6 +743,12 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
 
     const int G = cfg_.graph_slots;
     auto launch_slots = [&](long slots) {
+        if constexpr (!F32) {
+            if (runner_) {  // attempts until the freeze, in one launch
+                dev::launch_run_exact(ib_, eb_, RUNNER_CAP, s_);
+                return;
+            }
+        }
         if (G > 0) {
             const long reps = (slots + G - 1) / G;
             for (long r = 0; r < reps; ++r) HIPCK(hipGraphLaunch(exec_, s_));
@@ -774,7 +825,7 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
                       !(std::isfinite(t.gap) && t.gap <= TOL)))
                     continue;
                 if (cc == 1) {
-                    full(tmpn_.p, F32 ? xref_.p : nullptr, xavg_.p, n_);
+                    full_x(tmpn_.p, F32 ? xref_.p : nullptr, xavg_.p);
                     full(tmpm_.p, F32 ? yref_.p : nullptr, yavg_.p, m_);
                 } else {
                     copy_d(tmpn_.p, xb_.p, n_);
@@ -850,7 +901,7 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
         if (inc_src >= 0) {
             const double* xr = F32 ? xref_.p : nullptr;
             const double* yr = F32 ? yref_.p : nullptr;
-            full(xb_.p, xr, inc_src ? xavg_.p : xcur(cur), n_);
+            full_x(xb_.p, xr, inc_src ? xavg_.p : xcur(cur));
             full(yb_.p, yr, inc_src ? yavg_.p : ycur(cur), m_);
         }
         // rollback (pdhg.cu:965-988)
@@ -885,11 +936,12 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
             ++restarts;
             if constexpr (F32) {
                 // fold the anchor's correction into the FP64 reference
+                const dev::BoundSnap b = snap_x();
                 if (ea < ec) {
-                    dev::fold(xref_.p, xavg_.p, n_, s_);
+                    dev::fold(xref_.p, xavg_.p, n_, s_, &b);
                     dev::fold(yref_.p, yavg_.p, m_, s_);
                 } else {
-                    dev::fold(xref_.p, xcur(cur), n_, s_);
+                    dev::fold(xref_.p, xcur(cur), n_, s_, &b);
                     dev::fold(yref_.p, ycur(cur), m_, s_);
                 }
                 for (V* p : {x0_.p, x1_.p, xavg_.p}) zero_v(p, n_);
@@ -987,8 +1039,11 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
         } else {
             const V* fx = src == 1 ? xavg_.p : xcur(st.cur);
             const V* fy = src == 1 ? yavg_.p : ycur(st.cur);
-            if constexpr (F32)
-                dev::unscale_out(m_, n_, dc_.p, dr_.p, xr, nullptr, fx, yr, nullptr, fy, ox.p, oy.p, s_);
+            if constexpr (F32) {
+                const dev::BoundSnap b = snap_x();
+                dev::unscale_out(m_, n_, dc_.p, dr_.p, xr, nullptr, fx, yr, nullptr, fy, ox.p,
+                                 oy.p, s_, &b);
+            }
             else
                 dev::unscale_out(m_, n_, dc_.p, dr_.p, nullptr, fx, nullptr, nullptr, fy, nullptr, ox.p, oy.p, s_);
         }

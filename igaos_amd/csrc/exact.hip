@@ -214,6 +214,191 @@ void launch_attempt_exact(const IterBufs<double>& b, const ExactBufs& e, hipStre
     k_x_decide<<<1, XB, 0, s>>>(b, e, gD);
 }
 
+// ------------------------------------------------- one-workgroup runner --
+// Small LPs: one workgroup of RT threads runs line-search attempts until
+// the state freezes (or `cap` attempts), with exactly the arithmetic of the
+// six attempt kernels above: A'y and A xbar summed per row in storage
+// order, the same steps, and the dots in the reference's two-stage layout.
+// Element i of that layout is thread i % 256 of virtual workgroup i / 256;
+// a round of the runner covers virtual workgroups 4q .. 4q + 3 (elements
+// 1024 q + r, r = threadIdx.x) and each of its four 256-thread quarters
+// reduces one of them by the same halving tree (the levels below 64 with
+// shuffles inside the first wavefront: the same additions in the same
+// order).  Element i is stepped and then read back by the same thread, so
+// only the gathers of y and xbar cross threads, after a barrier.  One
+// launch replaces up to a whole check period of six-kernel attempts.
+constexpr int RT = 1024;
+
+// Halving trees of kern_dots / kern_dotfin over the three quantities'
+// 256 values of one quarter (tree[q][base .. base + 255]): every thread of
+// the workgroup calls it; lane 0 of the quarter's first wavefront returns
+// the three sums.
+__device__ __forceinline__ void quarter_trees(double (*tree)[RT], int base, double out[3]) {
+    const int t = threadIdx.x & (XB - 1);
+    __syncthreads();
+    if (t < 128)
+        for (int q = 0; q < 3; ++q) tree[q][base + t] = tree[q][base + t] + tree[q][base + t + 128];
+    __syncthreads();
+    if (t < 64)
+        for (int q = 0; q < 3; ++q) tree[q][base + t] = tree[q][base + t] + tree[q][base + t + 64];
+    __syncthreads();
+    if (t < 64)
+        for (int q = 0; q < 3; ++q) {
+            double a = tree[q][base + t];
+#pragma unroll
+            for (int s = 32; s > 0; s >>= 1) a = a + __shfl_down(a, s, 64);
+            out[q] = a;
+        }
+}
+
+// Products val[p] * x[idx[p]] of all nnz nonzeros into prod[p]: every
+// thread issues its loads back to back (no dependence between them).
+__device__ __forceinline__ void run_products(long nnz, const int* __restrict__ idx,
+                                             const double* __restrict__ val,
+                                             const double* __restrict__ x, double* prod) {
+#pragma unroll 8
+    for (long p = threadIdx.x; p < nnz; p += RT) prod[p] = val[p] * x[idx[p]];
+}
+// Row r of a product staged by run_products, summed in storage order.
+__device__ __forceinline__ double run_row(const int* __restrict__ ptr, const double* prod,
+                                          int r) {
+    double s = 0.0;
+    const int p1 = ptr[r + 1];
+    for (int p = ptr[r]; p < p1; ++p) s = s + prod[p];
+    return s;
+}
+
+__global__ __launch_bounds__(RT) void k_x_run(IterBufs<double> b, ExactBufs e, int gD,
+                                              int cap) {
+    __shared__ LsState sh;
+    // the staged products of one SpMV; the dots' trees reuse the space
+    __shared__ double prod[RUN_MAX_NNZ];
+    __shared__ double part[3][RUN_MAX_GD];
+    double (*tree)[RT] = reinterpret_cast<double (*)[RT]>(prod);
+    const int tid = threadIdx.x;
+    const int n = b.n, m = b.m;
+    const int quarter = tid / XB, t = tid & (XB - 1);
+    if (tid == 0) sh = *b.st;
+    __syncthreads();
+    for (int att = 0; att < cap && !sh.frozen; ++att) {
+        const int cur = sh.cur;
+        const double tau = sh.eta / sh.omega;
+        const double sig = sh.eta * sh.omega;
+        const double w = sh.mix_w;
+        const bool fresh = sh.fresh != 0;
+        const bool mix = fresh && sh.mix_pending != 0;
+        const double* __restrict__ x = b.x[cur];
+        const double* __restrict__ y = b.y[cur];
+        // k_x_ata + k_x_primal
+        if (fresh) run_products(b.cp[n], b.ci, b.acx, y, prod);
+        __syncthreads();
+        for (int j = tid; j < n; j += RT) {
+            double a;
+            if (fresh) {
+                a = run_row(b.cp, prod, j);
+                b.ata[j] = a;
+            } else {
+                a = b.ata[j];
+            }
+            const double xo = x[j];
+            if (mix) {
+                const double xa = b.xavg[j];
+                b.xavg[j] = xa + w * (xo - xa);
+            }
+            const double g = b.gref[j] + a;
+            const double v = fmin(fmax(xo - tau * g, b.lo[j]), b.hi[j]);
+            b.x[cur ^ 1][j] = v;
+            b.xbar[j] = 2.0 * v - xo;
+            e.dx[j] = v - xo;
+        }
+        __syncthreads();  // xbar complete, prod free
+        // k_x_axbar + k_x_dual
+        run_products(b.ap[m], b.ai, b.ax, b.xbar, prod);
+        __syncthreads();
+        for (int i = tid; i < m; i += RT) {
+            const double tm = run_row(b.ap, prod, i);
+            e.tm[i] = tm;
+            const double yo = y[i];
+            double axc = b.axcur[i];
+            if (mix) {
+                const double ya = b.yavg[i];
+                b.yavg[i] = ya + w * (yo - ya);
+                axc = axc + b.adx[i];
+                b.axcur[i] = axc;
+            }
+            const double tt = yo + sig * tm;
+            const double yv = tt - sig * fmin(fmax(tt / sig, b.rlo[i]), b.rhi[i]);
+            b.y[cur ^ 1][i] = yv;
+            e.dy[i] = yv - yo;
+            b.adx[i] = 0.5 * (tm - axc);
+        }
+        __syncthreads();  // prod free for the trees
+        // k_x_dots: virtual workgroups 4q .. 4q + 3 per round
+        for (int q = 0; 4 * q < gD; ++q) {
+            const long i = static_cast<long>(q) * RT + tid;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            if (i < n) a0 = a0 + e.dx[i] * e.dx[i];
+            if (i < m) {
+                a1 = a1 + e.dy[i] * e.dy[i];
+                a2 = a2 + e.dy[i] * b.adx[i];
+            }
+            tree[0][tid] = a0;
+            tree[1][tid] = a1;
+            tree[2][tid] = a2;
+            const int vb = 4 * q + quarter;
+            double r[3] = {0.0, 0.0, 0.0};
+            quarter_trees(tree, quarter * XB, r);
+            if (t == 0 && vb < gD) {
+                part[0][vb] = r[0];
+                part[1][vb] = r[1];
+                part[2][vb] = r[2];
+            }
+            __syncthreads();  // the tree slices are reused by the next round
+        }
+        // k_x_decide: kern_dotfin on the first quarter, then the rule
+        {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            if (quarter == 0)
+                for (int k = t; k < gD; k += XB) {
+                    s0 = s0 + part[0][k];
+                    s1 = s1 + part[1][k];
+                    s2 = s2 + part[2][k];
+                }
+            tree[0][tid] = s0;
+            tree[1][tid] = s1;
+            tree[2][tid] = s2;
+            double d[3] = {0.0, 0.0, 0.0};
+            quarter_trees(tree, quarter * XB, d);
+            const double d0 = d[0], d1 = d[1], d2 = d[2];
+            if (tid == 0) {
+                LsState st = sh;
+                const long long k = st.k;
+                double p3, p6;
+                if (e.powtab && k >= 1 && k <= e.pow_cap) {
+                    p3 = e.powtab[2 * (k - 1)];
+                    p6 = e.powtab[2 * (k - 1) + 1];
+                } else {
+                    p3 = pow(static_cast<double>(k), -0.3);
+                    p6 = pow(static_cast<double>(k), -0.6);
+                }
+                pdhg::ls_decide_pw(st, d0, d1, d2, p3, p6);
+                sh = st;
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) *b.st = sh;
+}
+
+bool run_exact_fits(long n, long m, long nnz) {
+    const long mx = n > m ? n : m;
+    return n > 0 && m >= 0 && nnz <= RUN_MAX_NNZ && (mx + XB - 1) / XB <= RUN_MAX_GD;
+}
+
+void launch_run_exact(const IterBufs<double>& b, const ExactBufs& e, int cap, hipStream_t s) {
+    k_x_run<<<1, RT, 0, s>>>(b, e, dots_blocks(b.n, b.m), cap);
+}
+
 __global__ __launch_bounds__(XB) void k_x_flush(int n, int m, const double* x, double* xavg,
                                                 const double* y, double* yavg,
                                                 const double* adx, double* axcur, double w) {

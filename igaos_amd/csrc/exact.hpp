@@ -25,6 +25,15 @@ struct ExactBufs {
 
 // One line-search attempt (no-ops once the state is frozen).
 void launch_attempt_exact(const IterBufs<double>& b, const ExactBufs& e, hipStream_t s);
+// Small LPs (max(n, m) <= 256 RUN_MAX_GD, nnz <= RUN_MAX_NNZ): one workgroup
+// runs attempts with the arithmetic of launch_attempt_exact until the state
+// freezes or `cap` attempts have run, in one launch; each sparse product is
+// staged in LDS (120 KB) and summed per row from there.  e.powtab may be
+// null (device pow).
+constexpr int RUN_MAX_GD = 64;
+constexpr int RUN_MAX_NNZ = 15360;
+bool run_exact_fits(long n, long m, long nnz);
+void launch_run_exact(const IterBufs<double>& b, const ExactBufs& e, int cap, hipStream_t s);
 // The pending averaging / A x commit of the last accepted step.
 void launch_flush_exact(const IterBufs<double>& b, int cur, double w, hipStream_t s);
 // y = A x, one thread per row, each row summed in storage order.

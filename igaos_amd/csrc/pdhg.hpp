@@ -23,6 +23,9 @@ struct EngineConfig {
     int scaling = 0;         // 0: Ruiz (the reference); 1: Ruiz then Pock-Chambolle (alpha 1)
     int exact = 0;           // 1: exact FP64 mode (reference summation orders, no FMA;
                              // precision 64, iteration-bounded runs only)
+    int small_runner = -1;   // FP64 LPs with max(n, m) <= 16384 and nnz <= 15360: one
+                             // workgroup runs a check period's attempts per launch
+                             // (-1: in exact mode, 0: never, 1: also in the default mode)
 };
 
 struct SolveInfo {

@@ -984,23 +984,38 @@ void to_float(const double* in, float* out, long n, hipStream_t s) {
     if (n > 0) k_to_float<<<elem_grid(n), BS, 0, s>>>(in, out, n);
 }
 
-__global__ void k_combine(const double* ref, const float* d, double* out, long n) {
-    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
-    if (i < n) out[i] = ref[i] + static_cast<double>(d[i]);
-}
-void combine(const double* ref, const float* d, double* out, long n, hipStream_t s) {
-    if (n > 0) k_combine<<<elem_grid(n), BS, 0, s>>>(ref, d, out, n);
+// ref + d, or the bound itself where d sits exactly on its clamp bound
+__device__ __forceinline__ double snapped(const double* ref, const float* d, long i,
+                                          BoundSnap b) {
+    const float v = d[i];
+    if (b.hi) {
+        if (v == b.hi[i]) return b.scu[i];
+        if (v == b.lo[i]) return b.scl[i];
+    }
+    const double dv = static_cast<double>(v);
+    return ref ? ref[i] + dv : dv;
 }
 
-__global__ void k_fold(double* ref, float* d, long n) {
+__global__ void k_combine(const double* ref, const float* d, double* out, long n,
+                          BoundSnap b) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < n) out[i] = snapped(ref, d, i, b);
+}
+void combine(const double* ref, const float* d, double* out, long n, hipStream_t s,
+             const BoundSnap* snap) {
+    if (n > 0)
+        k_combine<<<elem_grid(n), BS, 0, s>>>(ref, d, out, n, snap ? *snap : BoundSnap{});
+}
+
+__global__ void k_fold(double* ref, float* d, long n, BoundSnap b) {
     const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
     if (i < n) {
-        ref[i] += static_cast<double>(d[i]);
+        ref[i] = snapped(ref, d, i, b);
         d[i] = 0.0f;
     }
 }
-void fold(double* ref, float* d, long n, hipStream_t s) {
-    if (n > 0) k_fold<<<elem_grid(n), BS, 0, s>>>(ref, d, n);
+void fold(double* ref, float* d, long n, hipStream_t s, const BoundSnap* snap) {
+    if (n > 0) k_fold<<<elem_grid(n), BS, 0, s>>>(ref, d, n, snap ? *snap : BoundSnap{});
 }
 
 __global__ void k_refine_data(int m, int n, const double* scl,
@@ -1040,11 +1055,16 @@ __global__ void k_unscale_out(int m, int n, const double* dc, const double* dr,
                               const double* xref, const double* xd64,
                               const float* xd32, const double* yref,
                               const double* yd64, const float* yd32,
-                              double* x, double* y) {
+                              double* x, double* y, BoundSnap b) {
     const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
     if (i < n) {
-        double sx = xd64 ? xd64[i] : static_cast<double>(xd32[i]);
-        if (xref) sx = xref[i] + sx;
+        double sx;
+        if (xd64) {
+            sx = xd64[i];
+            if (xref) sx = xref[i] + sx;
+        } else {
+            sx = snapped(xref, xd32, i, b);
+        }
         x[i] = sx * dc[i];
     }
     if (i < m) {
@@ -1056,11 +1076,12 @@ __global__ void k_unscale_out(int m, int n, const double* dc, const double* dr,
 void unscale_out(int m, int n, const double* dc, const double* dr,
                  const double* xref, const double* xd64, const float* xd32,
                  const double* yref, const double* yd64, const float* yd32,
-                 double* x, double* y, hipStream_t s) {
+                 double* x, double* y, hipStream_t s, const BoundSnap* snap) {
     const long mx = m > n ? m : n;
     if (mx > 0)
-        k_unscale_out<<<elem_grid(mx), BS, 0, s>>>(m, n, dc, dr, xref, xd64,
-                                                   xd32, yref, yd64, yd32, x, y);
+        k_unscale_out<<<elem_grid(mx), BS, 0, s>>>(m, n, dc, dr, xref, xd64, xd32, yref,
+                                                   yd64, yd32, x, y,
+                                                   snap ? *snap : BoundSnap{});
 }
 
 }  // namespace igaos::dev

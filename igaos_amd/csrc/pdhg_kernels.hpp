@@ -179,10 +179,23 @@ void fill(double* v, long n, double val, hipStream_t s);
 void fill(float* v, long n, float val, hipStream_t s);
 void scale_into(double* out, const double* in, long n, double f, hipStream_t s);
 void to_float(const double* in, float* out, long n, hipStream_t s);
+// FP64 iterate of an FP32 correction: out = ref + d.  With `snap` (the
+// primal side: the correction's clamp bounds lo, hi and the scaled column
+// bounds scl, scu), a correction exactly at lo or hi stands for the column
+// at its bound and gives scl or scu exactly: ref + (double)hi differs from
+// scu by the FP32 rounding of scu - ref, up to 6e-8 |scu - ref|, which the
+// KKT test's bound tolerance (1e-9 (1 + |bound|), pdhg.cu:206-215) does
+// not absorb, so the column would count as dual infeasible.
+struct BoundSnap {
+    const float* lo = nullptr;
+    const float* hi = nullptr;
+    const double* scl = nullptr;
+    const double* scu = nullptr;
+};
 void combine(const double* ref, const float* d, double* out, long n,
-             hipStream_t s);
-// out = ref + d for each, in place on ref; d zeroed.
-void fold(double* ref, float* d, long n, hipStream_t s);
+             hipStream_t s, const BoundSnap* snap = nullptr);
+// ref = ref + d (snapped as combine does), in place on ref; d zeroed.
+void fold(double* ref, float* d, long n, hipStream_t s, const BoundSnap* snap = nullptr);
 // Shifted FP32 data of the correction problem (see engine.cpp).
 void refine_data(int m, int n, const double* scl, const double* scu,
                  const double* sc, const double* xref, const double* ata_ref,
@@ -190,10 +203,11 @@ void refine_data(int m, int n, const double* scl, const double* scu,
                  const double* aref, const double* yref, float* lo, float* hi,
                  float* gref, float* rlo, float* rhi, float* yoff,
                  hipStream_t s);
-// Original-unit output: x = dc * sx (sx = ref + d), y = sy / dr.
+// Original-unit output: x = dc * sx (sx = ref + d, snapped as combine
+// does when `snap` is given), y = sy / dr.
 void unscale_out(int m, int n, const double* dc, const double* dr,
                  const double* xref, const double* xd64, const float* xd32,
                  const double* yref, const double* yd64, const float* yd32,
-                 double* x, double* y, hipStream_t s);
+                 double* x, double* y, hipStream_t s, const BoundSnap* snap = nullptr);
 
 }  // namespace igaos::dev

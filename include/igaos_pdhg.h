@@ -14,7 +14,7 @@
 extern "C" {
 #endif
 
-#define IGAOS_PDHG_ABI_VERSION 2
+#define IGAOS_PDHG_ABI_VERSION 3
 
 /* Status codes: the order of igaos::Status (include/igaos/status.h:5-14). */
 enum {
@@ -133,6 +133,16 @@ typedef struct {
                           iteration-bounded runs only (time_limit_s >= 1e6
                           or <= 0, max_iterations <= 2^21); far slower than
                           the default kernels.  0: default. */
+    int small_runner;  /* FP64 LPs with max(n, m) <= 16384 and nnz <=
+                          15360: one workgroup of 1024 threads runs the
+                          attempts of a whole check period in one launch,
+                          with the exact mode's arithmetic (each sparse
+                          product staged in LDS and summed per row in
+                          storage order, the reference's dot layout).
+                          -1 (default): in exact mode (bit-identical to its
+                          six kernels per attempt, and faster); 1: also in
+                          the default mode (slower there than its three
+                          kernels per attempt); 0: never. */
 } igaos_engine_config;
 void igaos_engine_config_default(igaos_engine_config* cfg);
 

@@ -47,7 +47,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_and_status_names():
     L = _core.lib()
-    assert L.igaos_pdhg_abi_version() == 2
+    assert L.igaos_pdhg_abi_version() == 3
     names = [L.igaos_status_name(i).decode() for i in range(8)]
     assert names == ["optimal", "near-optimal", "feasible", "infeasible",
                      "unbounded", "iteration-limit", "time-limit", "error"]

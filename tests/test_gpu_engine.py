@@ -122,11 +122,16 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(1.0, np.abs(b).max())
 
 
+# runner 0: three launches per attempt (row-block products); runner 1: the
+# one-workgroup runner (rows summed in storage order, the reference's dot
+# layout); the LP fits both
+@pytest.mark.parametrize("runner", [0, 1])
 @pytest.mark.parametrize("K", [16, 64])
-def test_fixed_iteration_trajectory_parity(K):
+def test_fixed_iteration_trajectory_parity(K, runner):
     lp = lpgen.gen_random_eq(seed=0)
     r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
-    s = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
+    s = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K,
+                           small_runner=runner)
     assert s.iterations == r.iterations == K
     assert s.info["attempts"] == r.attempts
     assert s.info["restarts"] == r.restarts
@@ -138,7 +143,8 @@ def test_fixed_iteration_trajectory_parity(K):
     assert math.isclose(s.info["omega0"], r.omega0, rel_tol=1e-14)
 
 
-def test_trajectory_stays_close_through_two_checks():
+@pytest.mark.parametrize("runner", [0, 1])
+def test_trajectory_stays_close_through_two_checks(runner):
     # past one check period the adaptive line search amplifies rounding
     # differences about 10x every 8 iterations (measured on the oracle
     # alone: a 1e-16 perturbation grows to 1.7e-9 by k = 128; engine vs
@@ -147,7 +153,8 @@ def test_trajectory_stays_close_through_two_checks():
     lp = lpgen.gen_random_eq(seed=0)
     K = 128
     r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
-    s = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
+    s = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K,
+                           small_runner=runner)
     assert s.iterations == r.iterations == K
     assert s.info["restarts"] == r.restarts
     assert _rel(s.x, r.x) <= 1e-4 and _rel(s.y, r.y) <= 1e-4
@@ -159,11 +166,12 @@ def _check_solution(s, ref_obj, tol=1e-6):
     assert abs(s.objective - ref_obj) <= tol * (1 + abs(ref_obj)), (s.objective, ref_obj)
 
 
-@pytest.mark.parametrize("precision", [64, 32])
-def test_random_eq_solves_to_fixture(precision, objectives):
+@pytest.mark.parametrize("precision,runner", [(64, 0), (64, 1), (32, -1)])
+def test_random_eq_solves_to_fixture(precision, runner, objectives):
     lp = lpgen.gen_random_eq(seed=0)
     ent = objectives["gen_random_eq(seed=0)"]
-    s = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=60, precision=precision)
+    s = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=60, precision=precision,
+                           small_runner=runner)
     _check_solution(s, ent["ref_simplex"])
     _check_solution(s, ent["scipy_highs"])
     # the reported pair satisfies the KKT bar in the oracle's own evaluation

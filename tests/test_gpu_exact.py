@@ -36,11 +36,16 @@ def _netlib(name):
     return igaos_amd.read_mps(netlib_path(name)).to_lp(name)
 
 
-def _pair(lp, tol, maxit):
+def _engine(lp, tol, maxit, runner=-1):
+    # runner -1 / 1: the one-workgroup runner where the LP fits it (every
+    # case here); 0: six kernels per attempt.  Both sum alike.
+    return igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=tol, max_iterations=maxit,
+                              exact=True, small_runner=runner)
+
+
+def _pair(lp, tol, maxit, runner=-1):
     ref = oracle.solve(lp, time_limit=1e7, tolerance=tol, max_iterations=maxit)
-    got = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=tol, max_iterations=maxit,
-                             exact=True)
-    return ref, got
+    return ref, _engine(lp, tol, maxit, runner)
 
 
 def _assert_identical(ref, got):
@@ -62,12 +67,13 @@ def _assert_identical(ref, got):
     assert np.array_equal(got.y, ref.y), np.abs(got.y - ref.y).max()
 
 
+@pytest.mark.parametrize("runner", [0, 1])
 @pytest.mark.parametrize("iters", [1, 2, 63, 64, 65, 128, 1000])
-def test_exact_trajectory_random_lp(iters):
+def test_exact_trajectory_random_lp(iters, runner):
     # tolerance 0: the checks run but never stop the solve, so the run
     # ends at `iters` with restarts, weight updates and the incumbent in it
     lp = lpgen.gen_random_eq(seed=3, m=120, n=300)
-    ref, got = _pair(lp, 0.0, iters)
+    ref, got = _pair(lp, 0.0, iters, runner)
     _assert_identical(ref, got)
 
 
@@ -93,6 +99,7 @@ def test_exact_long_runs_with_check_time_paths(name, tol, maxit, event):
     assert getattr(ref, event), f"the oracle no longer takes the {event} path on {name}"
     assert ref.restarts >= 20 and ref.omega != ref.omega0
     _assert_identical(ref, got)
+    _assert_identical(ref, _engine(lp, tol, maxit, 0))
 
 
 @pytest.mark.parametrize("name", ["afiro", "sc105", "share2b"])
