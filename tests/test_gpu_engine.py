@@ -80,7 +80,7 @@ def test_spmv_f64_matches_oracle(case):
         mats = [(ptr, idx, val, n)]
     else:
         lp = (lpgen.gen_random_eq(seed=3) if case == "random_eq"
-              else lpgen.gen_zipf_badly_scaled(seed=3, m=2000, n=5000))
+              else lpgen.gen_zipf_badly_scaled(seed=3, m=2000, n=5000, kmax=500))
         mats = [(lp.ap, lp.ai, lp.ax, lp.n), (lp.cp, lp.ci, lp.acx, lp.m)]
     for ptr, idx, val, ncols in mats:
         x = rng.standard_normal(ncols)
@@ -227,7 +227,8 @@ def test_iteration_cap_and_time_limit():
 
 
 def test_run_to_run_bit_identical():
-    lp = lpgen.gen_zipf_badly_scaled(seed=1, m=2000, n=5000)
+    # columns of at most m/4 rows: distinct rows are drawn quickly
+    lp = lpgen.gen_zipf_badly_scaled(seed=1, m=2000, n=5000, kmax=500)
     kw = dict(tolerance=0.0, time_limit=1e7, max_iterations=3000)
     a = igaos_amd.solve_lp(lp, **kw)
     b = igaos_amd.solve_lp(lp, **kw)
