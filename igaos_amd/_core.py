@@ -31,7 +31,7 @@ class EngineConfig(C.Structure):
                 ("max_repair_m", C.c_int), ("device", C.c_int),
                 ("column_slices", C.c_int), ("xcd_pinned", C.c_int),
                 ("scaling", C.c_int), ("exact", C.c_int),
-                ("small_runner", C.c_int)]
+                ("small_runner", C.c_int), ("stream_loads", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -92,6 +92,8 @@ SIGNATURES = {
     "igaos_rowblocks_build": (C.c_int, [C.c_int, _ip, _ip, C.c_int]),
     "igaos_slices_build": (C.c_int, [C.c_int, C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip,
                                      _ip, _ip, C.c_int]),
+    "igaos_slices_gather_order": (C.c_int, [C.c_int, C.c_int, _ip, _ip, C.c_int, _ip, _ip,
+                                            _vp]),
     "igaos_spmv_csr_f64": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),
     "igaos_spmv_csr_f32": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,

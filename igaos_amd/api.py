@@ -283,11 +283,11 @@ class Engine:
 
     def __init__(self, lp, precision=64, graph_slots=None, max_repair_m=None,
                  column_slices=None, exact=False, xcd_pinned=None, scaling="ruiz",
-                 small_runner=None):
+                 small_runner=None, stream_loads=None):
         self.model = lp if isinstance(lp, Model) else model_from_lp(lp)
         self.precision = precision
         cfg = _config(precision, graph_slots, max_repair_m, column_slices, exact,
-                      xcd_pinned, scaling, small_runner)
+                      xcd_pinned, scaling, small_runner, stream_loads)
         h = _core.lib().igaos_pdhg_create(self.model._h, C.byref(cfg))
         if not h:
             raise RuntimeError(_core.last_error())
@@ -361,7 +361,7 @@ SCALINGS = {"ruiz": 0, "ruiz+pock-chambolle": 1}
 
 def _config(precision=64, graph_slots=None, max_repair_m=None,
             column_slices=None, exact=False, xcd_pinned=None,
-            scaling="ruiz", small_runner=None) -> EngineConfig:
+            scaling="ruiz", small_runner=None, stream_loads=None) -> EngineConfig:
     c = EngineConfig()
     _core.lib().igaos_engine_config_default(C.byref(c))
     c.precision = int(precision)
@@ -379,6 +379,8 @@ def _config(precision=64, graph_slots=None, max_repair_m=None,
         c.column_slices = int(column_slices)
     if small_runner is not None:
         c.small_runner = int(small_runner)
+    if stream_loads is not None:
+        c.stream_loads = int(stream_loads)
     return c
 
 
@@ -404,7 +406,8 @@ def solve(path, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
 def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
              precision=64, verbosity=0, graph_slots=None,
              max_repair_m=None, column_slices=None, exact=False,
-             xcd_pinned=None, scaling="ruiz", small_runner=None) -> Solution:
+             xcd_pinned=None, scaling="ruiz", small_runner=None,
+             stream_loads=None) -> Solution:
     """Solve an in-memory LP (``lpgen.LP`` or ``Model``) with the engine.
 
     ``precision=64`` is the plain FP64 engine; ``precision=32`` runs FP32
@@ -415,13 +418,15 @@ def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
     ``small_runner`` (-1: exact mode only, 0: never, 1: also the default
     mode): FP64 LPs with max(n, m) <= 16384 and nnz <= 15360 run a check
     period of attempts per launch in one workgroup, with the exact mode's
-    arithmetic.
+    arithmetic.  ``stream_loads`` (-1: nontemporal when both matrices
+    exceed the last-level cache, 0: cached, 1: nontemporal) picks how the
+    attempt kernels load the matrix streams; results do not depend on it.
     """
     model = lp if isinstance(lp, Model) else model_from_lp(lp)
     return _run(model, _options(time_limit, tolerance, max_iterations, True,
                                 0, verbosity),
                 _config(precision, graph_slots, max_repair_m, column_slices, exact,
-                        xcd_pinned, scaling, small_runner))
+                        xcd_pinned, scaling, small_runner, stream_loads))
 
 
 has_pdhg = _has_pdhg()

@@ -319,6 +319,7 @@ void igaos_engine_config_default(igaos_engine_config* cfg) {
     cfg->scaling = e.scaling;
     cfg->exact = e.exact;
     cfg->small_runner = e.small_runner;
+    cfg->stream_loads = e.stream_loads;
 }
 
 namespace {
@@ -355,6 +356,8 @@ const char* to_config(const igaos_engine_config* cfg, igaos::pdhg::EngineConfig&
     ec.exact = cfg->exact;
     if (cfg->small_runner < -1 || cfg->small_runner > 1) return "small_runner must be -1, 0 or 1";
     ec.small_runner = cfg->small_runner;
+    if (cfg->stream_loads < -1 || cfg->stream_loads > 1) return "stream_loads must be -1, 0 or 1";
+    ec.stream_loads = cfg->stream_loads;
     return nullptr;
 }
 
@@ -618,6 +621,26 @@ int igaos_slices_build(int nrows, int ncols, const int* ptr, const int* idx, int
         if (blocks) std::memcpy(blocks, h.quad.data(), sizeof(int) * h.quad.size());
         if (bslice) std::memcpy(bslice, h.bslice.data(), sizeof(int) * nb);
         return nb;
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+}
+
+int igaos_slices_gather_order(int nrows, int ncols, const int* ptr, const int* idx,
+                              int slices, int* sidx, int* src, unsigned short* slot) {
+    if (nrows < 0 || ncols < 0 || !ptr || (ptr[nrows] > 0 && !idx))
+        return fail("invalid CSR arrays");
+    if (slices < 1 || slices > igaos::dev::MAX_SLICES) return fail("slices must be 1..16");
+    const std::string bad = igaos::io::check_compressed(nrows, ncols, ptr, idx);
+    if (!bad.empty()) return fail(bad);
+    try {
+        const igaos::dev::HostSlices h =
+            igaos::dev::build_slices(nrows, ncols, ptr, idx, slices, true);
+        const size_t nnz = h.idx.size();
+        if (sidx) std::memcpy(sidx, h.idx.data(), sizeof(int) * nnz);
+        if (src) std::memcpy(src, h.src.data(), sizeof(int) * nnz);
+        if (slot) std::memcpy(slot, h.slot.data(), sizeof(unsigned short) * nnz);
+        return static_cast<int>(nnz);
     } catch (const std::exception& e) {
         return fail(e.what());
     }

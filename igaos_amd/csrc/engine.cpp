@@ -211,6 +211,7 @@ class Engine final : public EngineBase {
     // column-sliced copies of A (gathers x) and A' (gathers y), when used
     struct SliceBufs {
         DBuf<int> ptr, idx, quad, bslice, xmap;
+        DBuf<unsigned short> slot;
         DBuf<V> val, part;
     } slA_, slT_;
     dev::IterBufs<V> ib_;
@@ -240,7 +241,7 @@ class Engine final : public EngineBase {
     int slices_for(int cols, int rows, long nnz) const;
     void make_slices(dev::Sliced<V>& sl, SliceBufs& B, int rows, int cols,
                      const std::vector<int>& ptr, const std::vector<int>& idx,
-                     const double* sval, int S);
+                     const double* sval, int S, bool gather_order);
     void reset();
     double power_iteration();
     double dsum(const double* a, const double* b, long n);
@@ -420,8 +421,11 @@ void Engine<V>::setup() {
     xb_.alloc(n_); yb_.alloc(m_); pesx_.alloc(n_); pesy_.alloc(m_);
     {
         const int sT = slices_for(m_, n_, nnz), sA = slices_for(n_, m_, nnz);
-        if (sT) make_slices(ib_.sT, slT_, n_, m_, lp_.cp, lp_.ci, sacx_.p, sT);
-        if (sA) make_slices(ib_.sA, slA_, m_, n_, lp_.ap, lp_.ai, sax_.p, sA);
+        // A's slices gather x: on a column-renumbered copy its first columns
+        // are the heavy ones, and blocks sorted by column gather them from
+        // few cache lines; y has no such order, so A' keeps row order
+        if (sT) make_slices(ib_.sT, slT_, n_, m_, lp_.cp, lp_.ci, sacx_.p, sT, false);
+        if (sA) make_slices(ib_.sA, slA_, m_, n_, lp_.ap, lp_.ai, sax_.p, sA, own_ != nullptr);
         ib_.nT_part = sT ? (n_ + dev::BS - 1) / dev::BS : rT_.nblk;
         ib_.nA_part = sA ? std::max((m_ + dev::BS - 1) / dev::BS, 1) : std::max(rA_.nblk, 1);
     }
@@ -443,6 +447,12 @@ void Engine<V>::setup() {
     ib_.axcur = axcur_.p; ib_.adx = adx_.p;
     ib_.partT = partT_.p; ib_.partA = partA_.p;
     ib_.st = st_.p;
+    // matrices that cannot stay in the 256 MB last-level cache between
+    // iterations stream with nontemporal loads (about 6 TB/s against 4 TB/s
+    // for cached loads on gfx950, and the gathered vectors keep L2)
+    ib_.nt = cfg_.stream_loads == 1 ||
+             (cfg_.stream_loads < 0 &&
+              2.0 * static_cast<double>(nnz) * (sizeof(V) + sizeof(int)) > 256.0e6);
     if constexpr (F32) {
         vax_.alloc(nnz);
         vacx_.alloc(nnz);
@@ -549,13 +559,15 @@ int Engine<V>::slices_for(int cols, int rows, long nnz) const {
 template <typename V>
 void Engine<V>::make_slices(dev::Sliced<V>& sl, SliceBufs& B, int rows, int cols,
                             const std::vector<int>& ptr, const std::vector<int>& idx,
-                            const double* sval, int S) {
-    const dev::HostSlices h = dev::build_slices(rows, cols, ptr.data(), idx.data(), S);
+                            const double* sval, int S, bool gather_order) {
+    const dev::HostSlices h =
+        dev::build_slices(rows, cols, ptr.data(), idx.data(), S, gather_order);
     const long nnz = static_cast<long>(h.idx.size());
     B.ptr.upload(h.ptr.data(), h.ptr.size(), s_);
     B.idx.upload(h.idx.data(), h.idx.size(), s_);
     B.quad.upload(h.quad.data(), h.quad.size(), s_);
     B.bslice.upload(h.bslice.data(), h.bslice.size(), s_);
+    if (gather_order) B.slot.upload(h.slot.data(), h.slot.size(), s_);
     B.val.alloc(nnz);
     B.part.alloc(static_cast<size_t>(S) * rows);
     {
@@ -582,6 +594,7 @@ void Engine<V>::make_slices(dev::Sliced<V>& sl, SliceBufs& B, int rows, int cols
     sl.ptr = B.ptr.p;
     sl.idx = B.idx.p;
     sl.val = B.val.p;
+    sl.slot = gather_order ? B.slot.p : nullptr;
     sl.quad = reinterpret_cast<const int4*>(B.quad.p);
     sl.bslice = B.bslice.p;
     sl.part = B.part.p;

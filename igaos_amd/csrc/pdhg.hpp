@@ -26,6 +26,8 @@ struct EngineConfig {
     int small_runner = -1;   // FP64 LPs with max(n, m) <= 16384 and nnz <= 15360: one
                              // workgroup runs a check period's attempts per launch
                              // (-1: in exact mode, 0: never, 1: also in the default mode)
+    int stream_loads = -1;   // matrix streams of the attempt kernels: -1 nontemporal when
+                             // both matrices exceed the last-level cache, 0 cached, 1 nontemporal
 };
 
 struct SolveInfo {

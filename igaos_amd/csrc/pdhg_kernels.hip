@@ -20,6 +20,9 @@
 // column-sliced (k_primal_slices / k_dual_slices plus an epilogue kernel
 // with the same step, below).
 
+#include <algorithm>
+#include <numeric>
+
 #include "dev_common.hpp"
 #include "pdhg_kernels.hpp"
 
@@ -100,7 +103,7 @@ void spmv_ref(const RowBlocks& rb, const int* ptr, const int* idx,
 
 // ----------------------------------------------------- primal / dual --
 
-template <typename V>
+template <typename V, bool NTL>
 __global__ __launch_bounds__(BS) void k_primal(IterBufs<V> b) {
     __shared__ V s_prod[NNZ_MAX];
     __shared__ V s_rows[ROWS_MAX];
@@ -119,8 +122,8 @@ __global__ __launch_bounds__(BS) void k_primal(IterBufs<V> b) {
     int r0, r1, p0, p1;
     block_rows(b.rbT, blockIdx.x, r0, r1, p0, p1);
     if (fresh)
-        rowblock_sums<V>(b.cp, b.ci, b.acx, GatherPlain<V>{y}, r0, r1, p0, p1, s_rows,
-                         s_prod, red_v);
+        rowblock_sums_t<NTL, V>(b.cp, b.ci, b.acx, GatherPlain<V>{y}, r0, r1, p0, p1, s_rows,
+                                s_prod, red_v, nullptr);
     double acc = 0.0;
     for (int t = threadIdx.x; t < r1 - r0; t += BS) {
         const int j = r0 + t;
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(BS) void k_primal(IterBufs<V> b) {
     if (threadIdx.x == 0) b.partT[blockIdx.x] = acc;
 }
 
-template <typename V>
+template <typename V, bool NTL>
 __global__ __launch_bounds__(BS) void k_dual(IterBufs<V> b) {
     __shared__ V s_prod[NNZ_MAX];
     __shared__ V s_rows[ROWS_MAX];
@@ -163,8 +166,8 @@ __global__ __launch_bounds__(BS) void k_dual(IterBufs<V> b) {
     V* __restrict__ yn = b.y[cur ^ 1];
     int r0, r1, p0, p1;
     block_rows(b.rbA, blockIdx.x, r0, r1, p0, p1);
-    rowblock_sums<V>(b.ap, b.ai, b.ax, GatherPlain<V>{b.xbar}, r0, r1, p0, p1, s_rows,
-                     s_prod, red_v);
+    rowblock_sums_t<NTL, V>(b.ap, b.ai, b.ax, GatherPlain<V>{b.xbar}, r0, r1, p0, p1, s_rows,
+                            s_prod, red_v, nullptr);
     double a1 = 0.0, a2 = 0.0;
     for (int t = threadIdx.x; t < r1 - r0; t += BS) {
         const int i = r0 + t;
@@ -232,25 +235,26 @@ __global__ __launch_bounds__(BS) void k_flush(int n, int m, const V* x,
 // partials in slice order and applies the same primal / dual step as
 // k_primal / k_dual, one thread per column / row.
 
-template <typename V>
+template <bool NTL, typename V>
 __device__ __forceinline__ void slice_rows(const Sliced<V>& sl, int blk, const V* x,
                                            V* s_prod, V* s_rows, V* red_v) {
     const int4 q = sl.quad[blk];
     const int sc = sl.bslice[blk];
-    rowblock_sums<V>(sl.ptr + static_cast<long>(sc) * (sl.rows + 1), sl.idx, sl.val,
-                     GatherPlain<V>{x}, q.x, q.y, q.z, q.w, s_rows, s_prod, red_v);
+    rowblock_sums_t<NTL, V>(sl.ptr + static_cast<long>(sc) * (sl.rows + 1), sl.idx, sl.val,
+                            GatherPlain<V>{x}, q.x, q.y, q.z, q.w, s_rows, s_prod, red_v,
+                            sl.slot);
     V* out = sl.part + static_cast<long>(sc) * sl.rows;
     for (int t = threadIdx.x; t < q.y - q.x; t += BS) out[q.x + t] = s_rows[t];
 }
 
-template <typename V>
+template <typename V, bool NTL>
 __global__ __launch_bounds__(BS) void k_primal_slices(IterBufs<V> b, int b0) {
     __shared__ V s_prod[NNZ_MAX];
     __shared__ V s_rows[ROWS_MAX];
     __shared__ V red_v[BS / WAVE];
     const LsState st = *b.st;
     if (st.frozen || !st.fresh) return;  // a retry reuses the stored A'y
-    slice_rows(b.sT, b0 + static_cast<int>(blockIdx.x), b.y[st.cur], s_prod, s_rows, red_v);
+    slice_rows<NTL>(b.sT, b0 + static_cast<int>(blockIdx.x), b.y[st.cur], s_prod, s_rows, red_v);
 }
 
 template <typename V>
@@ -291,13 +295,13 @@ __global__ __launch_bounds__(BS) void k_primal_epi(IterBufs<V> b) {
     if (threadIdx.x == 0) b.partT[blockIdx.x] = acc;
 }
 
-template <typename V>
+template <typename V, bool NTL>
 __global__ __launch_bounds__(BS) void k_dual_slices(IterBufs<V> b, int b0) {
     __shared__ V s_prod[NNZ_MAX];
     __shared__ V s_rows[ROWS_MAX];
     __shared__ V red_v[BS / WAVE];
     if (b.st->frozen) return;
-    slice_rows(b.sA, b0 + static_cast<int>(blockIdx.x), b.xbar, s_prod, s_rows, red_v);
+    slice_rows<NTL>(b.sA, b0 + static_cast<int>(blockIdx.x), b.xbar, s_prod, s_rows, red_v);
 }
 
 template <typename V>
@@ -359,7 +363,8 @@ void gather_values(const double* src_val, const int* src, float* dst, long nnz,
         k_gather_values<float><<<(nnz + BS - 1) / BS, BS, 0, s>>>(src_val, src, dst, nnz);
 }
 
-HostSlices build_slices(int rows, int cols, const int* ptr, const int* idx, int S) {
+HostSlices build_slices(int rows, int cols, const int* ptr, const int* idx, int S,
+                        bool gather_order) {
     HostSlices h;
     h.S = S;
     const long nnz = ptr[rows];
@@ -414,6 +419,34 @@ HostSlices build_slices(int rows, int cols, const int* ptr, const int* idx, int 
             h.bslice.push_back(sc);
         }
     }
+    if (gather_order) {
+        // sort each block of two or more rows by column: a wavefront's
+        // gathers then fall on few cache lines wherever columns are hot
+        // (the renumbered heavy columns); slot restores row-major order
+        h.slot.assign(nnz, 0);
+        std::vector<int> ord, ti, ts;
+        for (size_t k = 0; k < h.bslice.size(); ++k) {
+            const int r0 = h.quad[4 * k], r1 = h.quad[4 * k + 1];
+            const int z0 = h.quad[4 * k + 2], z1 = h.quad[4 * k + 3];
+            const int nz = z1 - z0;
+            if (nz > NNZ_MAX) continue;  // a long single row: summed in storage order
+            ord.resize(nz);
+            std::iota(ord.begin(), ord.end(), 0);
+            if (r1 - r0 > 1)
+                std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+                    return h.idx[z0 + a] < h.idx[z0 + b];
+                });
+            ti.resize(nz);
+            ts.resize(nz);
+            for (int t = 0; t < nz; ++t) {
+                ti[t] = h.idx[z0 + ord[t]];
+                ts[t] = h.src[z0 + ord[t]];
+                h.slot[z0 + t] = static_cast<unsigned short>(ord[t]);
+            }
+            std::copy(ti.begin(), ti.end(), h.idx.begin() + z0);
+            std::copy(ts.begin(), ts.end(), h.src.begin() + z0);
+        }
+    }
     return h;
 }
 
@@ -438,7 +471,7 @@ std::vector<int> build_xcd_map(const std::vector<int>& bslice, int S) {
     return map;
 }
 
-template <typename V>
+template <typename V, bool NTL>
 __global__ __launch_bounds__(BS) void k_primal_slices_x(IterBufs<V> b) {
     __shared__ V s_prod[NNZ_MAX];
     __shared__ V s_rows[ROWS_MAX];
@@ -447,10 +480,10 @@ __global__ __launch_bounds__(BS) void k_primal_slices_x(IterBufs<V> b) {
     if (st.frozen || !st.fresh) return;  // a retry reuses the stored A'y
     const int blk = b.sT.xmap[blockIdx.x];
     if (blk < 0) return;
-    slice_rows(b.sT, blk, b.y[st.cur], s_prod, s_rows, red_v);
+    slice_rows<NTL>(b.sT, blk, b.y[st.cur], s_prod, s_rows, red_v);
 }
 
-template <typename V>
+template <typename V, bool NTL>
 __global__ __launch_bounds__(BS) void k_dual_slices_x(IterBufs<V> b) {
     __shared__ V s_prod[NNZ_MAX];
     __shared__ V s_rows[ROWS_MAX];
@@ -458,38 +491,48 @@ __global__ __launch_bounds__(BS) void k_dual_slices_x(IterBufs<V> b) {
     if (b.st->frozen) return;
     const int blk = b.sA.xmap[blockIdx.x];
     if (blk < 0) return;
-    slice_rows(b.sA, blk, b.xbar, s_prod, s_rows, red_v);
+    slice_rows<NTL>(b.sA, blk, b.xbar, s_prod, s_rows, red_v);
 }
 
-template <typename V>
-static void primal_impl(const IterBufs<V>& b, hipStream_t s) {
+template <typename V, bool NTL>
+static void primal_launch(const IterBufs<V>& b, hipStream_t s) {
     if (b.sT.S && b.sT.xmap) {
-        if (b.sT.xgrid > 0) k_primal_slices_x<V><<<b.sT.xgrid, BS, 0, s>>>(b);
+        if (b.sT.xgrid > 0) k_primal_slices_x<V, NTL><<<b.sT.xgrid, BS, 0, s>>>(b);
         if (b.nT_part > 0) k_primal_epi<V><<<b.nT_part, BS, 0, s>>>(b);
     } else if (b.sT.S) {
         for (int sc = 0; sc < b.sT.S; ++sc) {
             const int nb = b.sT.sb[sc + 1] - b.sT.sb[sc];
-            if (nb > 0) k_primal_slices<V><<<nb, BS, 0, s>>>(b, b.sT.sb[sc]);
+            if (nb > 0) k_primal_slices<V, NTL><<<nb, BS, 0, s>>>(b, b.sT.sb[sc]);
         }
         if (b.nT_part > 0) k_primal_epi<V><<<b.nT_part, BS, 0, s>>>(b);
     } else if (b.rbT.nblk > 0) {
-        k_primal<V><<<b.rbT.nblk, BS, 0, s>>>(b);
+        k_primal<V, NTL><<<b.rbT.nblk, BS, 0, s>>>(b);
     }
 }
 template <typename V>
-static void dual_impl(const IterBufs<V>& b, hipStream_t s) {
+static void primal_impl(const IterBufs<V>& b, hipStream_t s) {
+    if (b.nt) primal_launch<V, true>(b, s);
+    else primal_launch<V, false>(b, s);
+}
+template <typename V, bool NTL>
+static void dual_launch(const IterBufs<V>& b, hipStream_t s) {
     if (b.sA.S && b.sA.xmap) {
-        if (b.sA.xgrid > 0) k_dual_slices_x<V><<<b.sA.xgrid, BS, 0, s>>>(b);
+        if (b.sA.xgrid > 0) k_dual_slices_x<V, NTL><<<b.sA.xgrid, BS, 0, s>>>(b);
         k_dual_epi<V><<<grid_of(b.nA_part), BS, 0, s>>>(b);
     } else if (b.sA.S) {
         for (int sc = 0; sc < b.sA.S; ++sc) {
             const int nb = b.sA.sb[sc + 1] - b.sA.sb[sc];
-            if (nb > 0) k_dual_slices<V><<<nb, BS, 0, s>>>(b, b.sA.sb[sc]);
+            if (nb > 0) k_dual_slices<V, NTL><<<nb, BS, 0, s>>>(b, b.sA.sb[sc]);
         }
         k_dual_epi<V><<<grid_of(b.nA_part), BS, 0, s>>>(b);
     } else {
-        k_dual<V><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
+        k_dual<V, NTL><<<grid_of(b.rbA.nblk), BS, 0, s>>>(b);
     }
+}
+template <typename V>
+static void dual_impl(const IterBufs<V>& b, hipStream_t s) {
+    if (b.nt) dual_launch<V, true>(b, s);
+    else dual_launch<V, false>(b, s);
 }
 template <typename V>
 static void attempt_impl(const IterBufs<V>& b, hipStream_t s) {

@@ -30,10 +30,14 @@ struct HostSlices {
     std::vector<int> ptr;     // S * (rows + 1) offsets into idx
     std::vector<int> idx;     // column indices, slice-major
     std::vector<int> src;     // position of each entry in the original CSR
+    std::vector<unsigned short> slot;  // gather order only: see Sliced::slot
     std::vector<int> quad;    // per block: first row, end row, first nz, end nz
     std::vector<int> bslice;  // per block: its slice
 };
-HostSlices build_slices(int rows, int cols, const int* ptr, const int* idx, int S);
+// With `gather_order`, the entries of every block of two or more rows are
+// stored sorted by column (stable) and `slot` holds their row-major places.
+HostSlices build_slices(int rows, int cols, const int* ptr, const int* idx, int S,
+                        bool gather_order = false);
 
 constexpr int MAX_SLICES = 16;
 template <typename V>
@@ -43,6 +47,10 @@ struct Sliced {
     const int* ptr = nullptr;
     const int* idx = nullptr;
     const V* val = nullptr;
+    // gather order (optional): the nonzeros of every multi-row block are
+    // stored sorted by column, slot[q] = the entry's place in row-major
+    // order (the row sums do not change; see rowblock_sums)
+    const unsigned short* slot = nullptr;
     const int4* quad = nullptr;
     const int* bslice = nullptr;
     V* part = nullptr;        // S * rows partial row sums
@@ -97,6 +105,9 @@ struct IterBufs {
     // the fused row-block kernels run instead
     Sliced<V> sT, sA;
     pdhg::LsState* st = nullptr;
+    // the attempt kernels load the matrix streams with nontemporal hints
+    // (for matrices too large to stay in the last-level cache)
+    bool nt = false;
 };
 
 // One line-search attempt: fused A'y + primal step, fused A xbar + dual

@@ -14,7 +14,7 @@
 extern "C" {
 #endif
 
-#define IGAOS_PDHG_ABI_VERSION 3
+#define IGAOS_PDHG_ABI_VERSION 4
 
 /* Status codes: the order of igaos::Status (include/igaos/status.h:5-14). */
 enum {
@@ -143,6 +143,11 @@ typedef struct {
                           six kernels per attempt, and faster); 1: also in
                           the default mode (slower there than its three
                           kernels per attempt); 0: never. */
+    int stream_loads;  /* how the attempt kernels load the matrix streams
+                          (indices, values): -1 (default) nontemporal when
+                          both matrices together exceed the 256 MB
+                          last-level cache, else cached; 0: cached; 1:
+                          nontemporal.  Results are identical either way. */
 } igaos_engine_config;
 void igaos_engine_config_default(igaos_engine_config* cfg);
 
@@ -444,6 +449,16 @@ int igaos_rowblocks_build(int nrows, const int* ptr, int* out, int cap);
  * (igaos_last_error) for a malformed pattern. */
 int igaos_slices_build(int nrows, int ncols, const int* ptr, const int* idx, int slices,
                        int* sptr, int* sidx, int* src, int* blocks, int* bslice, int cap);
+/* Host: the gather order the engine gives the slices of A when its columns
+ * are renumbered by length (no reference counterpart): the layout of
+ * igaos_slices_build with the entries of every block of two or more rows
+ * (at most 2048 nonzeros) sorted by column (stable), and slot[q] = the
+ * place of entry q in its block's row-major order (0 in a block of one
+ * row longer than 2048, which keeps its order).  Writes sidx, src and
+ * slot (nnz each); returns nnz, or -1 (igaos_last_error) for a malformed
+ * pattern. */
+int igaos_slices_gather_order(int nrows, int ncols, const int* ptr, const int* idx,
+                              int slices, int* sidx, int* src, unsigned short* slot);
 /* y = A x with A in CSR (device arrays); replaces the cusparseSpMV calls
  * of src/pdhg/pdhg.cu:150-156 for A and A' (CSR of A' = CSC of A). */
 int igaos_spmv_csr_f64(int nblk, const int* rowblocks, const int* ptr,

@@ -47,7 +47,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_and_status_names():
     L = _core.lib()
-    assert L.igaos_pdhg_abi_version() == 3
+    assert L.igaos_pdhg_abi_version() == 4
     names = [L.igaos_status_name(i).decode() for i in range(8)]
     assert names == ["optimal", "near-optimal", "feasible", "infeasible",
                      "unbounded", "iteration-limit", "time-limit", "error"]
@@ -245,6 +245,41 @@ def _slices(ptr, idx, ncols, S):
                                 ip(src), ip(blocks), ip(bslice), nb) == nb
     return (sptr.reshape(S, nrows + 1), sidx[:nnz], src[:nnz], blocks[:4 * nb].reshape(-1, 4),
             bslice[:nb])
+
+
+@pytest.mark.parametrize("S", [1, 3, 8])
+def test_gather_ordered_slices_restore_row_order(S):
+    # the gather order of A's slices (renumbered heavy columns): within
+    # every block of two or more rows the entries ascend by column, and
+    # slot sends each entry back to its place in the plain layout, so the
+    # kernels' row sums do not change; other blocks keep their order
+    import ctypes as C
+    L = _core.lib()
+    lp = lpgen.gen_zipf_badly_scaled(seed=2, m=2000, n=9000, kmax=600)
+    ptr = np.ascontiguousarray(lp.ap, np.int32)
+    idx = np.ascontiguousarray(lp.ai, np.int32)
+    nrows, nnz = lp.m, lp.nnz
+    sptr, sidx, src, blocks, bslice = _slices(ptr, idx, lp.n, S)
+    gidx, gsrc = np.zeros(nnz, np.int32), np.zeros(nnz, np.int32)
+    slot = np.zeros(nnz, np.uint16)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa
+    assert L.igaos_slices_gather_order(nrows, lp.n, ip(ptr), ip(idx), S, ip(gidx), ip(gsrc),
+                                       slot.ctypes.data_as(C.c_void_p)) == nnz
+    sorted_blocks = 0
+    for r0, r1, z0, z1 in blocks:
+        g, s, q = gidx[z0:z1], gsrc[z0:z1], slot[z0:z1].astype(np.int64)
+        if r1 - r0 > 1:
+            assert np.all(np.diff(g) >= 0)
+            assert np.array_equal(np.sort(q), np.arange(z1 - z0))
+            back_idx, back_src = np.empty_like(g), np.empty_like(s)
+            back_idx[q], back_src[q] = g, s
+            assert np.array_equal(back_idx, sidx[z0:z1]) and np.array_equal(back_src, src[z0:z1])
+            sorted_blocks += not np.array_equal(g, sidx[z0:z1])
+        else:
+            assert np.array_equal(g, sidx[z0:z1]) and np.array_equal(s, src[z0:z1])
+    assert sorted_blocks > 0
+    bad = np.array([0, 5, 1], np.int32)
+    assert L.igaos_slices_gather_order(2, 9, ip(bad), ip(idx), S, None, None, None) < 0
 
 
 @pytest.mark.parametrize("S", [1, 3, 8])

@@ -287,6 +287,32 @@ def test_column_locality_order_keeps_parity():
     # column-specific upper bounds (U[1, 100]) hold
     assert s32.iterations == K
     assert np.all(s32.x >= lp.cl - 1e-6) and np.all(s32.x <= lp.cu * (1 + 1e-6) + 1e-6)
+    # column slices of the renumbered copy: A's multi-row blocks are stored
+    # sorted by column (gather order) and each product lands at its
+    # row-major slot, so rows sum as in plain slices; with nontemporal
+    # matrix loads
+    for slices in (1, 8):
+        g = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K,
+                               column_slices=slices, stream_loads=1)
+        assert g.iterations == K and g.info["attempts"] == r.attempts
+        assert _rel(g.x, r.x) <= 1e-9 and _rel(g.y, r.y) <= 1e-9
+        if slices == 1:  # the fused kernels' blocks and row sums
+            dx, dy = _rel(g.x, s.x), _rel(g.y, s.y)
+            assert dx <= 1e-12 and dy <= 1e-12, (dx, dy)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_nontemporal_stream_loads_change_nothing(precision):
+    # the hint changes how the matrix streams are cached, not one value
+    lp = lpgen.gen_zipf_badly_scaled(seed=4, m=6_000, n=20_000, kmax=600)
+    for slices in (0, 3):
+        kw = dict(time_limit=1e7, tolerance=0.0, max_iterations=192, precision=precision,
+                  column_slices=slices)
+        a = igaos_amd.solve_lp(lp, stream_loads=0, **kw)
+        b = igaos_amd.solve_lp(lp, stream_loads=1, **kw)
+        assert a.iterations == b.iterations == 192
+        assert a.info["attempts"] == b.info["attempts"]
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
 
 
 @pytest.mark.parametrize("slices", [1, 3, 8])
