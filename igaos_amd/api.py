@@ -415,10 +415,10 @@ def solve_lp(lp, time_limit=120.0, tolerance=1e-4, max_iterations=500000,
     (FP64, iteration-bounded runs) fixes every summation order to the
     reference's sequential one and disables fused multiply-adds, so the run
     is bit-reproducible against a sequential FP64 host replay.
-    ``small_runner`` (-1: exact mode only, 0: never, 1: also the default
-    mode): FP64 LPs with max(n, m) <= 16384 and nnz <= 15360 run a check
-    period of attempts per launch in one workgroup, with the exact mode's
-    arithmetic.  ``stream_loads`` (-1: nontemporal when both matrices
+    ``small_runner`` (-1 or 1: wherever the LP fits, 0: never): small LPs
+    run a check period of attempts per launch in one workgroup (exact
+    mode: the exact runner; default mode: the register / LDS runner for
+    max(n, m) <= 3072 whose products, x and y fit in LDS).  ``stream_loads`` (-1: nontemporal when both matrices
     exceed the last-level cache, 0: cached, 1: nontemporal) picks how the
     attempt kernels load the matrix streams; results do not depend on it.
     """

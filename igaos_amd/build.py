@@ -21,7 +21,8 @@ CLI = os.path.join(HERE, "igaos")
 
 LIB_SOURCES = ["mps.cpp", "repair.cpp", "crossover.cpp", "engine.cpp", "capi.cpp",
                "pdhg_kernels.hip", "exact.hip", "batch.hip", "batch_host.cpp",
-               "adjoint.hip", "unroll.hip", "noalloc.hip", "transpose.hip", "repair_dev.hip"]
+               "adjoint.hip", "unroll.hip", "noalloc.hip", "transpose.hip", "repair_dev.hip",
+               "small.hip"]
 # per-source extra flags: the exact mode's kernels must not fuse a*b + c
 EXTRA = {"exact.hip": ["-ffp-contract=off"]}
 CLI_SOURCES = ["cli.cpp"]

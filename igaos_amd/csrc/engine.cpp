@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "exact.hpp"
+#include "small.hpp"
 #include "repair_dev.hpp"
 #include "pdhg.hpp"
 #include "pdhg_kernels.hpp"
@@ -226,9 +227,12 @@ class Engine final : public EngineBase {
     long repair_cg_ = 0, repair_its_ = 0;
     long long pow_filled_ = 0;
     dev::ExactBufs eb_;
-    // small FP64 LPs: one workgroup runs a check period's attempts per
-    // launch (exact.hip, k_x_run), with the exact mode's arithmetic
+    // small FP64 LPs in exact mode: one workgroup runs a check period's
+    // attempts per launch (exact.hip, k_x_run), with the exact arithmetic
     bool runner_ = false;
+    // small LPs in the default mode: the register / LDS runner (small.hip)
+    bool small_ = false;
+    long nnz_ = 0;
     static constexpr long long EXACT_POW_CAP = 1LL << 21;
     // attempts per runner launch: a check period of 64 accepted
     // iterations takes at most 64 x 21 attempts (pdhg.cu:785: the 21st
@@ -357,6 +361,7 @@ void Engine<V>::setup() {
     HIPCK(hipEventCreate(&ev0_));
     HIPCK(hipEventCreate(&ev1_));
     const long nnz = lp_.nnz();
+    nnz_ = nnz;
     ap_.upload(lp_.ap.data(), m_ + 1, s_);
     ai_.upload(lp_.ai.data(), nnz, s_);
     ax_.upload(lp_.ax.data(), nnz, s_);
@@ -488,21 +493,17 @@ void Engine<V>::setup() {
     kd_.partC = partC_.p; kd_.partR = partR_.p; kd_.partR2 = partR2_.p;
     kd_.rowres = rowres_.p;
 
-    if constexpr (!F32) {
+    {
         const int sr = cfg_.small_runner;
-        const bool fits = dev::run_exact_fits(n_, m_, nnz) && !ib_.sT.S && !ib_.sA.S;
-        // by default in exact mode only: on the 1,000 x 3,000 LP the runner
-        // takes about 25 us per attempt on its one CU, the three-kernel
-        // attempts of the default mode about 15 us
-        runner_ = fits && (sr == 1 || (sr == -1 && cfg_.exact));
-    }
-    if (runner_ && !cfg_.exact) {  // the runner's per-attempt vectors
-        xdx_.alloc(n_);
-        xdy_.alloc(m_);
-        xtm_.alloc(m_);
-        eb_.dx = xdx_.p;
-        eb_.dy = xdy_.p;
-        eb_.tm = xtm_.p;
+        const bool unsliced = !ib_.sT.S && !ib_.sA.S;
+        if (cfg_.exact) {
+            // exact mode: the exact runner (the reference's summation orders)
+            if constexpr (!F32)
+                runner_ = sr != 0 && dev::run_exact_fits(n_, m_, nnz) && unsliced;
+        } else {
+            // default mode: the register / LDS runner (small.hip)
+            small_ = sr != 0 && dev::small_fits(n_, m_, nnz, F32) && unsliced;
+        }
     }
     if (cfg_.exact) {
         xdx_.alloc(n_);
@@ -523,7 +524,7 @@ void Engine<V>::setup() {
         eb_.pow_cap = EXACT_POW_CAP;
     }
 
-    const int G = runner_ ? 0 : cfg_.graph_slots;
+    const int G = (runner_ || small_) ? 0 : cfg_.graph_slots;
     if (G > 0) {  // G attempt slots, replayed until the state freezes
         HIPCK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         for (int g = 0; g < G; ++g) attempt(s_);
@@ -761,6 +762,10 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
                 dev::launch_run_exact(ib_, eb_, RUNNER_CAP, s_);
                 return;
             }
+        }
+        if (small_) {  // attempts until the freeze, in one launch
+            HIPCK(dev::launch_small_run(ib_, nnz_, RUNNER_CAP, s_));
+            return;
         }
         if (G > 0) {
             const long reps = (slots + G - 1) / G;

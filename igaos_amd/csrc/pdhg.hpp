@@ -23,9 +23,10 @@ struct EngineConfig {
     int scaling = 0;         // 0: Ruiz (the reference); 1: Ruiz then Pock-Chambolle (alpha 1)
     int exact = 0;           // 1: exact FP64 mode (reference summation orders, no FMA;
                              // precision 64, iteration-bounded runs only)
-    int small_runner = -1;   // FP64 LPs with max(n, m) <= 16384 and nnz <= 15360: one
-                             // workgroup runs a check period's attempts per launch
-                             // (-1: in exact mode, 0: never, 1: also in the default mode)
+    int small_runner = -1;   // small LPs: one workgroup runs a check period's attempts per
+                             // launch (-1 or 1: wherever the LP fits, 0: never); exact mode:
+                             // exact.hip's runner, default mode: small.hip's
+
     int stream_loads = -1;   // matrix streams of the attempt kernels: -1 nontemporal when
                              // both matrices exceed the last-level cache, 0 cached, 1 nontemporal
 };

@@ -133,16 +133,20 @@ typedef struct {
                           iteration-bounded runs only (time_limit_s >= 1e6
                           or <= 0, max_iterations <= 2^21); far slower than
                           the default kernels.  0: default. */
-    int small_runner;  /* FP64 LPs with max(n, m) <= 16384 and nnz <=
-                          15360: one workgroup of 1024 threads runs the
-                          attempts of a whole check period in one launch,
-                          with the exact mode's arithmetic (each sparse
-                          product staged in LDS and summed per row in
-                          storage order, the reference's dot layout).
-                          -1 (default): in exact mode (bit-identical to its
-                          six kernels per attempt, and faster); 1: also in
-                          the default mode (slower there than its three
-                          kernels per attempt); 0: never. */
+    int small_runner;  /* small LPs: one workgroup of 1024 threads runs the
+                          attempts of a whole check period in one launch.
+                          Exact mode (FP64 LPs with max(n, m) <= 16384 and
+                          nnz <= 15360): the exact runner, bit-identical to
+                          the exact mode's six kernels per attempt.  Default
+                          mode (max(n, m) <= 3072, the products, x and y
+                          within the LDS of a workgroup: nnz + n + m <=
+                          about 20,000 in FP64, 40,000 in FP32): the
+                          register / LDS runner, each column's and row's
+                          state in registers, the matrices streamed from
+                          L2, rows summed in storage order (1.1x the three
+                          kernels per attempt on the 1,000 x 3,000 LP in
+                          FP64, 1.4x in FP32).  -1 (default) and 1: wherever
+                          the LP fits; 0: never. */
     int stream_loads;  /* how the attempt kernels load the matrix streams
                           (indices, values): -1 (default) nontemporal when
                           both matrices together exceed the 256 MB

@@ -123,8 +123,8 @@ def _rel(a, b):
 
 
 # runner 0: three launches per attempt (row-block products); runner 1: the
-# one-workgroup runner (rows summed in storage order, the reference's dot
-# layout); the LP fits both
+# default mode's one-workgroup runner (the LP's state in registers and LDS,
+# rows summed in storage order); the LP fits both
 @pytest.mark.parametrize("runner", [0, 1])
 @pytest.mark.parametrize("K", [16, 64])
 def test_fixed_iteration_trajectory_parity(K, runner):
