@@ -13,6 +13,13 @@ Generated LPs are named by generator, parameters and seed (see
 igaos_amd/lpgen.py, GENERATOR_VERSION).
 
 Run:  python tests/golden/make_fixtures.py   (about 3 minutes)
+      python tests/golden/make_fixtures.py --add   (only the LARGE entries,
+      added to the existing file: about 3 minutes)
+
+LARGE entries use HiGHS's interior-point method with crossover
+(method="highs-ipm"): its default simplex needs over 15 minutes on the
+8-period LP.  On the same family at 2 periods of 1,000 x 1,600 both
+methods return the same objective to the last bit (-399.9303203719868).
 """
 from __future__ import annotations
 
@@ -33,7 +40,7 @@ from igaos_amd import lpgen  # noqa: E402
 import oracle  # noqa: E402
 
 
-def scipy_highs(lp):
+def scipy_highs(lp, method="highs"):
     import scipy.sparse as sp
     from scipy.optimize import linprog
     A = sp.csr_matrix((lp.ax, lp.ai, lp.ap), shape=(lp.m, lp.n))
@@ -49,11 +56,38 @@ def scipy_highs(lp):
         kw.update(A_ub=A_ub, b_ub=b_ub)
     if eq.any():
         kw.update(A_eq=A[eq], b_eq=lp.rmin[eq])
-    r = linprog(lp.c, bounds=bounds, method="highs", **kw)
+    r = linprog(lp.c, bounds=bounds, method=method, **kw)
     return (float(r.fun) + lp.obj_const) if r.status == 0 else None
 
 
+# (generator, parameters): solved with HiGHS's interior-point method
+LARGE = [("gen_multiperiod", dict(seed=0, periods=8, rows_per=1000, cols_per=1600, inv=200))]
+
+
+def key_of(fn, kw):
+    return fn + "(" + ",".join(f"{k}={v}" for k, v in kw.items()) + ")"
+
+
+def add_large(E):
+    for fn, kw in LARGE:
+        lp = getattr(lpgen, fn)(**kw)
+        t = time.time()
+        E[key_of(fn, kw)] = {"m": lp.m, "n": lp.n, "nnz": lp.nnz,
+                             "scipy_highs": scipy_highs(lp, method="highs-ipm"),
+                             "scipy_highs_method": "highs-ipm",
+                             "scipy_highs_seconds": time.time() - t}
+        print(key_of(fn, kw), E[key_of(fn, kw)], flush=True)
+
+
 def main():
+    if "--add" in sys.argv:
+        path = os.path.join(HERE, "objectives.json")
+        with open(path) as f:
+            out = json.load(f)
+        add_large(out["entries"])
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+        return
     out = {"generator_version": lpgen.GENERATOR_VERSION, "entries": {}}
     E = out["entries"]
     with open(os.path.join(HERE, "netlib", "highs_results.csv")) as f:
@@ -89,6 +123,7 @@ def main():
                        ref_simplex_seconds=time.time() - t)
         E[key] = ent
         print(key, ent, flush=True)
+    add_large(E)
     with open(os.path.join(HERE, "objectives.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
 

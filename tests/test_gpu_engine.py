@@ -250,6 +250,24 @@ def test_badly_scaled_and_multiperiod(precision, objectives):
         _check_solution(s, objectives[key]["scipy_highs"])
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_eight_period_blending_lp_against_highs(precision, objectives):
+    # the block-structured multi-period family at 8 periods of 1,000 x
+    # 1,600 (200 inventory columns each, linking consecutive periods):
+    # 8,000 rows, 12,800 columns, 79,800 nonzeros; HiGHS's objective from
+    # its interior-point method with crossover (tests/golden/make_fixtures.py)
+    key = "gen_multiperiod(seed=0,periods=8,rows_per=1000,cols_per=1600,inv=200)"
+    lp = lpgen.gen_multiperiod(seed=0, periods=8, rows_per=1000, cols_per=1600, inv=200)
+    ent = objectives[key]
+    assert (lp.m, lp.n, lp.nnz) == (ent["m"], ent["n"], ent["nnz"])
+    s = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=120, precision=precision)
+    assert s.status == "near-optimal", (s.status, s.message)
+    assert s.pinf <= 1e-6 and s.dinf <= 1e-6 and s.rel_gap <= 1e-6
+    # the KKT bar bounds the gap, not the distance to the optimum: 2e-6
+    ref = ent["scipy_highs"]
+    assert abs(s.objective - ref) <= 2e-6 * (1 + abs(ref)), (s.objective, ref)
+
+
 def test_graph_and_direct_launches_agree():
     lp = lpgen.gen_random_eq(seed=1, m=200, n=600)
     a = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=60, graph_slots=16)
