@@ -160,6 +160,28 @@ def test_trajectory_stays_close_through_two_checks(runner):
     assert _rel(s.x, r.x) <= 1e-4 and _rel(s.y, r.y) <= 1e-4
 
 
+@pytest.mark.parametrize("m,n", [(300, 900), (600, 1800), (1000, 3000)])
+def test_small_runner_register_layouts(m, n):
+    # the runner keeps 1, 2 or 3 columns (and rows) per thread: 64
+    # iterations of each layout follow the oracle like the three-kernel
+    # path; in FP32 the two paths sum rows in different orders, and the
+    # line search amplifies that rounding (about 10x every 8 iterations),
+    # so they are compared over 16 iterations
+    lp = lpgen.gen_random_eq(seed=7, m=m, n=n)
+    K = 64
+    r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=K)
+    for runner in (0, 1):
+        s = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=K,
+                               small_runner=runner)
+        assert s.iterations == r.iterations == K
+        assert s.info["attempts"] == r.attempts and s.info["restarts"] == r.restarts
+        assert _rel(s.x, r.x) <= 1e-9 and _rel(s.y, r.y) <= 1e-9
+    a, b = (igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=16,
+                               precision=32, small_runner=runner) for runner in (0, 1))
+    assert a.iterations == b.iterations == 16
+    assert _rel(b.x, a.x) <= 1e-3 and _rel(b.y, a.y) <= 1e-3
+
+
 def _check_solution(s, ref_obj, tol=1e-6):
     assert s.status == "near-optimal", (s.status, s.message)
     assert s.pinf <= tol and s.rel_gap <= tol
