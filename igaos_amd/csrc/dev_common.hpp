@@ -257,7 +257,9 @@ __device__ __forceinline__ void rowblock_sums(const int* __restrict__ ptr,
 // 6 TB/s against about 4 TB/s for loads that allocate in the caches, and
 // they leave L2 to the gathered vector; they only pay when the matrices
 // are too large to stay in the 256 MB last-level cache between
-// iterations.  Same sums as rowblock_sums.
+// iterations.  Same sums as rowblock_sums.  A copy rather than a flag on
+// rowblock_sums: the batch's persistent kernel (register-capped, with
+// spills) inlines rowblock_sums, and a flag changed its code generation.
 template <typename Acc, typename Val, typename G, typename I>
 __device__ __forceinline__ void rowblock_sums_nt(const int* __restrict__ ptr,
                                                  const I* __restrict__ idx,
