@@ -74,8 +74,8 @@ __device__ __forceinline__ T block_max(T v, T* red) {
 // results are run-to-run deterministic.
 constexpr int FIN = 1024;
 
-__device__ __forceinline__ double fin_sum(const double* p, long cnt,
-                                          long stride, double* red) {
+// This thread's part of fin_sum / fin_max.
+__device__ __forceinline__ double fin_part_sum(const double* p, long cnt, long stride) {
     double s0 = 0.0, s1 = 0.0;
     long i = threadIdx.x;
     for (; i + FIN < cnt; i += 2 * FIN) {
@@ -83,13 +83,45 @@ __device__ __forceinline__ double fin_sum(const double* p, long cnt,
         s1 += p[(i + FIN) * stride];
     }
     if (i < cnt) s0 += p[i * stride];
-    return block_sum<double, FIN>(s0 + s1, red);
+    return s0 + s1;
+}
+__device__ __forceinline__ double fin_part_max(const double* p, long cnt, long stride) {
+    double s = 0.0;
+    for (long i = threadIdx.x; i < cnt; i += FIN) s = fmax(s, p[i * stride]);
+    return s;
+}
+__device__ __forceinline__ double fin_sum(const double* p, long cnt,
+                                          long stride, double* red) {
+    return block_sum<double, FIN>(fin_part_sum(p, cnt, stride), red);
 }
 __device__ __forceinline__ double fin_max(const double* p, long cnt,
                                           long stride, double* red) {
-    double s = 0.0;
-    for (long i = threadIdx.x; i < cnt; i += FIN) s = fmax(s, p[i * stride]);
-    return block_max<double, FIN>(s, red);
+    return block_max<double, FIN>(fin_part_max(p, cnt, stride), red);
+}
+
+// K workgroup reductions at once: v[k] becomes, in every thread, the sum
+// of v[k] over the workgroup (the maximum where bit k of max_mask is set),
+// combined in block_sum's (block_max's) order, so to the same value; one
+// pair of barriers for all K.  `red` must hold K * NT / WAVE entries.
+template <int K, int NT>
+__device__ __forceinline__ void block_reduce_k(double* v, unsigned max_mask, double* red) {
+    constexpr int W = NT / WAVE;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = (max_mask >> k & 1u) ? wave_max(v[k]) : wave_sum(v[k]);
+    const int w = threadIdx.x / WAVE;
+    __syncthreads();
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) red[k * W + w] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const bool mx = max_mask >> k & 1u;
+        double s = red[k * W];
+        for (int i = 1; i < W; ++i) s = mx ? fmax(s, red[k * W + i]) : s + red[k * W + i];
+        v[k] = s;
+    }
 }
 
 // Gather functors: x[j] for plain vectors, xref[j] + (double)xd[j] for an

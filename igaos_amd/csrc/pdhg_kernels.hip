@@ -203,14 +203,14 @@ __global__ __launch_bounds__(BS) void k_dual(IterBufs<V> b) {
 // the accept/reject rule on the device-resident state.
 template <typename V>
 __global__ __launch_bounds__(FIN) void k_decide(IterBufs<V> b) {
-    __shared__ double red[FIN / WAVE];
+    __shared__ double red[3 * FIN / WAVE];
     if (b.st->frozen) return;
-    const double d0 = fin_sum(b.partT, b.nT_part, 1, red);
-    const double d1 = fin_sum(b.partA, b.nA_part, 2, red);
-    const double d2 = fin_sum(b.partA + 1, b.nA_part, 2, red);
+    double v[3] = {fin_part_sum(b.partT, b.nT_part, 1), fin_part_sum(b.partA, b.nA_part, 2),
+                   fin_part_sum(b.partA + 1, b.nA_part, 2)};
+    block_reduce_k<3, FIN>(v, 0u, red);
     if (threadIdx.x == 0) {
         LsState s = *b.st;
-        pdhg::ls_decide(s, d0, d1, d2);
+        pdhg::ls_decide(s, v[0], v[1], v[2]);
         *b.st = s;
     }
 }
@@ -574,7 +574,7 @@ template <typename V>
 __global__ __launch_bounds__(BS) void k_kkt_cols(KktData d, KktIter<V> it) {
     __shared__ double s_prod[NNZ_MAX];
     __shared__ double s_rows[ROWS_MAX];
-    __shared__ double red[BS / WAVE];
+    __shared__ double red[4 * BS / WAVE];
     const int q = blockIdx.y;
     int r0, r1, p0, p1;
     block_rows(d.rbT, blockIdx.x, r0, r1, p0, p1);
@@ -606,16 +606,11 @@ __global__ __launch_bounds__(BS) void k_kkt_cols(KktData d, KktIter<V> it) {
             else if (-hd > dz) unb = 1.0;
         }
     }
-    dv = block_max(dv, red);
-    op = block_sum(op, red);
-    od = block_sum(od, red);
-    unb = block_max(unb, red);
+    double v[4] = {dv, op, od, unb};
+    block_reduce_k<4, BS>(v, 0x9u, red);  // maxima: dv, unb
     if (threadIdx.x == 0) {
         double* p = d.partC + (static_cast<long>(q) * gridDim.x + blockIdx.x) * 4;
-        p[0] = dv;
-        p[1] = op;
-        p[2] = od;
-        p[3] = unb;
+        for (int k = 0; k < 4; ++k) p[k] = v[k];
     }
 }
 
@@ -623,7 +618,7 @@ template <typename V>
 __global__ __launch_bounds__(BS) void k_kkt_rows(KktData d, KktIter<V> it) {
     __shared__ double s_prod[NNZ_MAX];
     __shared__ double s_rows[ROWS_MAX];
-    __shared__ double red[BS / WAVE];
+    __shared__ double red[4 * BS / WAVE];
     const int q = blockIdx.y;
     int r0, r1, p0, p1;
     block_rows(d.rbA, blockIdx.x, r0, r1, p0, p1);
@@ -648,32 +643,29 @@ __global__ __launch_bounds__(BS) void k_kkt_rows(KktData d, KktIter<V> it) {
         const double hy = gy(i) * (1.0 / d.dr[i]);
         ymax = fmax(ymax, fabs(hy));
     }
-    pv = block_max(pv, red);
-    pa = block_max(pa, red);
-    prow = block_max(prow, red);
-    ymax = block_max(ymax, red);
+    double v[4] = {pv, pa, prow, ymax};
+    block_reduce_k<4, BS>(v, 0xFu, red);
     if (threadIdx.x == 0) {
         double* p = d.partR + (static_cast<long>(q) * gridDim.x + blockIdx.x) * 4;
-        p[0] = pv;
-        p[1] = pa;
-        p[2] = prow;
-        p[3] = ymax;
+        for (int k = 0; k < 4; ++k) p[k] = v[k];
     }
 }
 
 // Row-pass maxima of both iterates -> rowres[q*4 + k].
 __global__ __launch_bounds__(FIN) void k_kkt_rowfin(KktData d, int nb) {
-    __shared__ double red[FIN / WAVE];
+    __shared__ double red[8 * FIN / WAVE];
+    double v[8];
     for (int q = 0; q < 2; ++q)
-        for (int k = 0; k < 4; ++k) {
-            const double r = fin_max(d.partR + static_cast<long>(q) * nb * 4 + k, nb, 4, red);
-            if (threadIdx.x == 0) d.rowres[q * 4 + k] = r;
-        }
+        for (int k = 0; k < 4; ++k)
+            v[q * 4 + k] = fin_part_max(d.partR + static_cast<long>(q) * nb * 4 + k, nb, 4);
+    block_reduce_k<8, FIN>(v, 0xFFu, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; ++k) d.rowres[k] = v[k];
 }
 
 template <typename V>
 __global__ __launch_bounds__(BS) void k_kkt_bound(KktData d, KktIter<V> it) {
-    __shared__ double red[BS / WAVE];
+    __shared__ double red[2 * BS / WAVE];
     const int q = blockIdx.y;
     const double yz = 1e-8 * (1.0 + d.rowres[q * 4 + 3]);
     const GatherRef<V> gy{it.yref[q], it.yd[q]};
@@ -689,28 +681,34 @@ __global__ __launch_bounds__(BS) void k_kkt_bound(KktData d, KktIter<V> it) {
             else unb = 1.0;
         }
     }
-    od = block_sum(od, red);
-    unb = block_max(unb, red);
+    double v[2] = {od, unb};
+    block_reduce_k<2, BS>(v, 0x2u, red);
     if (threadIdx.x == 0) {
         double* p = d.partR2 + (static_cast<long>(q) * gridDim.x + blockIdx.x) * 2;
-        p[0] = od;
-        p[1] = unb;
+        p[0] = v[0];
+        p[1] = v[1];
     }
 }
 
 // Final metrics of both iterates (pdhg.cu:195-253).
 __global__ __launch_bounds__(FIN) void k_kkt_done(KktData d, int nbC, int nbB,
                                                   Metrics* out) {
-    __shared__ double red[FIN / WAVE];
+    __shared__ double red[12 * FIN / WAVE];
+    double v[12];
     for (int q = 0; q < 2; ++q) {
         const double* pc = d.partC + static_cast<long>(q) * nbC * 4;
         const double* pb = d.partR2 + static_cast<long>(q) * nbB * 2;
-        const double dv = fin_max(pc + 0, nbC, 4, red);
-        const double op = fin_sum(pc + 1, nbC, 4, red);
-        const double odc = fin_sum(pc + 2, nbC, 4, red);
-        const double unbc = fin_max(pc + 3, nbC, 4, red);
-        const double odr = fin_sum(pb + 0, nbB, 2, red);
-        const double unbr = fin_max(pb + 1, nbB, 2, red);
+        v[6 * q + 0] = fin_part_max(pc + 0, nbC, 4);
+        v[6 * q + 1] = fin_part_sum(pc + 1, nbC, 4);
+        v[6 * q + 2] = fin_part_sum(pc + 2, nbC, 4);
+        v[6 * q + 3] = fin_part_max(pc + 3, nbC, 4);
+        v[6 * q + 4] = fin_part_sum(pb + 0, nbB, 2);
+        v[6 * q + 5] = fin_part_max(pb + 1, nbB, 2);
+    }
+    block_reduce_k<12, FIN>(v, 0x29u | (0x29u << 6), red);  // maxima: dv, unbc, unbr
+    for (int q = 0; q < 2; ++q) {
+        const double dv = v[6 * q + 0], op = v[6 * q + 1], odc = v[6 * q + 2];
+        const double unbc = v[6 * q + 3], odr = v[6 * q + 4], unbr = v[6 * q + 5];
         if (threadIdx.x == 0) {
             const double pv = d.rowres[q * 4 + 0];
             const double pa = d.rowres[q * 4 + 1];
