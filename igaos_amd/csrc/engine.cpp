@@ -231,8 +231,12 @@ class Engine final : public EngineBase {
     // attempts per launch (exact.hip, k_x_run), with the exact arithmetic
     bool runner_ = false;
     // small LPs in the default mode: the register / LDS runner (small.hip)
+    // with sliced-ELL copies of A' and A
     bool small_ = false;
-    long nnz_ = 0;
+    DBuf<int> epermT_, epermA_, eoffT_, eoffA_;
+    DBuf<unsigned short> eidxT_, eidxA_;
+    DBuf<V> evalT_, evalA_;
+    dev::SmallEll<V> eT_, eA_;
     static constexpr long long EXACT_POW_CAP = 1LL << 21;
     // attempts per runner launch: a check period of 64 accepted
     // iterations takes at most 64 x 21 attempts (pdhg.cu:785: the 21st
@@ -246,6 +250,9 @@ class Engine final : public EngineBase {
     void make_slices(dev::Sliced<V>& sl, SliceBufs& B, int rows, int cols,
                      const std::vector<int>& ptr, const std::vector<int>& idx,
                      const double* sval, int S, bool gather_order);
+    void make_ell(dev::SmallEll<V>& E, DBuf<int>& perm, DBuf<int>& off, DBuf<unsigned short>& idx,
+                  DBuf<V>& val, int rows, const std::vector<int>& ptr, const int* didx,
+                  const V* dval);
     void reset();
     double power_iteration();
     double dsum(const double* a, const double* b, long n);
@@ -361,7 +368,6 @@ void Engine<V>::setup() {
     HIPCK(hipEventCreate(&ev0_));
     HIPCK(hipEventCreate(&ev1_));
     const long nnz = lp_.nnz();
-    nnz_ = nnz;
     ap_.upload(lp_.ap.data(), m_ + 1, s_);
     ai_.upload(lp_.ai.data(), nnz, s_);
     ax_.upload(lp_.ax.data(), nnz, s_);
@@ -503,6 +509,10 @@ void Engine<V>::setup() {
         } else {
             // default mode: the register / LDS runner (small.hip)
             small_ = sr != 0 && dev::small_fits(n_, m_, nnz, F32) && unsliced;
+            if (small_) {
+                make_ell(eT_, epermT_, eoffT_, eidxT_, evalT_, n_, lp_.cp, ib_.ci, ib_.acx);
+                make_ell(eA_, epermA_, eoffA_, eidxA_, evalA_, m_, lp_.ap, ib_.ai, ib_.ax);
+            }
         }
     }
     if (cfg_.exact) {
@@ -555,6 +565,29 @@ int Engine<V>::slices_for(int cols, int rows, long nnz) const {
     if (bytes < 3.0e6 || nnz < 8000000 || rows <= 0) return 0;
     const double per = static_cast<double>(nnz) / rows >= 128.0 ? 0.5e6 : 1.6e6;
     return static_cast<int>(std::clamp(std::lround(bytes / per), 2L, 8L));
+}
+
+// The runner's sliced-ELL copy of a CSR (CSC) pattern: the map is built on
+// the host from the pointers, the indices and scaled values are gathered on
+// the device.
+template <typename V>
+void Engine<V>::make_ell(dev::SmallEll<V>& E, DBuf<int>& perm, DBuf<int>& off,
+                         DBuf<unsigned short>& idx, DBuf<V>& val, int rows,
+                         const std::vector<int>& ptr, const int* didx, const V* dval) {
+    std::vector<int> hperm, hoff, hsrc;
+    const long total = dev::small_ell_map(rows, ptr.data(), hperm, hoff, hsrc);
+    DBuf<int> src;
+    perm.upload(hperm.data(), hperm.size(), s_);
+    off.upload(hoff.data(), hoff.size(), s_);
+    src.upload(hsrc.data(), hsrc.size(), s_);
+    idx.alloc(total);
+    val.alloc(total);
+    HIPCK(dev::small_ell_fill(src.p, total, didx, dval, idx.p, val.p, s_));
+    sync();  // src and the host maps go out of scope
+    E.perm = perm.p;
+    E.off = off.p;
+    E.idx = idx.p;
+    E.val = val.p;
 }
 
 template <typename V>
@@ -764,7 +797,7 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
             }
         }
         if (small_) {  // attempts until the freeze, in one launch
-            HIPCK(dev::launch_small_run(ib_, nnz_, RUNNER_CAP, s_));
+            HIPCK(dev::launch_small_run(ib_, eT_, eA_, RUNNER_CAP, s_));
             return;
         }
         if (G > 0) {

@@ -1,25 +1,62 @@
 #pragma once
 // One-workgroup runner of the default mode for small LPs (small.hip): a
 // check period of line-search attempts per launch, the LP's vectors in
-// registers and LDS, its matrices streamed from L2.
+// registers and LDS, sliced-ELL copies of its matrices streamed from L2.
 
 #include <hip/hip_runtime.h>
+
+#include <vector>
 
 #include "pdhg_kernels.hpp"
 
 namespace igaos::dev {
 
 constexpr int SMALL_MAX_DIM = 3 * 1024;  // max(n, m): up to 3 columns / rows a thread
+// Past this many nonzeros one CU's share of L2 bandwidth streams the two
+// products slower than the whole GPU runs the three-kernel attempt.
+constexpr long SMALL_MAX_NNZ = 1L << 17;
+
+// Sliced-ELL copy of a scaled matrix for the runner (rows of A for A xbar,
+// columns for A'y).  Rows sit in slots in order of decreasing length
+// (perm[s] = the row in slot s), and slots go in slices of 64 (q = s / 64,
+// lane s % 64); a row's entries, in storage order, go in chunks of
+// SMALL_CHUNK, chunk c of slot s at chunk index (off[q] + c) * 64 + lane,
+// so that a wavefront loads one contiguous run per chunk; a slice is as
+// many chunk rows wide as its longest row needs, and padding entries are 0
+// with index 0.  Values: val[chunk * SMALL_CHUNK + g]; 16-bit indices
+// likewise.
+constexpr int SMALL_CHUNK = 4;
+template <typename V>
+struct SmallEll {
+    const int* perm = nullptr;  // rows: the row in each slot
+    const int* off = nullptr;   // ceil(rows / 64) + 1 chunk-row offsets
+    const unsigned short* idx = nullptr;
+    const V* val = nullptr;
+};
 
 // Whether an LP (n columns, m rows, nnz nonzeros) fits the runner on the
-// current device: its products, x and y within the device's LDS per
-// workgroup (160 KB on gfx950) and at most 3 columns and rows per thread.
+// current device: x, y and the per-launch constants (bounds, gradient)
+// within the device's LDS per workgroup, at most 3 columns and rows per
+// thread, at most SMALL_MAX_NNZ nonzeros.
 bool small_fits(int n, int m, long nnz, bool f32);
+// The slot order, chunk-row offsets of a CSR (CSC) pattern's ELL copy and,
+// per entry position, the CSR (CSC) position it holds (-1: padding);
+// returns the number of entry positions (at least one chunk row).
+long small_ell_map(int rows, const int* ptr, std::vector<int>& perm, std::vector<int>& off,
+                   std::vector<int>& src);
+// Fills an ELL copy's indices and values from the CSR (CSC) arrays.
+hipError_t small_ell_fill(const int* src, long total, const int* idx, const double* val,
+                          unsigned short* eidx, double* eval, hipStream_t s);
+hipError_t small_ell_fill(const int* src, long total, const int* idx, const float* val,
+                          unsigned short* eidx, float* eval, hipStream_t s);
 // Attempts until the state freezes at a check boundary (or MAXIT), at most
 // cap, in one launch; the buffers and the state are left as the
-// three-kernel attempts leave them.  Requires small_fits and unsliced
-// products.  Returns the launch's error (the engine raises it).
-hipError_t launch_small_run(const IterBufs<double>& b, long nnz, int cap, hipStream_t s);
-hipError_t launch_small_run(const IterBufs<float>& b, long nnz, int cap, hipStream_t s);
+// three-kernel attempts leave them.  Requires small_fits, unsliced
+// products and the ELL copies of A' (eT) and A (eA).  Returns the launch's
+// error (the engine raises it).
+hipError_t launch_small_run(const IterBufs<double>& b, const SmallEll<double>& eT,
+                            const SmallEll<double>& eA, int cap, hipStream_t s);
+hipError_t launch_small_run(const IterBufs<float>& b, const SmallEll<float>& eT,
+                            const SmallEll<float>& eA, int cap, hipStream_t s);
 
 }  // namespace igaos::dev

@@ -160,10 +160,11 @@ def test_trajectory_stays_close_through_two_checks(runner):
     assert _rel(s.x, r.x) <= 1e-4 and _rel(s.y, r.y) <= 1e-4
 
 
-@pytest.mark.parametrize("m,n", [(300, 900), (600, 1800), (1000, 3000)])
+@pytest.mark.parametrize("m,n", [(300, 900), (600, 1800), (1000, 3000), (1500, 900),
+                                 (2500, 1500), (2500, 2100)])
 def test_small_runner_register_layouts(m, n):
-    # the runner keeps 1, 2 or 3 columns (and rows) per thread: 64
-    # iterations of each layout follow the oracle like the three-kernel
+    # the runner keeps 1, 2 or 3 columns and, separately, 1, 2 or 3 rows per
+    # thread: 64 iterations of each layout follow the oracle like the three-kernel
     # path; in FP32 the two paths sum rows in different orders, and the
     # line search amplifies that rounding (about 10x every 8 iterations),
     # so they are compared over 16 iterations
