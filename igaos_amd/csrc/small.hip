@@ -447,7 +447,8 @@ bool small_fits(int n, int m, long nnz, bool f32) {
     const size_t lds = f32 ? small_lds<float>(n, m, true) : small_lds<double>(n, m, false);
     // the kernel's static LDS (the state) comes on top of the dynamic part
     static_assert(sizeof(LsState) <= 256, "state outgrew its LDS allowance");
-    return n > 0 && m > 0 && mx <= SMALL_MAX_DIM && nnz <= SMALL_MAX_NNZ &&
+    const long bytes = nnz * static_cast<long>((f32 ? sizeof(float) : sizeof(double)) + 2);
+    return n > 0 && m > 0 && mx <= SMALL_MAX_DIM && bytes <= SMALL_MAX_BYTES &&
            lds + 256 <= static_cast<size_t>(cap);
 }
 

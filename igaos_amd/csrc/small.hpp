@@ -12,9 +12,11 @@
 namespace igaos::dev {
 
 constexpr int SMALL_MAX_DIM = 3 * 1024;  // max(n, m): up to 3 columns / rows a thread
-// Past this many nonzeros one CU's share of L2 bandwidth streams the two
-// products slower than the whole GPU runs the three-kernel attempt.
-constexpr long SMALL_MAX_NNZ = 1L << 17;
+// Matrix bytes per product (values and 16-bit indices) up to which one CU
+// streams an attempt's two products faster than the whole GPU runs the
+// three-kernel attempt (MI355X: 2.0x at 150 KB, 1.5x at 300 KB, even at
+// 600 KB, 0.83x at 720 KB).
+constexpr long SMALL_MAX_BYTES = 480L * 1024;
 
 // Sliced-ELL copy of a scaled matrix for the runner (rows of A for A xbar,
 // columns for A'y).  Rows sit in slots in order of decreasing length
@@ -37,7 +39,7 @@ struct SmallEll {
 // Whether an LP (n columns, m rows, nnz nonzeros) fits the runner on the
 // current device: x, y and the per-launch constants (bounds, gradient)
 // within the device's LDS per workgroup, at most 3 columns and rows per
-// thread, at most SMALL_MAX_NNZ nonzeros.
+// thread, at most SMALL_MAX_BYTES of matrix per product.
 bool small_fits(int n, int m, long nnz, bool f32);
 // The slot order, chunk-row offsets of a CSR (CSC) pattern's ELL copy and,
 // per entry position, the CSR (CSC) position it holds (-1: padding);

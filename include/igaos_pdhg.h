@@ -138,14 +138,14 @@ typedef struct {
                           Exact mode (FP64 LPs with max(n, m) <= 16384 and
                           nnz <= 15360): the exact runner, bit-identical to
                           the exact mode's six kernels per attempt.  Default
-                          mode (max(n, m) <= 3072, the products, x and y
-                          within the LDS of a workgroup: nnz + n + m <=
-                          about 20,000 in FP64, 40,000 in FP32): the
-                          register / LDS runner, each column's and row's
-                          state in registers, the matrices streamed from
-                          L2, rows summed in storage order (1.1x the three
-                          kernels per attempt on the 1,000 x 3,000 LP in
-                          FP64, 1.4x in FP32).  -1 (default) and 1: wherever
+                          mode (max(n, m) <= 3072, x, y and the bounds
+                          within the LDS of a workgroup, nnz * (value + 2
+                          bytes) <= 480 KiB): the register / LDS runner,
+                          each column's and row's state in registers,
+                          length-sorted sliced-ELL copies of the matrices
+                          streamed from L2, rows summed in storage order
+                          (2.0x the three kernels per attempt on the
+                          1,000 x 3,000 LP in FP64, 2.3x in FP32).  -1 (default) and 1: wherever
                           the LP fits; 0: never. */
     int stream_loads;  /* how the attempt kernels load the matrix streams
                           (indices, values): -1 (default) nontemporal when
