@@ -575,7 +575,7 @@ void Engine<V>::make_ell(dev::SmallEll<V>& E, DBuf<int>& perm, DBuf<int>& off,
                          DBuf<unsigned short>& idx, DBuf<V>& val, int rows,
                          const std::vector<int>& ptr, const int* didx, const V* dval) {
     std::vector<int> hperm, hoff, hsrc;
-    const long total = dev::small_ell_map(rows, ptr.data(), hperm, hoff, hsrc);
+    const long total = dev::small_ell_map(rows, ptr.data(), dev::small_chunk<V>(), hperm, hoff, hsrc);
     DBuf<int> src;
     perm.upload(hperm.data(), hperm.size(), s_);
     off.upload(hoff.data(), hoff.size(), s_);

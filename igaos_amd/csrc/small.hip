@@ -104,24 +104,18 @@ __device__ __forceinline__ void fin3(const double* red, double& a, double& b, do
     }
 }
 
-template <typename V>
-struct Lane4;
-template <>
-struct Lane4<double> {
-    typedef double type __attribute__((ext_vector_type(4)));
+// a lane's chunk: C values, C 16-bit indices
+template <typename T, int C>
+struct LaneVec {
+    typedef T type __attribute__((ext_vector_type(C)));
 };
-template <>
-struct Lane4<float> {
-    typedef float type __attribute__((ext_vector_type(4)));
-};
-typedef unsigned short Idx4 __attribute__((ext_vector_type(4)));
 
 // out[k] = sum over the entries of the row in owned slot tid + k ST of
 // val * x[idx], in
 // storage order, the product rounded before the add (a separate multiply
 // and add, like the three-kernel path).  The row's entries come in chunks
-// of SMALL_CHUNK (one 16- or 32-byte value load and one 8-byte index load
-// per chunk): cb[k] is the chunk row where the row's slice starts (the same
+// of small_chunk<V>() (one 32-byte value load and one 16- or 8-byte index
+// load per chunk): cb[k] is the chunk row where the row's slice starts (the same
 // for the whole wavefront), nch[k] the row's chunks.  A chunk's padding
 // entries are 0 with index 0, and add +-0 (x is finite), which leaves a sum
 // that started at +0 unchanged bit for bit.  All K rows' chunks are loaded
@@ -131,11 +125,13 @@ template <typename V, int K>
 __device__ __forceinline__ void ell_sums(const SmallEll<V>& E, const int* cb, const int* nch,
                                          const V* x, V* out) {
 #pragma clang fp contract(off)
-    using V4 = typename Lane4<V>::type;
+    constexpr int C = small_chunk<V>();
+    using VC = typename LaneVec<V, C>::type;
+    using IC = typename LaneVec<unsigned short, C>::type;
     constexpr int UC = K == 1 ? 2 : 1;  // chunks of each row in flight
     const int lane = threadIdx.x & (WAVE - 1);
-    const V4* val = reinterpret_cast<const V4*>(E.val);
-    const Idx4* idx = reinterpret_cast<const Idx4*>(E.idx);
+    const VC* val = reinterpret_cast<const VC*>(E.val);
+    const IC* idx = reinterpret_cast<const IC*>(E.idx);
     int most = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -143,8 +139,8 @@ __device__ __forceinline__ void ell_sums(const SmallEll<V>& E, const int* cb, co
         out[k] = V(0);
     }
     for (int c = 0; c < most; c += UC) {
-        V4 v[K][UC];
-        Idx4 ix[K][UC];
+        VC v[K][UC];
+        IC ix[K][UC];
 #pragma unroll
         for (int k = 0; k < K; ++k)
 #pragma unroll
@@ -159,7 +155,7 @@ __device__ __forceinline__ void ell_sums(const SmallEll<V>& E, const int* cb, co
             for (int u = 0; u < UC; ++u) {
                 const V on = static_cast<V>(c + u < nch[k]);
 #pragma unroll
-                for (int g = 0; g < SMALL_CHUNK; ++g) out[k] += (v[k][u][g] * on) * x[ix[k][u][g]];
+                for (int g = 0; g < C; ++g) out[k] += (v[k][u][g] * on) * x[ix[k][u][g]];
             }
     }
 }
@@ -177,7 +173,8 @@ __device__ __forceinline__ void ell_rows(const SmallEll<V>& E, const int* __rest
         const int q = __builtin_amdgcn_readfirstlane(s / WAVE);
         cb[k] = q < slices ? E.off[q] : 0;
         const int r = s < rows ? E.perm[s] : 0;
-        nch[k] = s < rows ? (ptr[r + 1] - ptr[r] + SMALL_CHUNK - 1) / SMALL_CHUNK : 0;
+        constexpr int C = small_chunk<V>();
+        nch[k] = s < rows ? (ptr[r + 1] - ptr[r] + C - 1) / C : 0;
     }
 }
 
@@ -452,8 +449,8 @@ bool small_fits(int n, int m, long nnz, bool f32) {
            lds + 256 <= static_cast<size_t>(cap);
 }
 
-long small_ell_map(int rows, const int* ptr, std::vector<int>& perm, std::vector<int>& off,
-                   std::vector<int>& src) {
+long small_ell_map(int rows, const int* ptr, int chunk, std::vector<int>& perm,
+                   std::vector<int>& off, std::vector<int>& src) {
     // slots in order of decreasing row length (ties by row), so that the 64
     // rows of a slice are alike in length
     perm.resize(rows);
@@ -467,18 +464,18 @@ long small_ell_map(int rows, const int* ptr, std::vector<int>& perm, std::vector
         int width = 0;
         for (int s = q * WAVE; s < std::min(rows, (q + 1) * WAVE); ++s) {
             const int r = perm[s];
-            width = std::max(width, (ptr[r + 1] - ptr[r] + SMALL_CHUNK - 1) / SMALL_CHUNK);
+            width = std::max(width, (ptr[r + 1] - ptr[r] + chunk - 1) / chunk);
         }
         off[q + 1] = off[q] + width;
     }
     // at least one chunk row, so that chunk 0 exists
-    const long total = static_cast<long>(std::max(off[slices], 1)) * WAVE * SMALL_CHUNK;
+    const long total = static_cast<long>(std::max(off[slices], 1)) * WAVE * chunk;
     src.assign(static_cast<size_t>(total), -1);
     for (int s = 0; s < rows; ++s) {
         const int r = perm[s], q = s / WAVE, lane = s % WAVE;
         for (int e = 0; e < ptr[r + 1] - ptr[r]; ++e) {
-            const long chunk = static_cast<long>(off[q] + e / SMALL_CHUNK) * WAVE + lane;
-            src[static_cast<size_t>(chunk * SMALL_CHUNK + e % SMALL_CHUNK)] = ptr[r] + e;
+            const long c = static_cast<long>(off[q] + e / chunk) * WAVE + lane;
+            src[static_cast<size_t>(c * chunk + e % chunk)] = ptr[r] + e;
         }
     }
     return total;

@@ -22,12 +22,16 @@ constexpr long SMALL_MAX_BYTES = 480L * 1024;
 // columns for A'y).  Rows sit in slots in order of decreasing length
 // (perm[s] = the row in slot s), and slots go in slices of 64 (q = s / 64,
 // lane s % 64); a row's entries, in storage order, go in chunks of
-// SMALL_CHUNK, chunk c of slot s at chunk index (off[q] + c) * 64 + lane,
-// so that a wavefront loads one contiguous run per chunk; a slice is as
-// many chunk rows wide as its longest row needs, and padding entries are 0
-// with index 0.  Values: val[chunk * SMALL_CHUNK + g]; 16-bit indices
-// likewise.
-constexpr int SMALL_CHUNK = 4;
+// small_chunk<V>() entries, chunk c of slot s at chunk index
+// (off[q] + c) * 64 + lane, so that a wavefront loads one contiguous run
+// per chunk; a slice is as many chunk rows wide as its longest row needs,
+// and padding entries are 0 with index 0.  Values: val[chunk * C + g];
+// 16-bit indices likewise.  C = 8 in FP32 (32-byte value and 16-byte index
+// loads), 4 in FP64 (32-byte and 8-byte), within the register budget.
+template <typename V>
+constexpr int small_chunk() {
+    return sizeof(V) == 4 ? 8 : 4;
+}
 template <typename V>
 struct SmallEll {
     const int* perm = nullptr;  // rows: the row in each slot
@@ -44,8 +48,8 @@ bool small_fits(int n, int m, long nnz, bool f32);
 // The slot order, chunk-row offsets of a CSR (CSC) pattern's ELL copy and,
 // per entry position, the CSR (CSC) position it holds (-1: padding);
 // returns the number of entry positions (at least one chunk row).
-long small_ell_map(int rows, const int* ptr, std::vector<int>& perm, std::vector<int>& off,
-                   std::vector<int>& src);
+long small_ell_map(int rows, const int* ptr, int chunk, std::vector<int>& perm,
+                   std::vector<int>& off, std::vector<int>& src);
 // Fills an ELL copy's indices and values from the CSR (CSC) arrays.
 hipError_t small_ell_fill(const int* src, long total, const int* idx, const double* val,
                           unsigned short* eidx, double* eval, hipStream_t s);
