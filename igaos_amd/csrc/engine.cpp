@@ -169,6 +169,9 @@ class Engine final : public EngineBase {
     ~Engine() override {
         if (exec_) (void)hipGraphExecDestroy(exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
+        if (cexec_) (void)hipGraphExecDestroy(cexec_);
+        if (cgraph_) (void)hipGraphDestroy(cgraph_);
+        if (hck_) (void)hipHostFree(hck_);
         if (ev0_) (void)hipEventDestroy(ev0_);
         if (ev1_) (void)hipEventDestroy(ev1_);
         if (s_) (void)hipStreamDestroy(s_);
@@ -237,6 +240,17 @@ class Engine final : public EngineBase {
     DBuf<unsigned short> eidxT_, eidxA_;
     DBuf<V> evalT_, evalA_;
     dev::SmallEll<V> eT_, eA_;
+    // the runner's check period as one graph (graph_slots > 0): the runner,
+    // the state's pending flush, the KKT metrics at cur = 0 (where the
+    // runner leaves the current iterate) and the state and metrics copied
+    // to pinned host memory, so a check costs one launch and one wait
+    struct CheckOut {
+        LsState st;
+        Metrics met[2];
+    };
+    hipGraph_t cgraph_ = nullptr;
+    hipGraphExec_t cexec_ = nullptr;
+    CheckOut* hck_ = nullptr;
     static constexpr long long EXACT_POW_CAP = 1LL << 21;
     // attempts per runner launch: a check period of 64 accepted
     // iterations takes at most 64 x 21 attempts (pdhg.cu:785: the 21st
@@ -257,6 +271,7 @@ class Engine final : public EngineBase {
     double power_iteration();
     double dsum(const double* a, const double* b, long n);
     void check_metrics(int cur, Metrics& mc, Metrics& ma);
+    void kkt_launch(int cur);
     void recompute_refs();
     // y = A~ x (rows of A) in the engine's summation order
     void spmv_rows(const double* x, double* y) {
@@ -534,6 +549,17 @@ void Engine<V>::setup() {
         eb_.pow_cap = EXACT_POW_CAP;
     }
 
+    if (small_ && cfg_.graph_slots > 0) {
+        HIPCK(hipHostMalloc(reinterpret_cast<void**>(&hck_), sizeof(CheckOut)));
+        HIPCK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        HIPCK(dev::launch_small_run(ib_, eT_, eA_, RUNNER_CAP, s_));
+        dev::launch_flush_state(ib_, s_);
+        kkt_launch(0);
+        HIPCK(hipMemcpyAsync(&hck_->st, st_.p, sizeof(LsState), hipMemcpyDeviceToHost, s_));
+        HIPCK(hipMemcpyAsync(hck_->met, met_.p, sizeof(hck_->met), hipMemcpyDeviceToHost, s_));
+        HIPCK(hipStreamEndCapture(s_, &cgraph_));
+        HIPCK(hipGraphInstantiate(&cexec_, cgraph_, nullptr, nullptr, 0));
+    }
     const int G = (runner_ || small_) ? 0 : cfg_.graph_slots;
     if (G > 0) {  // G attempt slots, replayed until the state freezes
         HIPCK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
@@ -660,6 +686,17 @@ void Engine<V>::recompute_refs() {
 
 template <typename V>
 void Engine<V>::check_metrics(int cur, Metrics& mc, Metrics& ma) {
+    kkt_launch(cur);
+    Metrics h[2];
+    HIPCK(hipMemcpyAsync(h, met_.p, sizeof(h), hipMemcpyDeviceToHost, s_));
+    sync();
+    mc = h[0];
+    ma = h[1];
+}
+
+// Both iterates' KKT metrics into met_ (device work only).
+template <typename V>
+void Engine<V>::kkt_launch(int cur) {
     if constexpr (F32) {
         // both FP64 iterates (reference + FP32 correction) formed once, so
         // the KKT SpMVs gather one double per nonzero instead of a double
@@ -695,11 +732,6 @@ void Engine<V>::check_metrics(int cur, Metrics& mc, Metrics& ma) {
         it.yd[1] = yavg_.p;
         dev::launch_kkt(kd_, it, met_.p, s_);
     }
-    Metrics h[2];
-    HIPCK(hipMemcpyAsync(h, met_.p, sizeof(h), hipMemcpyDeviceToHost, s_));
-    sync();
-    mc = h[0];
-    ma = h[1];
 }
 
 // Device dual repair of the scaled point in tmpn_ / tmpm_ (its metrics t):
@@ -824,16 +856,27 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
 
     const double t0 = now_s();
     while (true) {
+        bool have_met = false;  // the check graph brought the metrics
+        Metrics gmet[2];
         if (!st.frozen) {
             // attempts until the device freezes at the next check boundary
             HIPCK(hipEventRecord(ev0_, s_));
-            while (true) {
+            if (cexec_) {
+                HIPCK(hipGraphLaunch(cexec_, s_));
+                sync();
+                st = hck_->st;  // flushed; cur = 0
+                if (st.frozen) {
+                    have_met = true;
+                    gmet[0] = hck_->met[0];
+                    gmet[1] = hck_->met[1];
+                }
+            }
+            while (!st.frozen) {
                 const long k = st.k;
                 long next = ((k - 1) / CHECK_EVERY + 1) * CHECK_EVERY;
                 if (next > MAXIT) next = MAXIT;
                 launch_slots(std::max(1L, next - k + 1));
                 st = read_state();
-                if (st.frozen) break;
             }
             HIPCK(hipEventRecord(ev1_, s_));
             HIPCK(hipEventSynchronize(ev1_));
@@ -855,7 +898,12 @@ Solution Engine<V>::run(const Options& opt_, SolveInfo* info) {
 
         // ---- check (pdhg.cu:801-1084)
         Metrics mc, ma;
-        check_metrics(cur, mc, ma);
+        if (have_met) {
+            mc = gmet[0];
+            ma = gmet[1];
+        } else {
+            check_metrics(cur, mc, ma);
+        }
         const double ec = mc.finite_res() ? mc.err() : INF;
         const double ea = ma.finite_res() ? ma.err() : INF;
         last = (ec <= ea) ? mc : ma;

@@ -42,7 +42,8 @@ struct SolveInfo {
     int last_is_avg = 0;
     int used_incumbent = 0;
     int repaired = 0;
-    double kernel_ms = 0.0;    // device time of the attempt kernels (events)
+    double kernel_ms = 0.0;    // device time of the attempt kernels (events; with the
+                               // small-LP runner's check graph, its check work too)
     long rollbacks = 0;        // incumbent rollbacks (pdhg.cu:965-988)
 };
 

@@ -564,6 +564,33 @@ static void flush_impl(const IterBufs<V>& b, int cur, double w, hipStream_t s) {
 void launch_flush(const IterBufs<double>& b, int cur, double w, hipStream_t s) { flush_impl(b, cur, w, s); }
 void launch_flush(const IterBufs<float>& b, int cur, double w, hipStream_t s) { flush_impl(b, cur, w, s); }
 
+// The flush as the device state asks for it (a no-op without a pending
+// step), then the flag cleared by a kernel of its own: a block of the
+// flush must not see the flag cleared by another.
+template <typename V>
+__global__ __launch_bounds__(BS) void k_flush_state(IterBufs<V> b) {
+    const LsState* st = b.st;
+    if (!st->mix_pending) return;
+    const int cur = st->cur;
+    const V w = static_cast<V>(st->mix_w);
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < b.n) b.xavg[i] += w * (b.x[cur][i] - b.xavg[i]);
+    if (i < b.m) {
+        b.yavg[i] += w * (b.y[cur][i] - b.yavg[i]);
+        b.axcur[i] += b.adx[i];
+    }
+}
+__global__ void k_clear_mix(LsState* st) { st->mix_pending = 0; }
+
+template <typename V>
+static void flush_state_impl(const IterBufs<V>& b, hipStream_t s) {
+    const long mx = b.n > b.m ? b.n : b.m;
+    if (mx > 0) k_flush_state<V><<<(mx + BS - 1) / BS, BS, 0, s>>>(b);
+    k_clear_mix<<<1, 1, 0, s>>>(b.st);
+}
+void launch_flush_state(const IterBufs<double>& b, hipStream_t s) { flush_state_impl(b, s); }
+void launch_flush_state(const IterBufs<float>& b, hipStream_t s) { flush_state_impl(b, s); }
+
 // ------------------------------------------------------------------ KKT --
 // Column pass: d = c + (A~'y~)/dc, dinf numerator, c'x and the column part
 // of the Lagrangian bound (pdhg.cu:206-233).  Row pass: Ax = dr (A~x~),

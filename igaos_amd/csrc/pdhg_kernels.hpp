@@ -124,6 +124,10 @@ void launch_decide(const IterBufs<float>& b, hipStream_t s);
 // Apply the averaging / A x commit of the last accepted step.
 void launch_flush(const IterBufs<double>& b, int cur, double w, hipStream_t s);
 void launch_flush(const IterBufs<float>& b, int cur, double w, hipStream_t s);
+// The same, with cur, w and whether a step is pending read from the device
+// state (b.st), whose pending flag it then clears: for captured graphs.
+void launch_flush_state(const IterBufs<double>& b, hipStream_t s);
+void launch_flush_state(const IterBufs<float>& b, hipStream_t s);
 
 // KKT evaluation in original units for two iterates at once
 // (src/pdhg/pdhg.cu:173-254 via metrics_of, 694-733).

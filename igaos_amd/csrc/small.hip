@@ -20,10 +20,12 @@
 //     order and thread 0 applies the decision rule to the state, which
 //     lives in LDS, so no global memory traffic is needed between
 //     attempts.
-// On exit the state and the vectors go back to the engine's buffers exactly
-// as the three kernels leave them (current iterate in x[cur], y[cur]; the
-// pending averaging flagged in the state), so the host's check code is the
-// same for both paths.  Summation orders: every sparse row in storage
+// On exit the state and the vectors go back to the engine's buffers as the
+// three kernels leave them, with the current iterate always in x[0], y[0]
+// (the state's cur set to 0; nothing reads the other slot before the next
+// attempt writes it) and the pending averaging flagged in the state, so the
+// host's check code is the same for both paths and a check's device work
+// can be captured once with cur = 0.  Summation orders: every sparse row in storage
 // order, dots per thread then wavefront trees then the 16 wavefronts in
 // order; the results agree with the oracle to rounding like the
 // three-kernel path's.
@@ -348,13 +350,13 @@ __global__ __launch_bounds__(ST) void k_small_run(IterBufs<V> b, SmallEll<V> eT,
             }
         }
     }
-    // back to the engine's buffers, as the three kernels leave them
+    // back to the engine's buffers, the current iterate in slot 0
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
         if (tid + k * ST < n) {
             const int j = jc[k];
-            b.x[st.cur][j] = xc[k];
-            b.x[st.cur ^ 1][j] = xn[k];
+            b.x[0][j] = xc[k];
+            b.x[1][j] = xn[k];
             b.xavg[j] = xa[k];
             b.ata[j] = at[k];
         }
@@ -364,14 +366,18 @@ __global__ __launch_bounds__(ST) void k_small_run(IterBufs<V> b, SmallEll<V> eT,
     for (int k = 0; k < KR; ++k) {
         if (tid + k * ST < m) {
             const int i = ir[k];
-            b.y[st.cur][i] = yc[k];
-            b.y[st.cur ^ 1][i] = ys[i];
+            b.y[0][i] = yc[k];
+            b.y[1][i] = ys[i];
             b.yavg[i] = ya[k];
             b.axcur[i] = ax[k];
             b.adx[i] = ad[k];
         }
     }
-    if (tid == 0) *b.st = st;
+    if (tid == 0) {
+        LsState t = st;
+        t.cur = 0;
+        *b.st = t;
+    }
 }
 
 // x, y and the constants (4 n + 3 m values, and m offsets with yoff)

@@ -166,7 +166,9 @@ typedef struct {
     long attempts, restarts, checks, refinements;
     double eta, omega, lam2, eta0, omega0;
     int last_is_avg, used_incumbent, repaired;
-    double kernel_ms;
+    double kernel_ms; /* device time of the attempts (events); with the
+                         small-LP runner's check graph, of the check's
+                         device work too */
     long rollbacks; /* incumbent rollbacks (pdhg.cu:965-988) */
 } igaos_solve_info;
 

@@ -183,6 +183,25 @@ def test_small_runner_register_layouts(m, n):
     assert _rel(b.x, a.x) <= 1e-3 and _rel(b.y, a.y) <= 1e-3
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_runner_check_graph_matches_direct_launches(precision):
+    # the runner's check periods replayed as one captured graph (runner,
+    # flush, KKT metrics, copies to the host) against the same work launched
+    # directly (graph_slots = 0): the same run, bit for bit, over 40 checks
+    # with their restarts, and the same solve to 1e-6
+    lp = lpgen.gen_random_eq(seed=0)
+    a, b = (igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=2560,
+                               precision=precision, small_runner=1, graph_slots=g)
+            for g in (16, 0))
+    assert a.iterations == b.iterations == 2560
+    assert a.info["restarts"] == b.info["restarts"] and a.info["checks"] == b.info["checks"]
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    a, b = (igaos_amd.solve_lp(lp, time_limit=60, tolerance=1e-6, precision=precision,
+                               small_runner=1, graph_slots=g) for g in (16, 0))
+    assert a.status == b.status == "near-optimal"
+    assert a.iterations == b.iterations and a.objective == b.objective
+
+
 def _check_solution(s, ref_obj, tol=1e-6):
     assert s.status == "near-optimal", (s.status, s.message)
     assert s.pinf <= tol and s.rel_gap <= tol
