@@ -257,6 +257,11 @@ __global__ __launch_bounds__(ST) void k_small_run(IterBufs<V> b, SmallEll<V> eT,
     ell_rows<V, KC>(eT, b.cp, n, cbT, nT);
     ell_rows<V, KR>(eA, b.ap, m, cbA, nA);
     __syncthreads();
+    // A'y of the last attempt's y, computed while thread 0 decides: the
+    // next attempt's A'y if the attempt is accepted (the same sums of the
+    // same ys, so the same values)
+    V atn[KC];
+    bool spec = false;
     for (int att = 0; att < cap && !st.frozen && st.k - k0 < SMALL_PW; ++att) {
         const V tau = static_cast<V>(st.eta / st.omega);
         const V sig = static_cast<V>(st.eta * st.omega);
@@ -264,7 +269,14 @@ __global__ __launch_bounds__(ST) void k_small_run(IterBufs<V> b, SmallEll<V> eT,
         const bool fresh = st.fresh != 0;
         const bool mix = fresh && st.mix_pending != 0;
         // A'y (a retry reuses the stored one) and the primal step (k_primal)
-        if (fresh) ell_sums<V, KC>(eT, cbT, nT, ys, at);
+        if (fresh) {
+            if (spec) {
+#pragma unroll
+                for (int k = 0; k < KC; ++k) at[k] = atn[k];
+            } else {
+                ell_sums<V, KC>(eT, cbT, nT, ys, at);
+            }
+        }
         // the step's data for every owned column in one batch of reads (a
         // slot past n reads slot n - 1 and is not stepped)
         V g0[KC], l0[KC], h0[KC];
@@ -340,6 +352,8 @@ __global__ __launch_bounds__(ST) void k_small_run(IterBufs<V> b, SmallEll<V> eT,
             pdhg::ls_decide_pw(t, s0, s1, s2, pw[kq], pw[SMALL_PW + kq]);
             st = t;
         }
+        ell_sums<V, KC>(eT, cbT, nT, ys, atn);
+        spec = true;
         __syncthreads();  // the state is decided; red is free again
         if (st.cur != before) {  // accepted: the attempt becomes the iterate
 #pragma unroll
