@@ -199,8 +199,9 @@ def bench_batch(args):
 
     # plain FP64 single-LP engine, instance by instance, on a fixed sample,
     # within its own budget (each instance gets what is left, at most the
-    # per-instance limit)
-    sample = list(range(0, count, max(1, count // args.fp64_sample)))
+    # per-instance limit); --fp64-sample 0 skips it
+    sample = (list(range(0, count, max(1, count // args.fp64_sample)))
+              if args.fp64_sample > 0 else [])
     t64 = 0.0
     solved64 = 0
     run64 = []
