@@ -113,11 +113,14 @@ class LP:
         """Row-sequential A @ x in FP64 (host helper for generators)."""
         x = np.asarray(x, dtype=np.float64)
         prod = self.ax * x[self.ai]
-        out = np.add.reduceat(prod, self.ap[:-1]) if self.nnz else \
-            np.zeros(self.m)
-        out = np.asarray(out, dtype=np.float64)
-        empty = self.ap[1:] == self.ap[:-1]
-        out[empty] = 0.0
+        out = np.zeros(self.m)
+        if self.nnz:
+            # reduceat over the rows up to the last non-empty one (a start
+            # index equal to nnz is out of its range); empty rows are 0
+            last = int(np.searchsorted(self.ap[:-1], self.nnz, side="left"))
+            out[:last] = np.add.reduceat(prod, self.ap[:last])
+            empty = self.ap[1:] == self.ap[:-1]
+            out[empty] = 0.0
         return out
 
     def to_scipy(self):
@@ -170,6 +173,48 @@ def gen_random_eq(seed=0, m=1000, n=3000, per_col=5, box=10.0, name=None):
     b = lp.matvec(x0)
     lp.rmin[:] = b
     lp.rmax[:] = b
+    return lp
+
+
+def gen_ragged(seed=5, m=700, n=1500, name=None):
+    """Small LP with the structural edge cases of sparse kernels: two empty
+    rows and three empty columns, one row and one column far longer than the
+    rest (every third column; every second row), ranged, one-sided and
+    free-below rows, and free, one-sided and fixed columns.  Rows: b = A x0
+    with x0 = clip(5 U[0,1], bounds); the costs of columns without an upper
+    bound are |N(0,1)| and free columns cost 0, so the LP is bounded."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for j in range(n - 3):  # columns n-3 .. n-1 stay empty
+        k = 1 + int(rng.integers(0, 7))
+        rows.extend(rng.choice(m - 2, size=k, replace=False))  # rows m-2, m-1 stay empty
+        cols.extend([j] * k)
+    rows.extend(range(0, m - 2, 2)); cols.extend([n - 4] * len(range(0, m - 2, 2)))  # long column
+    long_cols = [c for c in range(0, n - 4, 3) if c != n - 4]
+    rows.extend([1] * len(long_cols)); cols.extend(long_cols)  # long row
+    trip = {}
+    for r, c in zip(rows, cols):
+        trip[(int(r), int(c))] = rng.standard_normal()
+    keys = sorted(trip)
+    rr = np.array([k[0] for k in keys]); cc = np.array([k[1] for k in keys])
+    vv = np.array([trip[k] for k in keys])
+    cl = np.zeros(n); cu = np.full(n, 10.0)
+    cl[::11] = -np.inf; cu[::11] = np.inf          # free columns
+    cu[5::13] = np.inf                              # one-sided
+    cl[7::17] = cu[7::17] = 2.0                     # fixed
+    cl[n - 3:] = 0.0; cu[n - 3:] = 4.0              # the empty columns boxed
+    c = rng.standard_normal(n)
+    c[cu == np.inf] = np.abs(c[cu == np.inf])  # bounded below: no improving ray
+    c[cl == -np.inf] = 0.0
+    x0 = np.clip(rng.random(n) * 5, cl, cu)
+    lp = LP.from_triplets(m, n, rr, cc, vv, c, cl, cu, np.zeros(m), np.zeros(m),
+                          name=name or f"ragged_m{m}_n{n}_s{seed}")
+    b = lp.matvec(x0)
+    lp.rmin[:] = b; lp.rmax[:] = b
+    lp.rmin[::5] = b[::5] - 1.0                     # ranged rows
+    lp.rmax[3::7] = np.inf                          # one-sided rows
+    lp.rmin[4::9] = -np.inf; lp.rmax[4::9] = b[4::9] + 0.5
+    lp.rmin[m - 2:] = -1.0; lp.rmax[m - 2:] = 1.0   # the empty rows
     return lp
 
 

@@ -13,8 +13,8 @@ Generated LPs are named by generator, parameters and seed (see
 igaos_amd/lpgen.py, GENERATOR_VERSION).
 
 Run:  python tests/golden/make_fixtures.py   (about 3 minutes)
-      python tests/golden/make_fixtures.py --add   (only the LARGE entries,
-      added to the existing file: about 3 minutes)
+      python tests/golden/make_fixtures.py --add   (the EXTRA and LARGE
+      entries the existing file lacks, added to it)
 
 LARGE entries use HiGHS's interior-point method with crossover
 (method="highs-ipm"): its default simplex needs over 15 minutes on the
@@ -62,14 +62,35 @@ def scipy_highs(lp, method="highs"):
 
 # (generator, parameters): solved with HiGHS's interior-point method
 LARGE = [("gen_multiperiod", dict(seed=0, periods=8, rows_per=1000, cols_per=1600, inv=200))]
+# small cases added after the first generation (also part of a full one)
+EXTRA = [("gen_ragged", dict(seed=5))]
 
 
 def key_of(fn, kw):
     return fn + "(" + ",".join(f"{k}={v}" for k, v in kw.items()) + ")"
 
 
+def small_entry(lp, have_ref):
+    ent = {"m": lp.m, "n": lp.n, "nnz": lp.nnz, "scipy_highs": scipy_highs(lp)}
+    if have_ref and lp.m * lp.n <= 3_000_000:
+        t = time.time()
+        st, obj, _, _, its, ms = oracle.ref_simplex(lp, time_limit=600)
+        ent.update(ref_simplex=obj, ref_simplex_status=st, ref_simplex_iterations=its,
+                   ref_simplex_seconds=time.time() - t)
+    return ent
+
+
+def add_extra(E, have_ref):
+    for fn, kw in EXTRA:
+        if key_of(fn, kw) not in E:
+            E[key_of(fn, kw)] = small_entry(getattr(lpgen, fn)(**kw), have_ref)
+            print(key_of(fn, kw), E[key_of(fn, kw)], flush=True)
+
+
 def add_large(E):
     for fn, kw in LARGE:
+        if key_of(fn, kw) in E:
+            continue
         lp = getattr(lpgen, fn)(**kw)
         t = time.time()
         E[key_of(fn, kw)] = {"m": lp.m, "n": lp.n, "nnz": lp.nnz,
@@ -84,6 +105,7 @@ def main():
         path = os.path.join(HERE, "objectives.json")
         with open(path) as f:
             out = json.load(f)
+        add_extra(out["entries"], oracle.have_ref_simplex())
         add_large(out["entries"])
         with open(path, "w") as f:
             json.dump(out, f, indent=1, sort_keys=True)
@@ -111,18 +133,9 @@ def main():
                                        cols_per=320, inv=40))]
     cases += [("gen_zipf_badly_scaled", dict(seed=0, m=400, n=1000))]
     for fn, kw in cases:
-        lp = getattr(lpgen, fn)(**kw)
-        key = fn + "(" + ",".join(f"{k}={v}" for k, v in kw.items()) + ")"
-        ent = {"m": lp.m, "n": lp.n, "nnz": lp.nnz,
-               "scipy_highs": scipy_highs(lp)}
-        if have_ref and lp.m * lp.n <= 3_000_000:
-            t = time.time()
-            st, obj, _, _, its, ms = oracle.ref_simplex(lp, time_limit=600)
-            ent.update(ref_simplex=obj, ref_simplex_status=st,
-                       ref_simplex_iterations=its,
-                       ref_simplex_seconds=time.time() - t)
-        E[key] = ent
-        print(key, ent, flush=True)
+        E[key_of(fn, kw)] = small_entry(getattr(lpgen, fn)(**kw), have_ref)
+        print(key_of(fn, kw), E[key_of(fn, kw)], flush=True)
+    add_extra(E, have_ref)
     add_large(E)
     with open(os.path.join(HERE, "objectives.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)

@@ -206,6 +206,26 @@ def _rowblocks(ptr):
     return out
 
 
+def test_generator_row_products_with_empty_rows():
+    # lpgen's row-sequential A x (the generators' right-hand sides) when the
+    # last rows are empty (a reduceat start at nnz) and when an empty row
+    # sits inside; and the ragged LP holds the shapes its docstring names
+    lp = lpgen.gen_ragged(seed=5)
+    A = lp.to_scipy().tocsr()
+    x = np.random.default_rng(0).random(lp.n)
+    ax = A @ x
+    y = lp.matvec(x)
+    assert np.abs(y - ax).max() <= 1e-12 * (1 + np.abs(ax).max())
+    rl, cl = np.diff(A.indptr), np.diff(A.tocsc().indptr)
+    assert (rl[-2:] == 0).all() and (y[-2:] == 0).all() and (cl[-3:] == 0).all()
+    assert rl.max() > 20 * np.median(rl) and cl.max() > 20 * np.median(cl)
+    assert np.isinf(lp.cl).any() and np.isinf(lp.cu).any() and (lp.cl == lp.cu).any()
+    assert np.isinf(lp.rmax).any() and np.isinf(lp.rmin).any()
+    mid = lpgen.LP.from_triplets(3, 2, [0, 2, 2], [0, 0, 1], [1.0, 2.0, 3.0], np.zeros(2),
+                                 np.zeros(2), np.ones(2), np.zeros(3), np.zeros(3))
+    assert np.array_equal(mid.matvec(np.array([1.0, 10.0])), np.array([1.0, 0.0, 32.0]))
+
+
 @pytest.mark.parametrize("gen", ["zipf", "eq", "empty"])
 def test_rowblock_invariants(gen):
     if gen == "zipf":

@@ -183,6 +183,29 @@ def test_small_runner_register_layouts(m, n):
     assert _rel(b.x, a.x) <= 1e-3 and _rel(b.y, a.y) <= 1e-3
 
 
+def test_small_runner_on_ragged_lp(objectives):
+    # empty rows and columns, one long row and column, every bound kind: 64
+    # runner iterations follow the oracle like the three-kernel path does,
+    # and the runner's solve to 1e-6 certifies at HiGHS's objective
+    lp = lpgen.gen_ragged(seed=5)
+    r = oracle.solve(lp, time_limit=1e7, tolerance=0.0, max_iterations=64)
+    for runner in (0, 1):
+        s = igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=64,
+                               small_runner=runner)
+        assert s.iterations == r.iterations == 64
+        assert s.info["attempts"] == r.attempts and s.info["restarts"] == r.restarts
+        assert _rel(s.x, r.x) <= 1e-9 and _rel(s.y, r.y) <= 1e-9
+    a, b = (igaos_amd.solve_lp(lp, time_limit=1e7, tolerance=0.0, max_iterations=16,
+                               precision=32, small_runner=runner) for runner in (0, 1))
+    assert _rel(b.x, a.x) <= 1e-3 and _rel(b.y, a.y) <= 1e-3
+    ref = objectives["gen_ragged(seed=5)"]["scipy_highs"]
+    for prec in (64, 32):
+        s = igaos_amd.solve_lp(lp, tolerance=1e-6, time_limit=60, precision=prec,
+                               small_runner=1)
+        assert s.status == "near-optimal", (prec, s.status, s.message)
+        assert abs(s.objective - ref) <= 2e-6 * (1 + abs(ref))
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 def test_runner_check_graph_matches_direct_launches(precision):
     # the runner's check periods replayed as one captured graph (runner,
