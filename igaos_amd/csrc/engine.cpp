@@ -242,8 +242,9 @@ class Engine final : public EngineBase {
     dev::SmallEll<V> eT_, eA_;
     // the runner's check period as one graph (graph_slots > 0): the runner,
     // the state's pending flush, the KKT metrics at cur = 0 (where the
-    // runner leaves the current iterate) and the state and metrics copied
-    // to pinned host memory, so a check costs one launch and one wait
+    // runner leaves the current iterate; k_kkt_done clears the flush's
+    // flag) and the state and metrics copied to pinned host memory, so a
+    // check costs one launch and one wait
     struct CheckOut {
         LsState st;
         Metrics met[2];
@@ -554,7 +555,9 @@ void Engine<V>::setup() {
         HIPCK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         HIPCK(dev::launch_small_run(ib_, eT_, eA_, RUNNER_CAP, s_));
         dev::launch_flush_state(ib_, s_);
+        kd_.clear_mix = st_.p;  // k_kkt_done clears the flag in the graph
         kkt_launch(0);
+        kd_.clear_mix = nullptr;
         HIPCK(hipMemcpyAsync(&hck_->st, st_.p, sizeof(LsState), hipMemcpyDeviceToHost, s_));
         HIPCK(hipMemcpyAsync(hck_->met, met_.p, sizeof(hck_->met), hipMemcpyDeviceToHost, s_));
         HIPCK(hipStreamEndCapture(s_, &cgraph_));
@@ -701,11 +704,21 @@ void Engine<V>::kkt_launch(int cur) {
         // both FP64 iterates (reference + FP32 correction) formed once, so
         // the KKT SpMVs gather one double per nonzero instead of a double
         // and a float (same values)
-        const dev::BoundSnap b = snap_x();
-        dev::combine(xref_.p, xcur(cur), kx_[0].p, n_, s_, &b);
-        dev::combine(yref_.p, ycur(cur), ky_[0].p, m_, s_);
-        dev::combine(xref_.p, xavg_.p, kx_[1].p, n_, s_, &b);
-        dev::combine(yref_.p, yavg_.p, ky_[1].p, m_, s_);
+        dev::Combine4 c4;
+        c4.xref = xref_.p;
+        c4.yref = yref_.p;
+        c4.xd[0] = xcur(cur);
+        c4.xd[1] = xavg_.p;
+        c4.yd[0] = ycur(cur);
+        c4.yd[1] = yavg_.p;
+        for (int q = 0; q < 2; ++q) {
+            c4.ox[q] = kx_[q].p;
+            c4.oy[q] = ky_[q].p;
+        }
+        c4.n = n_;
+        c4.m = m_;
+        c4.snap = snap_x();
+        dev::combine4(c4, s_);
         dev::KktIter<double> it;
         for (int q = 0; q < 2; ++q) {
             it.xd[q] = kx_[q].p;

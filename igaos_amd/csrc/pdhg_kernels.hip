@@ -565,8 +565,9 @@ void launch_flush(const IterBufs<double>& b, int cur, double w, hipStream_t s) {
 void launch_flush(const IterBufs<float>& b, int cur, double w, hipStream_t s) { flush_impl(b, cur, w, s); }
 
 // The flush as the device state asks for it (a no-op without a pending
-// step), then the flag cleared by a kernel of its own: a block of the
-// flush must not see the flag cleared by another.
+// step).  The flag is cleared by a later kernel (k_kkt_done with
+// KktData::clear_mix): a block of the flush must not see it cleared by
+// another.
 template <typename V>
 __global__ __launch_bounds__(BS) void k_flush_state(IterBufs<V> b) {
     const LsState* st = b.st;
@@ -580,13 +581,10 @@ __global__ __launch_bounds__(BS) void k_flush_state(IterBufs<V> b) {
         b.axcur[i] += b.adx[i];
     }
 }
-__global__ void k_clear_mix(LsState* st) { st->mix_pending = 0; }
-
 template <typename V>
 static void flush_state_impl(const IterBufs<V>& b, hipStream_t s) {
     const long mx = b.n > b.m ? b.n : b.m;
     if (mx > 0) k_flush_state<V><<<(mx + BS - 1) / BS, BS, 0, s>>>(b);
-    k_clear_mix<<<1, 1, 0, s>>>(b.st);
 }
 void launch_flush_state(const IterBufs<double>& b, hipStream_t s) { flush_state_impl(b, s); }
 void launch_flush_state(const IterBufs<float>& b, hipStream_t s) { flush_state_impl(b, s); }
@@ -753,6 +751,7 @@ __global__ __launch_bounds__(FIN) void k_kkt_done(KktData d, int nbC, int nbB,
             out[q] = mt;
         }
     }
+    if (threadIdx.x == 0 && d.clear_mix) d.clear_mix->mix_pending = 0;
 }
 
 static int kkt_rows2_blocks(int m) {
@@ -1073,6 +1072,22 @@ void combine(const double* ref, const float* d, double* out, long n, hipStream_t
              const BoundSnap* snap) {
     if (n > 0)
         k_combine<<<elem_grid(n), BS, 0, s>>>(ref, d, out, n, snap ? *snap : BoundSnap{});
+}
+
+__global__ void k_combine4(Combine4 c) {
+    const long i = static_cast<long>(blockIdx.x) * BS + threadIdx.x;
+    if (i < c.n) {
+        c.ox[0][i] = snapped(c.xref, c.xd[0], i, c.snap);
+        c.ox[1][i] = snapped(c.xref, c.xd[1], i, c.snap);
+    }
+    if (i < c.m) {
+        c.oy[0][i] = snapped(c.yref, c.yd[0], i, BoundSnap{});
+        c.oy[1][i] = snapped(c.yref, c.yd[1], i, BoundSnap{});
+    }
+}
+void combine4(const Combine4& c, hipStream_t s) {
+    const long mx = c.n > c.m ? c.n : c.m;
+    if (mx > 0) k_combine4<<<elem_grid(mx), BS, 0, s>>>(c);
 }
 
 __global__ void k_fold(double* ref, float* d, long n, BoundSnap b) {

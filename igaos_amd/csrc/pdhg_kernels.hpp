@@ -125,7 +125,8 @@ void launch_decide(const IterBufs<float>& b, hipStream_t s);
 void launch_flush(const IterBufs<double>& b, int cur, double w, hipStream_t s);
 void launch_flush(const IterBufs<float>& b, int cur, double w, hipStream_t s);
 // The same, with cur, w and whether a step is pending read from the device
-// state (b.st), whose pending flag it then clears: for captured graphs.
+// state (b.st), for captured graphs; a later kernel must clear the state's
+// pending flag (k_kkt_done does, given KktData::clear_mix).
 void launch_flush_state(const IterBufs<double>& b, hipStream_t s);
 void launch_flush_state(const IterBufs<float>& b, hipStream_t s);
 
@@ -142,6 +143,8 @@ struct KktData {
     double cmax = 0.0;
     double *partC = nullptr, *partR = nullptr, *partR2 = nullptr;
     double* rowres = nullptr;        // 2 x 4 row-pass results
+    pdhg::LsState* clear_mix = nullptr;  // k_kkt_done clears this state's pending
+                                         // flag (captured check graphs)
 };
 template <typename V>
 struct KktIter {
@@ -209,6 +212,18 @@ struct BoundSnap {
 };
 void combine(const double* ref, const float* d, double* out, long n,
              hipStream_t s, const BoundSnap* snap = nullptr);
+// Four combines in one launch, each element as combine computes it:
+// ox[q] = xref + xd[q] (snapped), oy[q] = yref + yd[q].
+struct Combine4 {
+    const double *xref = nullptr, *yref = nullptr;
+    const float* xd[2] = {nullptr, nullptr};
+    const float* yd[2] = {nullptr, nullptr};
+    double* ox[2] = {nullptr, nullptr};
+    double* oy[2] = {nullptr, nullptr};
+    long n = 0, m = 0;
+    BoundSnap snap;
+};
+void combine4(const Combine4& c, hipStream_t s);
 // ref = ref + d (snapped as combine does), in place on ref; d zeroed.
 void fold(double* ref, float* d, long n, hipStream_t s, const BoundSnap* snap = nullptr);
 // Shifted FP32 data of the correction problem (see engine.cpp).
