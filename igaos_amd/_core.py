@@ -92,6 +92,7 @@ SIGNATURES = {
     "igaos_rowblocks_build": (C.c_int, [C.c_int, _ip, _ip, C.c_int]),
     "igaos_slices_build": (C.c_int, [C.c_int, C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip,
                                      _ip, _ip, C.c_int]),
+    "igaos_small_ell_layout": (C.c_long, [C.c_int, _ip, C.c_int, _ip, _ip, _ip, C.c_long]),
     "igaos_slices_gather_order": (C.c_int, [C.c_int, C.c_int, _ip, _ip, C.c_int, _ip, _ip,
                                             _vp]),
     "igaos_spmv_csr_f64": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -13,6 +13,7 @@
 #include "igaos/status.h"
 #include "model.hpp"
 #include "pdhg.hpp"
+#include "small.hpp"
 #include "exact.hpp"
 #include "repair_dev.hpp"
 #include "transpose.hpp"
@@ -641,6 +642,25 @@ int igaos_slices_gather_order(int nrows, int ncols, const int* ptr, const int* i
         if (src) std::memcpy(src, h.src.data(), sizeof(int) * nnz);
         if (slot) std::memcpy(slot, h.slot.data(), sizeof(unsigned short) * nnz);
         return static_cast<int>(nnz);
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+}
+
+long igaos_small_ell_layout(int nrows, const int* ptr, int chunk, int* perm, int* off, int* src,
+                            long src_cap) {
+    if (nrows < 0 || !ptr || !perm || !off) return fail("invalid arrays");
+    if (chunk < 1 || chunk > 16) return fail("chunk must be 1..16");
+    if (ptr[0] != 0) return fail("ptr[0] must be 0");
+    for (int r = 0; r < nrows; ++r)
+        if (ptr[r + 1] < ptr[r]) return fail("ptr must not decrease");
+    try {
+        std::vector<int> hperm, hoff, hsrc;
+        const long total = igaos::dev::small_ell_map(nrows, ptr, chunk, hperm, hoff, hsrc);
+        std::copy(hperm.begin(), hperm.end(), perm);
+        std::copy(hoff.begin(), hoff.end(), off);
+        if (src && src_cap >= total) std::copy(hsrc.begin(), hsrc.end(), src);
+        return total;
     } catch (const std::exception& e) {
         return fail(e.what());
     }

@@ -465,6 +465,18 @@ int igaos_slices_build(int nrows, int ncols, const int* ptr, const int* idx, int
  * pattern. */
 int igaos_slices_gather_order(int nrows, int ncols, const int* ptr, const int* idx,
                               int slices, int* sidx, int* src, unsigned short* slot);
+/* Host: the sliced-ELL layout the small-LP runner streams (no reference
+ * counterpart): rows go in slots in order of decreasing length (perm[s] =
+ * the row in slot s, ties by row), slots in slices of 64, and a row's
+ * entries, in storage order, in chunks of `chunk` (the runner uses 4 in
+ * FP64, 8 in FP32): entry e of slot s sits at position
+ * ((off[s / 64] + e / chunk) * 64 + s % 64) * chunk + e % chunk, and
+ * src[p] is the CSR position held at position p (-1: padding).  Writes
+ * perm (nrows) and off (ceil(nrows / 64) + 1), and src when src_cap holds
+ * all positions; returns the number of positions (at least one chunk row),
+ * or -1 (igaos_last_error) for a malformed pattern or chunk outside 1..16. */
+long igaos_small_ell_layout(int nrows, const int* ptr, int chunk, int* perm, int* off, int* src,
+                            long src_cap);
 /* y = A x with A in CSR (device arrays); replaces the cusparseSpMV calls
  * of src/pdhg/pdhg.cu:150-156 for A and A' (CSR of A' = CSC of A). */
 int igaos_spmv_csr_f64(int nblk, const int* rowblocks, const int* ptr,

@@ -267,6 +267,45 @@ def _slices(ptr, idx, ncols, S):
             bslice[:nb])
 
 
+@pytest.mark.parametrize("chunk", [4, 8])
+@pytest.mark.parametrize("which", ["ragged-rows", "ragged-cols", "eq-rows", "eq-cols"])
+def test_small_ell_layout_places_every_entry_once(which, chunk):
+    # the small-LP runner's sliced-ELL copy: slots by decreasing row length
+    # (ties by row), slice widths from their longest row, every CSR position
+    # at its chunk position in storage order, padding -1; the kernel sums a
+    # slot's entries in that order, so this layout fixes its row sums
+    import ctypes as C
+    L = _core.lib()
+    lp = lpgen.gen_ragged(seed=5) if which.startswith("ragged") else lpgen.gen_random_eq(seed=0)
+    ptr = np.ascontiguousarray(lp.ap if which.endswith("rows") else lp.cp, np.int32)
+    rows = len(ptr) - 1
+    slices = (rows + 63) // 64
+    perm, off = np.zeros(rows, np.int32), np.zeros(slices + 1, np.int32)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))  # noqa
+    total = L.igaos_small_ell_layout(rows, ip(ptr), chunk, ip(perm), ip(off), None, 0)
+    src = np.full(total, 7, np.int32)
+    assert L.igaos_small_ell_layout(rows, ip(ptr), chunk, ip(perm), ip(off), ip(src),
+                                    total) == total
+    lens = np.diff(ptr)
+    assert np.array_equal(np.sort(perm), np.arange(rows))
+    key = [(-lens[r], r) for r in perm]
+    assert key == sorted(key)
+    nch = -(-lens[perm] // chunk)
+    for q in range(slices):
+        assert off[q + 1] - off[q] == nch[q * 64:(q + 1) * 64].max()
+    assert total == max(off[-1], 1) * 64 * chunk
+    want = np.full(total, -1, np.int64)
+    for s_, r in enumerate(perm):
+        e = np.arange(lens[r])
+        pos = ((off[s_ // 64] + e // chunk) * 64 + s_ % 64) * chunk + e % chunk
+        want[pos] = ptr[r] + e
+    assert np.array_equal(src, want)
+    assert np.array_equal(np.sort(src[src >= 0]), np.arange(ptr[-1]))
+    bad = np.array([0, 5, 1], np.int32)
+    assert L.igaos_small_ell_layout(2, ip(bad), chunk, ip(perm), ip(off), None, 0) < 0
+    assert L.igaos_small_ell_layout(rows, ip(ptr), 0, ip(perm), ip(off), None, 0) < 0
+
+
 @pytest.mark.parametrize("S", [1, 3, 8])
 def test_gather_ordered_slices_restore_row_order(S):
     # the gather order of A's slices (renumbered heavy columns): within
