@@ -1,34 +1,37 @@
 // One-workgroup runner of the default mode for small LPs (gfx950).
 //
-// An LP with max(n, m) <= 3 * 1024 whose two matrices' products, x and y
-// fit in LDS runs a whole check period of line-search attempts in one
-// launch of one 1024-thread workgroup, with the attempt of the three-kernel
-// path (pdhg_kernels.hip: k_primal, k_dual, k_decide) restated so that
-// almost nothing but the matrices leaves the CU:
-//   * thread t owns columns t, t + 1024, ... and rows t, t + 1024, ...; its
-//     columns' x, average and A'y and its rows' y, average, A x commit and
-//     A dx live in registers for the whole launch (the per-epoch constants
-//     are read from L2 where they are used);
-//   * xbar and y, the vectors the products gather, live in LDS; each
-//     product streams a sliced-ELL copy of its matrix (A' for A'y, A for
-//     A xbar) from L2: the entries of the 64 rows of a slice sit
-//     entry-major, so the wavefront that owns the slice loads one
-//     contiguous run per entry, and every thread sums its own rows, in
-//     storage order, straight from the loads and LDS gathers (no staged
-//     products, no barrier between a product and its step);
+// An LP with max(n, m) <= 3 * 1024 whose x, y and bounds fit in LDS (and
+// whose matrix is at most SMALL_MAX_BYTES per product) runs a whole check
+// period of line-search attempts in one launch of one 1024-thread
+// workgroup, with the attempt of the three-kernel path (pdhg_kernels.hip:
+// k_primal, k_dual, k_decide) restated so that almost nothing but the
+// matrices leaves the CU:
+//   * columns and rows sit in slots in order of decreasing length, and
+//     thread t owns the column slots t, t + 1024, ... (KC of them) and the
+//     row slots likewise (KR); their x, average and A'y, and y, average,
+//     A x commit and A dx, live in registers for the whole launch;
+//   * xbar and y, the vectors the products gather, and the per-launch
+//     constants (shifted gradient, bounds) live in LDS;
+//   * each product streams a sliced-ELL copy of its matrix (A' for A'y, A
+//     for A xbar; small.hpp) from L2: a slice's 64 slots hold their
+//     entries in chunk-major order, so the wavefront that owns the slice
+//     loads one contiguous run per chunk, and every thread sums its own
+//     rows, in storage order, straight from the loads and LDS gathers (no
+//     staged products, no barrier between a product and its step);
 //   * the three line-search dots are reduced once per attempt in a fixed
 //     order and thread 0 applies the decision rule to the state, which
-//     lives in LDS, so no global memory traffic is needed between
-//     attempts.
+//     lives in LDS, while every wavefront already sums the next attempt's
+//     A'y (used if the attempt is accepted), so no global memory traffic
+//     but the matrices is needed between attempts.
 // On exit the state and the vectors go back to the engine's buffers as the
 // three kernels leave them, with the current iterate always in x[0], y[0]
 // (the state's cur set to 0; nothing reads the other slot before the next
-// attempt writes it) and the pending averaging flagged in the state, so the
-// host's check code is the same for both paths and a check's device work
-// can be captured once with cur = 0.  Summation orders: every sparse row in storage
-// order, dots per thread then wavefront trees then the 16 wavefronts in
-// order; the results agree with the oracle to rounding like the
-// three-kernel path's.
+// attempt writes it) and the pending averaging flagged in the state, so
+// the host's check code is the same for both paths and a check's device
+// work can be captured once with cur = 0.  Summation orders: every sparse
+// row in storage order, dots per thread then wavefront butterflies then the
+// 16 wavefronts in order; the results agree with the oracle to rounding
+// like the three-kernel path's.
 
 #include <algorithm>
 
@@ -113,16 +116,16 @@ struct LaneVec {
 };
 
 // out[k] = sum over the entries of the row in owned slot tid + k ST of
-// val * x[idx], in
-// storage order, the product rounded before the add (a separate multiply
-// and add, like the three-kernel path).  The row's entries come in chunks
-// of small_chunk<V>() (one 32-byte value load and one 16- or 8-byte index
-// load per chunk): cb[k] is the chunk row where the row's slice starts (the same
-// for the whole wavefront), nch[k] the row's chunks.  A chunk's padding
-// entries are 0 with index 0, and add +-0 (x is finite), which leaves a sum
-// that started at +0 unchanged bit for bit.  All K rows' chunks are loaded
-// together; a lane past its row's last chunk re-loads it (or chunk 0, which
-// always exists) and adds it times 0, so neither loads nor gathers branch.
+// val * x[idx], in storage order, the product rounded before the add (a
+// separate multiply and add, like the three-kernel path).  The row's
+// entries come in chunks of small_chunk<V>() (one 32-byte value load and
+// one 8- or 16-byte index load per chunk): cb[k] is the chunk row where
+// the row's slice starts (the same for the whole wavefront), nch[k] the
+// row's chunks.  A chunk's padding entries are 0 with index 0, and add +-0
+// (x is finite), which leaves a sum that started at +0 unchanged bit for
+// bit.  All K rows' chunks are loaded together; a lane past its row's last
+// chunk re-loads it (or chunk 0, which always exists) and adds it times 0,
+// so neither loads nor gathers branch.
 template <typename V, int K>
 __device__ __forceinline__ void ell_sums(const SmallEll<V>& E, const int* cb, const int* nch,
                                          const V* x, V* out) {
